@@ -1,0 +1,93 @@
+/* distfind C ABI of the gfx950 engine — the symbols the reference's ctypes layer binds
+ * (python/qubitserf/distfind/_native.py:39-107), with the reference's signatures and
+ * struct layouts (cpp/distfind/capi.h:12-111).
+ *
+ * What this engine answers: method "bz" with backend "gpu" or "auto", on the device.
+ * Return codes: 0 ok; 1 null pointer (capi.cpp:35, :71); 2 odd column count for a
+ * symplectic matrix (capi.cpp:171); 3 "cc" on a non-CSS code (capi.cpp:214); and, new
+ * here, 4 method not provided by this engine ("mitm"/"cc"; the operator-weight and
+ * subsystem entry points), 5 backend not provided ("cpu", or no gfx950 device), 6 device
+ * error (qsf_hip_last_error() / distfind_last_error() has the text). There is no silent
+ * CPU fallback: `DistfindResult.backend` is always "hip" on success.
+ */
+#ifndef QSA_DISTFIND_CAPI_H
+#define QSA_DISTFIND_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cpp/distfind/capi.h:12-19 */
+typedef struct {
+    int distance;     /* -1 when undefined (no logicals) */
+    int lower_bound;
+    int proven;
+    int levels;
+    double seconds;
+    char backend[16];
+} DistfindResult;
+
+/* cpp/distfind/capi.h:38-45 */
+typedef struct {
+    int z_weight;
+    int x_weight;
+    int proven;
+    double seconds;
+    char backend[16];
+} DistfindOpResult;
+
+/* which: 'Z', 'X' or 'M' (= min over both, interleaved). cpp/distfind/capi.h:24-29 */
+int distfind_css_distance(const uint8_t* Hx, int hx_rows, int hx_cols,
+                          const uint8_t* Hz, int hz_rows, int hz_cols,
+                          const char* method, char which, const char* backend,
+                          int threads, int verbose, DistfindResult* out);
+
+/* cpp/distfind/capi.h:32-36 */
+int distfind_classical_distance(const uint8_t* H, int rows, int cols,
+                                const char* method, const char* backend,
+                                int threads, int verbose, DistfindResult* out);
+
+/* S is rows x 2q in [z|x] order; CSS input is split and routed to the CSS solver, a
+ * non-CSS code runs BZ on the weight-doubling isometric image (cpp/distfind/capi.h:81-85). */
+int distfind_stabilizer_distance(const uint8_t* S, int rows, int cols,
+                                 const char* method, char which, const char* backend,
+                                 int threads, int verbose, DistfindResult* out);
+
+/* Exported so the reference's binding loads; these return 4 (cpp/distfind/capi.h:52-108). */
+int distfind_operator_weight(const uint8_t* Gx, int gx_rows, int gx_cols,
+                             const uint8_t* Gz, int gz_rows, int gz_cols,
+                             const uint8_t* z_op, const uint8_t* x_op, int n,
+                             const char* method, const char* backend,
+                             int threads, int verbose, DistfindOpResult* out);
+int distfind_subsystem_distance(const uint8_t* Gx, int gx_rows, int gx_cols,
+                                const uint8_t* Gz, int gz_rows, int gz_cols,
+                                const char* method, char which, const char* backend,
+                                int threads, int verbose, DistfindResult* out);
+int distfind_subsystem_stabilizer_distance(const uint8_t* G, int rows, int cols,
+                                           const char* method, char which, const char* backend,
+                                           int threads, int verbose, DistfindResult* out);
+int distfind_stabilizer_operator_weight(const uint8_t* G, int rows, int cols,
+                                        const uint8_t* op, int op_len,
+                                        const char* method, const char* backend,
+                                        int threads, int verbose, DistfindResult* out);
+
+/* 1 if usable: "gpu" / "auto" need a gfx950 device; "cpu" is 0 (cpp/distfind/capi.cpp:290-299). */
+int distfind_backend_available(const char* name);
+const char* distfind_version(void);
+
+/* Text for the last nonzero return code of any distfind_* call (new; no reference counterpart). */
+const char* distfind_last_error(void);
+
+/* Per-level trace of the last distfind_* solve on this thread: up to `cap` rows of
+ * {side ('S','Z','X'), d, active sets, total sets, K, inner, outer} and the codeword
+ * count and wall milliseconds of each level. Returns the number of levels recorded.
+ * New (the reference prints the same fields under DISTFIND_PROFILE, bz.cpp:147-155). */
+int distfind_last_trace(int32_t* ints7, uint64_t* codewords, double* ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,96 @@
+/* MI355X (gfx950) Brouwer–Zimmermann enumeration engine — plugin-level C ABI.
+ *
+ * These entry points replace the reference's GPU plugin, `qsf::Backend` as implemented by
+ * `CudaBackend` (cpp/qsf/backend.hpp:53-61; cpp/qsf/cuda/backend_cuda.cu:506-645). A
+ * host that already builds EnumPlans (the reference's bz.cpp / codeaut/bz.cpp drivers)
+ * calls them through a ~40-line `HipBackend : qsf::Backend` shim.
+ *
+ * Plain pointers and sizes only. Every call is synchronous with respect to the caller:
+ * it returns after the device work has finished. Calls from several host threads are
+ * serialised inside the library. Nothing here falls back to the CPU: when the device is
+ * missing or a HIP call fails the entry point returns a nonzero code and
+ * qsf_hip_last_error() says why.
+ */
+#ifndef QSF_HIP_H
+#define QSF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSF_WEIGHT_NONE (1 << 30) /* "no codeword" sentinel, cpp/qsf/backend.hpp:19 */
+
+/* One weight level of work. Field for field the layout of qsf::EnumPlan
+ * (cpp/qsf/backend.hpp:21-42), so a reference plan can be passed by address.
+ *   gamma : num_gamma_total x K x stride u64, row-major (generator, row, word); bit j of a
+ *           row is bit (j & 63) of word (j >> 6). Only the first num_gamma generators are
+ *           enumerated at this level.
+ *   check : kcheck x stride u64 logical-detector rows; kcheck == 0 => every codeword counts.
+ *   binom : accepted for layout compatibility and ignored (the engine keeps its own table).
+ *   buffers_key : unique per solve; device copies of gamma/check are cached under it and
+ *           reused across the levels of that solve (never keyed on host pointers). */
+typedef struct {
+    int n;
+    int stride;
+    int K;
+    int d;
+    int num_gamma;
+    int num_gamma_total;
+    const uint64_t* gamma;
+    int kcheck;
+    const uint64_t* check;
+    const uint64_t* binom;
+    int binom_maxN;
+    int binom_maxK;
+    int current_best;
+    uint64_t buffers_key;
+} qsf_enum_plan;
+
+/* 1 when a gfx950 device is present and the kernels are loadable (replaces
+ * CudaBackend::available, backend_cuda.cu:485-503). */
+int qsf_hip_available(void);
+
+/* Min mode (replaces CudaBackend::enumerate, backend_cuda.cu:506-557): *best_out =
+ * min(plan->current_best, lightest logical codeword that is the XOR of d rows of one of the
+ * first num_gamma generators). Exact; any value >= current_best means "none lighter".
+ * Returns 0 on success; 1 bad arguments; 2 no device; 3 HIP error; 4 level too large
+ * (more than 2^62 codewords). */
+int qsf_hip_enumerate(const qsf_enum_plan* plan, int* best_out);
+
+/* Like qsf_hip_enumerate, but the level may stop as soon as a logical of weight <= floor
+ * has been found: the caller knows no codeword of this solve is lighter than `floor`
+ * (the BZ lower bound after level d-1), so such a codeword is the answer. floor < 0
+ * disables it. The result is the same number qsf_hip_enumerate would return whenever that
+ * number is <= floor or no codeword <= floor exists. */
+int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out);
+
+/* Collect mode (replaces CudaBackend::collect, backend_cuda.cu:561-645): every codeword of
+ * Hamming weight in [1, keep_weight] that is the XOR of d rows of one of the first
+ * num_gamma generators, `stride` u64 words each, duplicates allowed. On success *hits is a
+ * buffer owned by the library (free with qsf_hip_free) holding *rows codewords. Complete
+ * or an error: device-buffer overflow is retried with a larger buffer, never truncated. */
+int qsf_hip_collect(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits, int64_t* rows);
+void qsf_hip_free(void* p);
+
+/* Statistics of the last enumerate/collect call on this thread's library instance:
+ * codewords examined (C(K,d) x num_gamma, the reference's `work`, bz.cpp:151-154), device
+ * time in milliseconds of the enumeration kernel, and which kernel ran
+ * (0 = none, 1 = row-sweep kernel, 2 = tiled split kernel). */
+typedef struct {
+    uint64_t codewords;
+    double kernel_ms;
+    int kernel;
+    int tiles;
+} qsf_hip_stats;
+void qsf_hip_last_stats(qsf_hip_stats* out);
+
+/* Human-readable reason for the last nonzero return code. */
+const char* qsf_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSF_HIP_H */

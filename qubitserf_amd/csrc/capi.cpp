@@ -1,0 +1,283 @@
+// distfind_* and codeaut_bz_* C ABI over the gfx950 engine (see include/distfind_capi.h,
+// include/codeaut_capi.h).
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/codeaut_capi.h"
+#include "../../include/distfind_capi.h"
+#include "../../include/qsf_hip.h"
+#include "driver.hpp"
+
+using namespace qsa;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local std::vector<LevelTrace> g_trace;
+
+constexpr int RC_OK = 0, RC_NULL = 1, RC_ODD = 2, RC_CC_NONCSS = 3, RC_METHOD = 4, RC_BACKEND = 5, RC_DEVICE = 6;
+
+void fill(DistfindResult* out, const BzResult& r) {
+    out->distance = r.distance;
+    out->lower_bound = r.lower_bound;
+    out->proven = r.proven ? 1 : 0;
+    out->levels = r.levels;
+    out->seconds = r.seconds;
+    std::memset(out->backend, 0, sizeof(out->backend));
+    std::strncpy(out->backend, "hip", sizeof(out->backend) - 1);
+}
+
+// Shared argument policy: method "bz" only; backend "gpu"/"auto" only, device present.
+int check_args(const char* method, const char* backend, bool noncss = false) {
+    std::string m = method ? method : "bz";
+    std::string b = backend ? backend : "auto";
+    if (m == "cc" && noncss) { g_err = "connected cluster has no non-CSS form"; return RC_CC_NONCSS; }
+    if (m != "bz") { g_err = "method '" + m + "' is not provided by the gfx950 engine (it runs 'bz')"; return RC_METHOD; }
+    if (b != "gpu" && b != "auto") { g_err = "backend '" + b + "' is not provided by the gfx950 engine (use 'gpu')"; return RC_BACKEND; }
+    if (!qsf_hip_available()) { g_err = std::string("no usable gfx950 device: ") + qsf_hip_last_error(); return RC_BACKEND; }
+    return RC_OK;
+}
+
+BzOptions options(int verbose) {
+    BzOptions o;
+    o.verbose = verbose != 0;
+    o.profile = std::getenv("DISTFIND_PROFILE") != nullptr;
+    g_trace.clear();
+    o.trace = &g_trace;
+    return o;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const EngineError& e) {
+        g_err = e.what;
+        return RC_DEVICE;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return RC_DEVICE;
+    }
+}
+
+int css_solve(const BitMat& Hx, const BitMat& Hz, char which, int verbose, DistfindResult* out) {
+    BzOptions o = options(verbose);
+    BzResult r;
+    if (which == 'Z' || which == 'X') r = bz_distance(css_side(Hx, Hz, which), o);
+    else r = bz_min_interleaved(css_side(Hx, Hz, 'Z'), css_side(Hx, Hz, 'X'), o);
+    fill(out, r);
+    return RC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int distfind_css_distance(const uint8_t* Hx, int hx_rows, int hx_cols, const uint8_t* Hz, int hz_rows,
+                          int hz_cols, const char* method, char which, const char* backend, int threads,
+                          int verbose, DistfindResult* out) {
+    (void)threads;
+    if (!out || !Hx || !Hz) { g_err = "null pointer"; return RC_NULL; }
+    if (int rc = check_args(method, backend)) return rc;
+    return guarded([&] {
+        return css_solve(BitMat::from_bytes(Hx, hx_rows, hx_cols), BitMat::from_bytes(Hz, hz_rows, hz_cols), which,
+                         verbose, out);
+    });
+}
+
+int distfind_classical_distance(const uint8_t* H, int rows, int cols, const char* method, const char* backend,
+                                int threads, int verbose, DistfindResult* out) {
+    (void)threads;
+    if (!out || !H) { g_err = "null pointer"; return RC_NULL; }
+    if (int rc = check_args(method, backend)) return rc;
+    return guarded([&] {
+        BzOptions o = options(verbose);
+        fill(out, bz_distance(classical(BitMat::from_bytes(H, rows, cols)), o));
+        return RC_OK;
+    });
+}
+
+int distfind_stabilizer_distance(const uint8_t* S, int rows, int cols, const char* method, char which,
+                                 const char* backend, int threads, int verbose, DistfindResult* out) {
+    (void)threads;
+    if (!out || !S) { g_err = "null pointer"; return RC_NULL; }
+    if (cols % 2 != 0) { g_err = "symplectic matrix needs an even column count"; return RC_ODD; }
+    BitMat m = BitMat::from_bytes(S, rows, cols);
+    const bool css = is_css_symplectic(m);
+    if (int rc = check_args(method, backend, !css)) return rc;
+    return guarded([&] {
+        if (css) {
+            BitMat Hx, Hz;
+            split_css(m, Hx, Hz);
+            return css_solve(Hx, Hz, which, verbose, out);
+        }
+        BzOptions o = options(verbose);
+        if (o.verbose)
+            std::fprintf(stderr, "distfind: non-CSS BZ via the (a|b)->(a|b|a^b) isometry; bounds are Hamming "
+                                 "weights of the length-%d code (twice the symplectic weight).\n", 3 * (cols / 2));
+        BzResult r = bz_distance(symplectic_isometric(m), o);
+        if (r.distance > 0) r.distance /= 2;       // Hamming weight = 2 x symplectic weight
+        if (r.lower_bound > 0) r.lower_bound /= 2;
+        fill(out, r);
+        return RC_OK;
+    });
+}
+
+int distfind_operator_weight(const uint8_t*, int, int, const uint8_t*, int, int, const uint8_t*, const uint8_t*, int,
+                             const char*, const char*, int, int, DistfindOpResult* out) {
+    if (out) std::memset(out, 0, sizeof(*out));
+    g_err = "operator weight is not provided by the gfx950 engine";
+    return RC_METHOD;
+}
+
+int distfind_subsystem_distance(const uint8_t*, int, int, const uint8_t*, int, int, const char*, char, const char*,
+                                int, int, DistfindResult* out) {
+    if (out) std::memset(out, 0, sizeof(*out));
+    g_err = "subsystem distance is not provided by the gfx950 engine";
+    return RC_METHOD;
+}
+
+int distfind_subsystem_stabilizer_distance(const uint8_t*, int, int, const char*, char, const char*, int, int,
+                                           DistfindResult* out) {
+    if (out) std::memset(out, 0, sizeof(*out));
+    g_err = "subsystem distance is not provided by the gfx950 engine";
+    return RC_METHOD;
+}
+
+int distfind_stabilizer_operator_weight(const uint8_t*, int, int, const uint8_t*, int, const char*, const char*, int,
+                                        int, DistfindResult* out) {
+    if (out) std::memset(out, 0, sizeof(*out));
+    g_err = "operator weight is not provided by the gfx950 engine";
+    return RC_METHOD;
+}
+
+int distfind_backend_available(const char* name) {
+    std::string b = name ? name : "auto";
+    if (b == "gpu" || b == "auto") return qsf_hip_available();
+    return 0;
+}
+
+const char* distfind_version(void) { return "0.1.0+gfx950"; }
+
+const char* distfind_last_error(void) { return g_err.c_str(); }
+
+int distfind_last_trace(int32_t* ints7, uint64_t* codewords, double* ms, int cap) {
+    const int n = (int)g_trace.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        const LevelTrace& t = g_trace[i];
+        if (ints7) {
+            int32_t* o = ints7 + 7 * i;
+            o[0] = t.side; o[1] = t.d; o[2] = t.active; o[3] = t.ngamma; o[4] = t.K; o[5] = t.inner; o[6] = t.outer;
+        }
+        if (codewords) codewords[i] = t.codewords;
+        if (ms) ms[i] = t.ms;
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------- codeaut collect
+
+namespace {
+struct CollectHandle {
+    int n = 0, stride = 0;
+    bool ok = false;
+    int64_t combos = 0, count = 0;
+    std::vector<u64> rows;
+    std::string backend = "hip";
+};
+constexpr int64_t kComboCap = (int64_t)1 << 62;
+}  // namespace
+
+void* codeaut_bz_collect(const uint8_t* G, int32_t m, int32_t n, int32_t p, int32_t keep_weight, int32_t backend,
+                         int32_t threads) {
+    (void)threads;
+    auto* H = new CollectHandle();
+    H->n = n;
+    H->stride = words(n);
+    if (m <= 0 || n <= 0 || p <= 0) { H->ok = true; return H; }
+    if (backend == 0 || !qsf_hip_available()) {
+        g_err = backend == 0 ? "cpu collect is not provided by the gfx950 engine" : "no usable gfx950 device";
+        H->ok = false;
+        return H;
+    }
+    const int st = H->stride;
+    BitMat Gm = BitMat::from_bytes(G, m, n);
+    const int maxsw = std::min<int>(p, m);
+    static std::atomic<u64> keys{0xc011ec7000000000ull};
+    const u64 key = ++keys;
+    std::vector<u64> hits;
+    try {
+        for (int sw = 1; sw <= maxsw; ++sw) {
+            qsf_enum_plan pl{};
+            pl.n = n; pl.stride = st; pl.K = m; pl.d = sw;
+            pl.num_gamma = 1; pl.num_gamma_total = 1;
+            pl.gamma = Gm.data.data(); pl.kcheck = 0; pl.check = nullptr;
+            pl.current_best = kNoWeight; pl.buffers_key = key;
+            uint64_t* buf = nullptr;
+            int64_t rows = 0;
+            int rc = qsf_hip_collect(&pl, keep_weight, &buf, &rows);
+            if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+            hits.insert(hits.end(), buf, buf + rows * st);
+            qsf_hip_free(buf);
+            // combination count, saturating like the reference's int64 counter (codeaut/bz.cpp:88-95)
+            long double c = 1;
+            for (int i = 0; i < sw; ++i) c = c * (m - i) / (i + 1);
+            int64_t total = c > (long double)kComboCap ? kComboCap : (int64_t)(c + 0.5L);
+            H->combos = H->combos > kComboCap - total ? kComboCap : H->combos + total;
+        }
+    } catch (const EngineError& e) {
+        g_err = e.what;
+        H->ok = false;
+        return H;
+    }
+    // sort and deduplicate (lexicographic over the packed words, cpp/codeaut/bz.cpp:98-113)
+    const int64_t nrows = (int64_t)hits.size() / st;
+    std::vector<int64_t> idx(nrows);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+        return std::lexicographical_compare(hits.begin() + a * st, hits.begin() + a * st + st, hits.begin() + b * st,
+                                            hits.begin() + b * st + st);
+    });
+    for (int64_t i = 0; i < nrows; ++i) {
+        const u64* a = hits.data() + idx[i] * st;
+        if (i > 0 && std::memcmp(a, hits.data() + idx[i - 1] * st, (size_t)st * 8) == 0) continue;
+        H->rows.insert(H->rows.end(), a, a + st);
+    }
+    H->count = (int64_t)H->rows.size() / st;
+    H->ok = true;
+    return H;
+}
+
+int32_t codeaut_bz_ok(void* h) { return h && ((CollectHandle*)h)->ok ? 1 : 0; }
+int64_t codeaut_bz_count(void* h) { return h ? ((CollectHandle*)h)->count : -1; }
+int64_t codeaut_bz_combos(void* h) { return h ? ((CollectHandle*)h)->combos : -1; }
+
+void codeaut_bz_copy_rows(void* h, uint8_t* buf) {
+    auto* H = (CollectHandle*)h;
+    if (!H || !buf) return;
+    for (int64_t r = 0; r < H->count; ++r) {
+        const u64* src = H->rows.data() + r * H->stride;
+        uint8_t* dst = buf + (size_t)r * H->n;
+        for (int j = 0; j < H->n; ++j) dst[j] = (uint8_t)((src[j >> 6] >> (j & 63)) & 1);
+    }
+}
+
+const char* codeaut_bz_backend(void* h) { return h ? ((CollectHandle*)h)->backend.c_str() : ""; }
+void codeaut_bz_free(void* h) { delete (CollectHandle*)h; }
+
+int32_t codeaut_backend_available(const char* name) {
+    if (!name) return 0;
+    std::string b = name;
+    if (b == "gpu" || b == "auto") return qsf_hip_available();
+    return 0;
+}
+
+const char* codeaut_version(void) { return "0.1.0+gfx950"; }
+
+}  // extern "C"

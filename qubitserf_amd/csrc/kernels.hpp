@@ -1,0 +1,82 @@
+// Device-side definitions shared by the enumeration kernels and their host launcher.
+//
+// The engine answers one question per weight level d (the reference's Backend::enumerate,
+// cpp/qsf/backend.hpp:57-58): the lightest logical codeword that is the XOR of d rows of
+// one of the active generators. Two kernels do it.
+//
+//  * bz_tiled<FD, RPL>  — the hot kernel. The K rows are cut into A = [0, h) and
+//    B = [h, K). A d-subset is an a-subset of A joined with a (d-a)-subset of B, so level d
+//    is the union over a of the complete products  subsets(A, a) x subsets(B, d-a).
+//    Each product is tiled like a GEMM: 64 lanes x RPL register-resident codewords of the
+//    larger list, times a chunk of S codewords of the smaller list streamed from LDS with
+//    a broadcast read. One candidate costs FD x (v_xor_b32 + v_bcnt_u32_b32) plus half a
+//    v_min3: no loads, no divergence, no per-candidate unranking.
+//  * bz_rowsweep — the plain kernel: every thread owns a run of lexicographic combinations
+//    and XORs full rows (the reference's two-level sweep, enum_core.hpp:109-194). Used for
+//    small levels, K > 128, and collect mode.
+//
+// Codeword representation in the tiled kernel ("residual form"). For generator g, a
+// column that holds a single 1, at row i, is a unit column of row i; one unit column per
+// row is set aside (row i is then "flagged"). A codeword's weight on the set-aside columns
+// is exactly the number of flagged rows XORed into it, so
+//     weight = popcount(XOR of residual parts) + #flagged rows chosen,
+// where the residual part of a row is the row without the set-aside columns, packed into
+// RD dwords. For a full-rank systematic generator every row is flagged and the residual
+// is the n-K redundancy bits. Logical test: the detector rows enter only through the
+// per-row syndrome (bit c = <check_c, row>), XORed like the residual (SD dwords).
+#pragma once
+#include <cstdint>
+
+namespace qsa {
+
+using u64 = std::uint64_t;
+using u32 = std::uint32_t;
+
+constexpr int kMaxTiledK = 128;     // both halves must fit a 64-bit subset mask
+constexpr int kMaxD = 64;           // binomial table columns
+constexpr int kStreamMax = 512;     // LDS stream chunk capacity (elements)
+constexpr int kMaxRW = 64;          // residual + syndrome dwords per row in the tiled kernel
+constexpr int kBigOffset = (1 << 30) + (1 << 29);  // masks lanes past the end of a list
+
+// One complete product subsets(A, ka) x subsets(B, kb) of one generator.
+struct Rect {
+    int g;
+    int ka, kb;
+    int lanes_on_a;        // 1: the register tile holds subsets of A, the stream subsets of B
+    u64 n_lane, n_stream;  // list lengths on each side
+    u64 lane_blocks;       // ceil(n_lane / (64 * RPL))
+    u64 stream_chunks;     // ceil(n_stream / S)
+    u64 tile_base;         // index of this rectangle's first tile
+};
+
+struct TiledArgs {
+    const u32* tbl;          // [g][row][RW]: RD residual dwords, then SD syndrome dwords
+    const u64* flags;        // [g][2]: flagged rows of A (bit i = row i) and of B (bit i = row h+i)
+    const Rect* rects;
+    int nrects;
+    u64 total_tiles;
+    const u64* binom;        // C(n, k) at binom[n * bk + k], n <= K, k < bk, saturating
+    int bk;
+    int K, h, RD, SD, RW;
+    int S;                   // stream chunk length (<= kStreamMax)
+    int* best;               // running bound; starts at current_best
+    unsigned long long* next_tile;
+    int floor;               // stop once best <= floor (floor < 0: never)
+};
+
+struct SweepArgs {
+    const u64* gamma;        // [g][row][stride] full rows
+    const u64* check;        // [c][stride]
+    int kcheck, stride, K, d, ngamma;
+    const u64* binom;
+    int bk;
+    u64 total;               // C(K, d)
+    u64 chunk;               // combinations per thread
+    int* best;               // min mode
+    int collect, keep;       // collect mode: weight in [1, keep]
+    unsigned long long* out_rows;
+    u64* out;
+    u64 out_cap;             // rows
+};
+
+}  // namespace qsa

@@ -1,0 +1,440 @@
+#include "problem.hpp"
+
+#include <algorithm>
+#include <functional>
+#include <numeric>
+#include <thread>
+
+namespace qsa {
+
+// ------------------------------------------------------------------ GF(2) elimination
+
+std::vector<int> eliminate(BitMat& m, const int* order, int count) {
+    std::vector<int> piv;
+    int rank = 0;
+    for (int t = 0; t < count && rank < m.rows; ++t) {
+        const int c = order[t];
+        const int w = c >> 6;
+        const u64 bitm = 1ull << (c & 63);
+        int sel = -1;
+        for (int r = rank; r < m.rows; ++r)
+            if (m.row(r)[w] & bitm) { sel = r; break; }
+        if (sel < 0) continue;
+        m.swap(rank, sel);
+        const u64* p = m.row(rank);
+        for (int r = 0; r < m.rows; ++r)
+            if (r != rank && (m.row(r)[w] & bitm)) m.xor_into(r, p);
+        piv.push_back(c);
+        ++rank;
+    }
+    return piv;
+}
+
+std::vector<int> eliminate_all(BitMat& m) {
+    std::vector<int> order(m.cols);
+    std::iota(order.begin(), order.end(), 0);
+    return eliminate(m, order.data(), m.cols);
+}
+
+BitMat kernel_basis(const BitMat& in) {
+    BitMat m = in;
+    std::vector<int> piv = eliminate_all(m);
+    std::vector<char> is_piv(m.cols, 0);
+    for (int p : piv) is_piv[p] = 1;
+    BitMat ker(m.cols - (int)piv.size(), m.cols);
+    int out = 0;
+    for (int f = 0; f < m.cols; ++f) {
+        if (is_piv[f]) continue;
+        ker.set(out, f);
+        for (size_t r = 0; r < piv.size(); ++r)
+            if (m.bit((int)r, f)) ker.set(out, piv[r]);
+        ++out;
+    }
+    return ker;
+}
+
+namespace {
+
+// Reducing basis kept in insertion order, each vector tagged with its lowest set bit.
+struct Reducer {
+    int ld;
+    std::vector<std::vector<u64>> vecs;
+    std::vector<int> low;
+    explicit Reducer(int cols) : ld(words(cols)) {}
+    bool reduce(std::vector<u64>& v) const {
+        for (size_t i = 0; i < vecs.size(); ++i)
+            if ((v[low[i] >> 6] >> (low[i] & 63)) & 1)
+                for (int k = 0; k < ld; ++k) v[k] ^= vecs[i][k];
+        for (u64 x : v) if (x) return true;
+        return false;
+    }
+    void add(const std::vector<u64>& v) {
+        int l = -1;
+        for (int k = 0; k < ld && l < 0; ++k) if (v[k]) l = k * 64 + __builtin_ctzll(v[k]);
+        vecs.push_back(v);
+        low.push_back(l);
+    }
+};
+
+bool every_row_even(const BitMat& m) {
+    for (int i = 0; i < m.rows; ++i) if (m.weight(i) & 1) return false;
+    return true;
+}
+
+int lightest_row(const BitMat& m) {
+    int b = kNoWeight;
+    for (int i = 0; i < m.rows; ++i) b = std::min(b, m.weight(i));
+    return b;
+}
+
+}  // namespace
+
+BitMat quotient(const BitMat& self, const BitMat& other) {
+    BitMat ko = kernel_basis(other);
+    Reducer red(self.cols);
+    for (int i = 0; i < self.rows; ++i) {
+        std::vector<u64> v(self.row(i), self.row(i) + self.ld);
+        if (red.reduce(v)) red.add(v);
+    }
+    BitMat out(0, self.cols);
+    for (int i = 0; i < ko.rows; ++i) {
+        std::vector<u64> v(ko.row(i), ko.row(i) + ko.ld);
+        if (!red.reduce(v)) continue;
+        red.add(v);
+        out.append(ko.row(i));
+    }
+    return out;
+}
+
+bool in_rowspan(const BitMat& G, const u64* v) {
+    BitMat R = G;
+    std::vector<int> piv = eliminate_all(R);
+    std::vector<u64> t(v, v + R.ld);
+    for (size_t r = 0; r < piv.size(); ++r)
+        if ((t[piv[r] >> 6] >> (piv[r] & 63)) & 1)
+            for (int k = 0; k < R.ld; ++k) t[k] ^= R.row((int)r)[k];
+    for (u64 x : t) if (x) return false;
+    return true;
+}
+
+// ------------------------------------------------------------------ problem builders
+
+Problem css_side(const BitMat& Hx, const BitMat& Hz, char side) {
+    const BitMat& self = side == 'X' ? Hz : Hx;
+    const BitMat& other = side == 'X' ? Hx : Hz;
+    Problem p;
+    p.n = self.cols;
+    p.gen = kernel_basis(self);
+    p.detector = quotient(self, other);
+    BitMat logicals = quotient(other, self);
+    p.even = every_row_even(p.gen);
+    p.seed_upper = logicals.rows ? lightest_row(logicals) : 0;
+    return p;
+}
+
+Problem classical(const BitMat& H) {
+    Problem p;
+    p.n = H.cols;
+    p.gen = kernel_basis(H);
+    p.detector = BitMat(0, H.cols);
+    p.every_nonzero = true;
+    p.even = every_row_even(p.gen);
+    p.seed_upper = p.gen.rows ? lightest_row(p.gen) : 0;
+    return p;
+}
+
+static BitMat swap_halves(const BitMat& m) {
+    const int q = m.cols / 2;
+    BitMat o(m.rows, m.cols);
+    for (int i = 0; i < m.rows; ++i)
+        for (int j = 0; j < q; ++j) {
+            if (m.bit(i, j)) o.set(i, q + j);
+            if (m.bit(i, q + j)) o.set(i, j);
+        }
+    return o;
+}
+
+bool is_css_symplectic(const BitMat& S) {
+    const int q = S.cols / 2;
+    for (int i = 0; i < S.rows; ++i) {
+        bool z = false, x = false;
+        for (int j = 0; j < q; ++j) { z = z || S.bit(i, j); x = x || S.bit(i, q + j); }
+        if (z && x) return false;
+    }
+    return true;
+}
+
+void split_css(const BitMat& S, BitMat& Hx, BitMat& Hz) {
+    const int q = S.cols / 2;
+    Hx = BitMat(0, q);
+    Hz = BitMat(0, q);
+    BitMat zr(1, q), xr(1, q);
+    for (int i = 0; i < S.rows; ++i) {
+        std::fill(zr.data.begin(), zr.data.end(), 0);
+        std::fill(xr.data.begin(), xr.data.end(), 0);
+        bool z = false, x = false;
+        for (int j = 0; j < q; ++j) {
+            if (S.bit(i, j)) { zr.set(0, j); z = true; }
+            if (S.bit(i, q + j)) { xr.set(0, j); x = true; }
+        }
+        if (x) Hx.append(xr.row(0));
+        else if (z) Hz.append(zr.row(0));
+    }
+}
+
+Problem symplectic_isometric(const BitMat& S) {
+    const int q = S.cols / 2;
+    BitMat sw = swap_halves(S);
+    BitMat gen = kernel_basis(sw);                 // centralizer of S
+    BitMat det = swap_halves(quotient(S, sw));     // pre-swapped so <det, c> is a dot
+    int best = 0;
+    for (int i = 0; i < gen.rows; ++i) {
+        if (in_rowspan(S, gen.row(i))) continue;   // a stabilizer, not a logical
+        int w = 0;
+        for (int j = 0; j < q; ++j) w += (gen.bit(i, j) || gen.bit(i, q + j)) ? 1 : 0;
+        if (best == 0 || w < best) best = w;
+    }
+    Problem e;
+    e.n = 3 * q;
+    e.even = true;                  // every image (a|b|a^b) has even weight
+    e.seed_upper = best > 0 ? 2 * best : 0;
+    e.gen = BitMat(gen.rows, e.n);
+    for (int i = 0; i < gen.rows; ++i)
+        for (int j = 0; j < q; ++j) {
+            bool a = gen.bit(i, j), b = gen.bit(i, q + j);
+            if (a) e.gen.set(i, j);
+            if (b) e.gen.set(i, q + j);
+            if (a != b) e.gen.set(i, 2 * q + j);
+        }
+    e.detector = BitMat(det.rows, e.n);
+    for (int i = 0; i < det.rows; ++i)
+        for (int j = 0; j < 2 * q; ++j)
+            if (det.bit(i, j)) e.detector.set(i, j);
+    return e;
+}
+
+// ------------------------------------------------------------------ information sets
+
+static InfoSet eliminate_set(const BitMat& gen, const std::vector<int>& cols) {
+    InfoSet s;
+    s.gen = gen;
+    s.pivots = eliminate(s.gen, cols.data(), (int)cols.size());
+    s.rank = (int)s.pivots.size();
+    return s;
+}
+
+static std::vector<InfoSet> greedy_sets(const BitMat& gen) {
+    std::vector<InfoSet> out;
+    const int n = gen.cols;
+    std::vector<int> live(n);
+    std::iota(live.begin(), live.end(), 0);
+    BitMat work = gen;
+    while (!live.empty()) {
+        std::vector<int> piv = eliminate(work, live.data(), (int)live.size());
+        InfoSet s;
+        s.gen = work;
+        s.rank = (int)piv.size();
+        s.pivots = piv;
+        out.push_back(s);
+        std::vector<char> used(n, 0), nonzero(n, 0);
+        for (int c : piv) used[c] = 1;
+        for (int r = 0; r < work.rows; ++r)
+            for (int c : live) if (work.bit(r, c)) nonzero[c] = 1;
+        std::vector<int> next;
+        for (int c : live) if (!used[c] && nonzero[c]) next.push_back(c);
+        if (next.size() == live.size()) break;
+        live.swap(next);
+    }
+    return out;
+}
+
+namespace {
+
+// Edmonds matroid partition of the column matroid of gen (columns as K-bit vectors),
+// by augmenting-path search: a column joins the first class it is independent of; if it
+// is dependent on every class, it displaces a member of a circuit, which is then placed
+// recursively (never back into the class it left). The span test and circuit bookkeeping
+// follow cpp/distfind/css.cpp:231-268 step for step, so the classes (and hence the
+// information sets and the BZ lower bounds) match the reference's.
+class Partitioner {
+  public:
+    explicit Partitioner(const BitMat& gen) : n_(gen.cols), kw_(words(gen.rows)), col_((size_t)gen.cols * words(gen.rows), 0) {
+        for (int j = 0; j < n_; ++j) {
+            bool any = false;
+            for (int i = 0; i < gen.rows; ++i)
+                if (gen.bit(i, j)) { col_[(size_t)j * kw_ + (i >> 6)] |= 1ull << (i & 63); any = true; }
+            (any ? nonzero_ : zero_).push_back(j);
+        }
+    }
+
+    bool run(std::vector<std::vector<int>>& out) {
+        seen_.assign(n_, 0);
+        for (int z : nonzero_) {
+            std::fill(seen_.begin(), seen_.end(), 0);
+            if (!place(z, -1)) classes_.push_back({z});
+        }
+        if (classes_.empty()) classes_.push_back({});
+        for (int j : zero_) classes_[0].push_back(j);
+        std::vector<char> hit(n_, 0);
+        int total = 0;
+        for (auto& c : classes_)
+            for (int j : c) { if (hit[j]) return false; hit[j] = 1; ++total; }
+        if (total != n_) return false;
+        out = classes_;
+        return true;
+    }
+
+  private:
+    static std::vector<int> symdiff(const std::vector<int>& a, const std::vector<int>& b) {
+        std::vector<int> r;
+        std::set_symmetric_difference(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(r));
+        return r;
+    }
+    const u64* colv(int j) const { return col_.data() + (size_t)j * kw_; }
+    int lowbit(const std::vector<u64>& v) const {
+        for (int k = 0; k < kw_; ++k) if (v[k]) return k * 64 + __builtin_ctzll(v[k]);
+        return -1;
+    }
+    // true iff column z lies in the span of class `cls`; then *circuit = class columns
+    // whose XOR is z.
+    bool spans(const std::vector<int>& cls, int z, std::vector<int>* circuit) const {
+        std::vector<std::vector<u64>> bv;
+        std::vector<int> bl;
+        std::vector<std::vector<int>> bo;
+        for (int idx : cls) {
+            std::vector<u64> v(colv(idx), colv(idx) + kw_);
+            std::vector<int> o{idx};
+            for (size_t b = 0; b < bv.size(); ++b)
+                if ((v[bl[b] >> 6] >> (bl[b] & 63)) & 1) {
+                    for (int k = 0; k < kw_; ++k) v[k] ^= bv[b][k];
+                    o = symdiff(o, bo[b]);
+                }
+            int l = lowbit(v);
+            if (l >= 0) { bv.push_back(std::move(v)); bl.push_back(l); bo.push_back(std::move(o)); }
+        }
+        std::vector<u64> v(colv(z), colv(z) + kw_);
+        std::vector<int> o;
+        for (size_t b = 0; b < bv.size(); ++b)
+            if ((v[bl[b] >> 6] >> (bl[b] & 63)) & 1) {
+                for (int k = 0; k < kw_; ++k) v[k] ^= bv[b][k];
+                o = symdiff(o, bo[b]);
+            }
+        for (u64 x : v) if (x) return false;
+        if (circuit) *circuit = std::move(o);
+        return true;
+    }
+    bool place(int z, int forbidden) {
+        for (int c = 0; c < (int)classes_.size(); ++c)
+            if (c != forbidden && !spans(classes_[c], z, nullptr)) { classes_[c].push_back(z); return true; }
+        for (int c = 0; c < (int)classes_.size(); ++c) {
+            if (c == forbidden) continue;
+            std::vector<int> circ;
+            if (!spans(classes_[c], z, &circ)) continue;
+            for (int y : circ) {
+                if (seen_[y]) continue;
+                seen_[y] = 1;
+                auto& cls = classes_[c];
+                cls.erase(std::find(cls.begin(), cls.end(), y));
+                cls.push_back(z);
+                if (place(y, c)) return true;
+                cls.erase(std::find(cls.begin(), cls.end(), z));
+                cls.push_back(y);
+            }
+        }
+        return false;
+    }
+
+    int n_, kw_;
+    std::vector<u64> col_;
+    std::vector<int> nonzero_, zero_;
+    std::vector<std::vector<int>> classes_;
+    std::vector<char> seen_;
+};
+
+int count_full(const std::vector<InfoSet>& s, int K) {
+    int f = 0;
+    for (auto& x : s) f += (x.rank == K);
+    return f;
+}
+
+void sort_by_rank(std::vector<InfoSet>& s) {
+    std::stable_sort(s.begin(), s.end(), [](const InfoSet& a, const InfoSet& b) { return a.rank > b.rank; });
+}
+
+}  // namespace
+
+std::vector<InfoSet> information_sets_from_parts(const BitMat& gen, const std::vector<std::vector<int>>& parts) {
+    std::vector<InfoSet> out;
+    for (auto& cols : parts)
+        if (!cols.empty()) out.push_back(eliminate_set(gen, cols));
+    sort_by_rank(out);
+    return out;
+}
+
+std::vector<InfoSet> information_sets(const BitMat& gen) {
+    const int K = gen.rows, n = gen.cols;
+    std::vector<InfoSet> greedy = greedy_sets(gen);
+    if (K == 0) { sort_by_rank(greedy); return greedy; }
+    int nonzero_cols = 0;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < K; ++i)
+            if (gen.bit(i, j)) { ++nonzero_cols; break; }
+    const int gfull = count_full(greedy, K);
+    if (gfull >= nonzero_cols / K || n > 200) { sort_by_rank(greedy); return greedy; }
+    std::vector<std::vector<int>> parts;
+    Partitioner P(gen);
+    if (!P.run(parts)) { sort_by_rank(greedy); return greedy; }
+    // plain (unstable) sort by size, as the reference orders its classes (css.cpp:354-355)
+    std::sort(parts.begin(), parts.end(),
+              [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
+    std::vector<InfoSet> seq;
+    for (auto& cols : parts)
+        if (!cols.empty()) seq.push_back(eliminate_set(gen, cols));
+    if (seq.empty() || count_full(seq, K) <= gfull) { sort_by_rank(greedy); return greedy; }
+    sort_by_rank(seq);
+    return seq;
+}
+
+// ------------------------------------------------------------------ random IS bound
+
+namespace {
+inline u64 mix64(u64& s) {   // splitmix64 step
+    u64 z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+int random_is_bound(const Problem& p, int init, int trials, u64 seed) {
+    const int K = p.gen.rows, n = p.n;
+    if (K == 0 || trials <= 0) return init;
+    int workers = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    workers = std::min(workers, trials);
+    std::vector<int> best(workers, init);
+    auto job = [&](int w) {
+        std::vector<int> order(n);
+        for (int t = w; t < trials; t += workers) {
+            u64 s = seed ^ (0xd1b54a32d192ed03ull * (u64)(t + 1));
+            std::iota(order.begin(), order.end(), 0);
+            for (int i = n - 1; i > 0; --i) std::swap(order[i], order[(int)(mix64(s) % (u64)(i + 1))]);
+            BitMat m = p.gen;
+            eliminate(m, order.data(), n);
+            for (int r = 0; r < m.rows; ++r) {
+                int wt = m.weight(r);
+                if (wt == 0 || wt >= best[w]) continue;
+                bool logical = p.every_nonzero;
+                for (int c = 0; c < p.detector.rows && !logical; ++c)
+                    logical = parity_and(p.detector.row(c), m.row(r), m.ld);
+                if (logical) best[w] = wt;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < workers; ++w) pool.emplace_back(job, w);
+    job(0);
+    for (auto& t : pool) t.join();
+    return *std::min_element(best.begin(), best.end());
+}
+
+}  // namespace qsa

@@ -1,0 +1,153 @@
+"""Distance entry points with the reference's signatures and conventions
+(python/qubitserf/distfind/api.py:66-247): every call returns a plain ``int``, ``-1`` when
+the distance is undefined (no logicals); ``which`` is one of min/z/x; ``backend`` must be
+one of auto/cpu/gpu (anything else is a ``ValueError``). This engine serves "gpu" and
+"auto" on a gfx950 device; asking for it without one — or asking for "cpu", which stays
+with the reference library — raises ``RuntimeError``. Only ``method="bz"`` runs here.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .. import _native
+
+_WHICH = {"min": "M", "m": "M", "z": "Z", "x": "X"}
+_BACKENDS = {"auto", "cpu", "gpu"}
+
+
+def _normalize_backend(backend) -> str:
+    b = "auto" if backend is None else str(backend).lower()
+    if b not in _BACKENDS:
+        raise ValueError("backend must be one of auto/cpu/gpu")
+    if b == "cpu":
+        raise RuntimeError("backend 'cpu' is served by the reference library, not the gfx950 engine")
+    if not _native.load().distfind_backend_available(b.encode()):
+        raise RuntimeError("no GPU backend is available on this machine")
+    return b
+
+
+def _as_u8(mat) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(mat, dtype=np.uint8) & 1)
+    if a.ndim != 2:
+        raise ValueError("expected a 2D 0/1 matrix")
+    return a
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_native.u8p)
+
+
+def _as_css(Hx, Hz):
+    if Hz is None:
+        obj = Hx
+        if isinstance(obj, (tuple, list)) and len(obj) == 2:
+            Hx, Hz = obj
+        elif hasattr(obj, "Hx") and hasattr(obj, "Hz"):
+            Hx, Hz = obj.Hx, obj.Hz
+        else:
+            raise TypeError("css_distance expects (Hx, Hz) or an object with .Hx and .Hz")
+    return _as_u8(Hx), _as_u8(Hz)
+
+
+def _check(rc: int, name: str):
+    if rc == 0:
+        return
+    msg = _native.last_error()
+    if rc in (3, 4):
+        raise ValueError(f"{name}: {msg}")
+    raise RuntimeError(f"{name} failed (rc={rc}): {msg}")
+
+
+def _which(which) -> str:
+    w = _WHICH.get(str(which).lower())
+    if w is None:
+        raise ValueError("which must be one of min/z/x")
+    return w
+
+
+def css_distance_raw(Hx, Hz, method="bz", which="M", backend="gpu", threads=0, verbose=False):
+    L = _native.load()
+    ax, az = _as_u8(Hx), _as_u8(Hz)
+    out = _native.DistfindResult()
+    rc = L.distfind_css_distance(_ptr(ax), ax.shape[0], ax.shape[1], _ptr(az), az.shape[0], az.shape[1],
+                                 method.encode(), which.encode()[:1] or b"M", backend.encode(),
+                                 int(threads), 1 if verbose else 0, ctypes.byref(out))
+    _check(rc, "distfind_css_distance")
+    return out
+
+
+def classical_distance_raw(H, method="bz", backend="gpu", threads=0, verbose=False):
+    L = _native.load()
+    a = _as_u8(H)
+    out = _native.DistfindResult()
+    rc = L.distfind_classical_distance(_ptr(a), a.shape[0], a.shape[1], method.encode(), backend.encode(),
+                                       int(threads), 1 if verbose else 0, ctypes.byref(out))
+    _check(rc, "distfind_classical_distance")
+    return out
+
+
+def stabilizer_distance_raw(S, method="bz", which="M", backend="gpu", threads=0, verbose=False):
+    L = _native.load()
+    a = _as_u8(S)
+    if a.shape[1] % 2 != 0:
+        raise ValueError("symplectic stabilizer matrix must have 2n (even) columns")
+    out = _native.DistfindResult()
+    rc = L.distfind_stabilizer_distance(_ptr(a), a.shape[0], a.shape[1], method.encode(),
+                                        which.encode()[:1] or b"M", backend.encode(), int(threads),
+                                        1 if verbose else 0, ctypes.byref(out))
+    _check(rc, "distfind_stabilizer_distance")
+    return out
+
+
+def css_distance(Hx, Hz=None, *, method="bz", which="min", backend="auto", threads=0, verbose=False) -> int:
+    """Distance of a CSS code from its X- and Z-check matrices (api.py:75-96)."""
+    Hx, Hz = _as_css(Hx, Hz)
+    w = _which(which)
+    b = _normalize_backend(backend)
+    return int(css_distance_raw(Hx, Hz, method, w, b, threads, verbose).distance)
+
+
+def classical_distance(H, *, method="bz", backend="auto", threads=0, verbose=False) -> int:
+    """Minimum distance of the classical code with parity-check matrix H (api.py:234-239)."""
+    b = _normalize_backend(backend)
+    return int(classical_distance_raw(H, method, b, threads, verbose).distance)
+
+
+def stabilizer_distance(S, *, method="bz", which="min", backend="auto", threads=0, verbose=False) -> int:
+    """Distance of a stabilizer code given as an (m, 2n) [z|x] matrix (api.py:152-182). CSS
+    input is routed to the CSS solver; a non-CSS code runs BZ on its weight-doubling
+    isometric image and the result is halved."""
+    a = np.ascontiguousarray(np.asarray(S, dtype=np.uint8) & 1)
+    if a.ndim != 2 or a.shape[1] % 2 != 0:
+        raise ValueError("expected a 2D symplectic matrix with 2n (even) columns in [z | x] order")
+    w = _which(which)
+    b = _normalize_backend(backend)
+    return int(stabilizer_distance_raw(a, method, w, b, threads, verbose).distance)
+
+
+def available_backends():
+    """Usable backends of this engine: ["gpu"] on a gfx950 machine, else []."""
+    return ["gpu"] if _native.load().distfind_backend_available(b"gpu") else []
+
+
+def version() -> str:
+    return _native.load().distfind_version().decode()
+
+
+def last_trace():
+    """Per-level records of the last solve on this thread: dicts with side, d, active,
+    ngamma, K, inner, outer, codewords, ms."""
+    L = _native.load()
+    n = L.distfind_last_trace(None, None, None, 0)
+    ints = (ctypes.c_int32 * (7 * max(n, 1)))()
+    cw = (ctypes.c_uint64 * max(n, 1))()
+    ms = (ctypes.c_double * max(n, 1))()
+    L.distfind_last_trace(ints, cw, ms, n)
+    out = []
+    for i in range(n):
+        s, d, act, ng, K, inner, outer = ints[7 * i:7 * i + 7]
+        out.append(dict(side=chr(s), d=d, active=act, ngamma=ng, K=K, inner=inner, outer=outer,
+                        codewords=int(cw[i]), ms=float(ms[i])))
+    return out

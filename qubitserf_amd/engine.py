@@ -1,0 +1,103 @@
+"""One weight level at a time through the plugin-level C ABI (include/qsf_hip.h).
+
+``enumerate_level`` / ``collect_level`` take numpy arrays shaped like the reference's
+EnumPlan (cpp/qsf/backend.hpp:21-42): ``gamma`` is (num_gamma_total, K, stride) uint64,
+``check`` (kcheck, stride) uint64.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+
+import numpy as np
+
+from . import _native
+
+WEIGHT_NONE = 1 << 30
+_keys = itertools.count(0x7e57000000000000)
+
+
+def new_key() -> int:
+    return next(_keys)
+
+
+def _plan(gamma, check, n, d, num_gamma, current_best, key):
+    g = np.ascontiguousarray(gamma, dtype=np.uint64)
+    if g.ndim != 3:
+        raise ValueError("gamma must be (num_gamma_total, K, stride)")
+    ngt, K, stride = g.shape
+    c = np.ascontiguousarray(check if check is not None else np.zeros((0, stride), np.uint64), dtype=np.uint64)
+    p = _native.EnumPlan()
+    p.n, p.stride, p.K, p.d = int(n), int(stride), int(K), int(d)
+    p.num_gamma = int(ngt if num_gamma is None else num_gamma)
+    p.num_gamma_total = int(ngt)
+    p.gamma = g.ctypes.data_as(_native.u64p)
+    p.kcheck = int(c.shape[0])
+    p.check = c.ctypes.data_as(_native.u64p) if c.shape[0] else None
+    p.binom = None
+    p.current_best = int(current_best)
+    p.buffers_key = int(key if key is not None else new_key())
+    return p, (g, c)
+
+
+def _err(rc, name):
+    raise RuntimeError(f"{name} failed (rc={rc}): {(_native.load().qsf_hip_last_error() or b'').decode()}")
+
+
+def available() -> bool:
+    return bool(_native.load().qsf_hip_available())
+
+
+def enumerate_level(gamma, n, d, *, check=None, num_gamma=None, current_best=WEIGHT_NONE, key=None, floor=-1):
+    """min(current_best, lightest logical XOR of d rows of one of the first num_gamma generators)."""
+    L = _native.load()
+    p, keep = _plan(gamma, check, n, d, num_gamma, current_best, key)
+    out = ctypes.c_int(0)
+    rc = L.qsf_hip_enumerate_floor(ctypes.byref(p), int(floor), ctypes.byref(out))
+    if rc:
+        _err(rc, "qsf_hip_enumerate")
+    del keep
+    return out.value
+
+
+def collect_level(gamma, n, d, keep_weight, *, num_gamma=None, key=None):
+    """(rows, stride) uint64 array of every XOR of d rows with weight in [1, keep_weight] (dups allowed)."""
+    L = _native.load()
+    p, keep = _plan(gamma, None, n, d, num_gamma, WEIGHT_NONE, key)
+    buf = _native.u64p()
+    rows = ctypes.c_int64(0)
+    rc = L.qsf_hip_collect(ctypes.byref(p), int(keep_weight), ctypes.byref(buf), ctypes.byref(rows))
+    if rc:
+        _err(rc, "qsf_hip_collect")
+    stride = p.stride
+    try:
+        if rows.value == 0:
+            return np.zeros((0, stride), np.uint64)
+        arr = np.ctypeslib.as_array(buf, shape=(rows.value * stride,)).copy()
+        return arr.reshape(rows.value, stride)
+    finally:
+        L.qsf_hip_free(ctypes.cast(buf, ctypes.c_void_p))
+        del keep
+
+
+def last_stats() -> dict:
+    s = _native.HipStats()
+    _native.load().qsf_hip_last_stats(ctypes.byref(s))
+    return dict(codewords=int(s.codewords), kernel_ms=float(s.kernel_ms), kernel=int(s.kernel), tiles=int(s.tiles))
+
+
+def pack_rows(M: np.ndarray) -> np.ndarray:
+    """Pack a (rows, n) 0/1 matrix into (rows, ceil(n/64)) little-endian uint64 words."""
+    M = np.asarray(M, dtype=np.uint8) & 1
+    rows, n = M.shape
+    stride = (n + 63) // 64
+    padded = np.zeros((rows, stride * 64), dtype=np.uint8)
+    padded[:, :n] = M
+    bits = padded.reshape(rows, stride, 64).astype(np.uint64)
+    return (bits << np.arange(64, dtype=np.uint64)).sum(axis=2, dtype=np.uint64)
+
+
+def unpack_rows(P: np.ndarray, n: int) -> np.ndarray:
+    P = np.asarray(P, dtype=np.uint64)
+    bits = (P[:, :, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)
+    return bits.reshape(P.shape[0], -1)[:, :n].astype(np.uint8)

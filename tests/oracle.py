@@ -1,0 +1,211 @@
+"""ctypes access to the checkers: the oracle restatement (oracle/_build/liboracle.so) and,
+when it has been built, the reference's own CPU library (oracle/_ref/libqubitserf_ref.so).
+Test infrastructure: used by tests/, __graft_entry__.smoke() and bench.py's CPU baseline."""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_LIB = ROOT / "oracle" / "_build" / "liboracle.so"
+REF_LIB = ROOT / "oracle" / "_ref" / "libqubitserf_ref.so"
+WEIGHT_NONE = 1 << 30
+
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class OracleResult(ctypes.Structure):
+    _fields_ = [("distance", ctypes.c_int), ("lower_bound", ctypes.c_int), ("proven", ctypes.c_int),
+                ("levels", ctypes.c_int), ("seconds", ctypes.c_double)]
+
+
+class RefResult(ctypes.Structure):
+    _fields_ = [("distance", ctypes.c_int), ("lower_bound", ctypes.c_int), ("proven", ctypes.c_int),
+                ("levels", ctypes.c_int), ("seconds", ctypes.c_double), ("backend", ctypes.c_char * 16)]
+
+
+_o = None
+_r = None
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not ORACLE_LIB.exists():
+            raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        L = ctypes.CDLL(str(ORACLE_LIB))
+        i, c = ctypes.c_int, ctypes.c_char
+        L.oracle_enumerate.argtypes = [i, i, i, i, i, u64p, i, u64p, i, i]
+        L.oracle_enumerate_brute.argtypes = [i, i, i, i, i, u64p, i, u64p, i]
+        L.oracle_collect.argtypes = [i, i, i, i, i, u64p, i, u64p, ctypes.c_int64, i]
+        L.oracle_collect.restype = ctypes.c_int64
+        rp = ctypes.POINTER(OracleResult)
+        L.oracle_classical_distance.argtypes = [u8p, i, i, i, i, rp]
+        L.oracle_css_distance.argtypes = [u8p, i, i, u8p, i, i, c, i, i, rp]
+        L.oracle_stabilizer_distance.argtypes = [u8p, i, i, c, i, i, rp]
+        L.oracle_bruteforce_distance.argtypes = [i, u8p, i, i, u8p, i, i]
+        L.oracle_mitm_distance.argtypes = [i, u8p, i, i, u8p, i, i, i]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_problem_info.argtypes = [i, u8p, i, i, u8p, i, i, ip, ip, ip, ip, ip, i]
+        _o = L
+    return _o
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def ref():
+    global _r
+    if _r is None:
+        L = ctypes.CDLL(str(REF_LIB))
+        i, c, s = ctypes.c_int, ctypes.c_char, ctypes.c_char_p
+        rp = ctypes.POINTER(RefResult)
+        L.distfind_css_distance.argtypes = [u8p, i, i, u8p, i, i, s, c, s, i, i, rp]
+        L.distfind_classical_distance.argtypes = [u8p, i, i, s, s, i, i, rp]
+        L.distfind_stabilizer_distance.argtypes = [u8p, i, i, s, c, s, i, i, rp]
+        L.codeaut_bz_collect.restype = ctypes.c_void_p
+        L.codeaut_bz_collect.argtypes = [u8p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32]
+        for f in ("codeaut_bz_count", "codeaut_bz_combos"):
+            getattr(L, f).restype = ctypes.c_int64
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.codeaut_bz_copy_rows.argtypes = [ctypes.c_void_p, u8p]
+        L.codeaut_bz_free.argtypes = [ctypes.c_void_p]
+        _r = L
+    return _r
+
+
+def _u8(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint8) & 1)
+
+
+def _p(a):
+    return a.ctypes.data_as(u8p)
+
+
+# ---------------------------------------------------------------- oracle restatement
+
+def classical_distance(H, threads=0, is_threads=0) -> OracleResult:
+    a = _u8(H)
+    r = OracleResult()
+    assert lib().oracle_classical_distance(_p(a), a.shape[0], a.shape[1], threads, is_threads, ctypes.byref(r)) == 0
+    return r
+
+
+def css_distance(Hx, Hz, which="M", threads=0, is_threads=0) -> OracleResult:
+    ax, az = _u8(Hx), _u8(Hz)
+    r = OracleResult()
+    assert lib().oracle_css_distance(_p(ax), ax.shape[0], ax.shape[1], _p(az), az.shape[0], az.shape[1],
+                                     which.encode()[:1], threads, is_threads, ctypes.byref(r)) == 0
+    return r
+
+
+def stabilizer_distance(S, which="M", threads=0, is_threads=0) -> OracleResult:
+    a = _u8(S)
+    r = OracleResult()
+    assert lib().oracle_stabilizer_distance(_p(a), a.shape[0], a.shape[1], which.encode()[:1], threads,
+                                            is_threads, ctypes.byref(r)) == 0
+    return r
+
+
+def _kind_args(kind, A, B):
+    a = _u8(A)
+    b = _u8(B) if B is not None else np.zeros((1, 1), np.uint8)
+    return a, b
+
+
+def bruteforce_distance(kind, A, B=None) -> int:
+    """kind 0: classical parity check A; 1/2: CSS Z/X side of (Hx=A, Hz=B). 2^K messages."""
+    a, b = _kind_args(kind, A, B)
+    return lib().oracle_bruteforce_distance(kind, _p(a), a.shape[0], a.shape[1], _p(b), b.shape[0], b.shape[1])
+
+
+def mitm_distance(kind, A, B=None, wmax=64) -> int:
+    a, b = _kind_args(kind, A, B)
+    return lib().oracle_mitm_distance(kind, _p(a), a.shape[0], a.shape[1], _p(b), b.shape[0], b.shape[1], wmax)
+
+
+def problem_info(kind, A, B=None):
+    a, b = _kind_args(kind, A, B)
+    K, kc, ev, su = (ctypes.c_int() for _ in range(4))
+    ranks = (ctypes.c_int * 512)()
+    ng = lib().oracle_problem_info(kind, _p(a), a.shape[0], a.shape[1], _p(b), b.shape[0], b.shape[1],
+                                   ctypes.byref(K), ctypes.byref(kc), ctypes.byref(ev), ctypes.byref(su), ranks, 512)
+    return dict(K=K.value, kcheck=kc.value, even=bool(ev.value), seed_upper=su.value, ranks=list(ranks[:ng]))
+
+
+def _plan_arrays(gamma, check):
+    g = np.ascontiguousarray(gamma, dtype=np.uint64)
+    ngt, K, stride = g.shape
+    c = np.ascontiguousarray(check if check is not None else np.zeros((0, stride), np.uint64), dtype=np.uint64)
+    cp = c.ctypes.data_as(u64p) if c.shape[0] else None
+    return g, c, cp, ngt, K, stride
+
+
+def enumerate_plan(gamma, n, d, check=None, num_gamma=None, current_best=WEIGHT_NONE, threads=0) -> int:
+    g, c, cp, ngt, K, stride = _plan_arrays(gamma, check)
+    ng = ngt if num_gamma is None else num_gamma
+    return lib().oracle_enumerate(n, stride, K, d, ng, g.ctypes.data_as(u64p), c.shape[0], cp, current_best, threads)
+
+
+def enumerate_brute(gamma, n, d, check=None, num_gamma=None, current_best=WEIGHT_NONE) -> int:
+    g, c, cp, ngt, K, stride = _plan_arrays(gamma, check)
+    ng = ngt if num_gamma is None else num_gamma
+    return lib().oracle_enumerate_brute(n, stride, K, d, ng, g.ctypes.data_as(u64p), c.shape[0], cp, current_best)
+
+
+def collect_plan(gamma, n, d, keep, num_gamma=None, threads=0) -> np.ndarray:
+    g, _, _, ngt, K, stride = _plan_arrays(gamma, None)
+    ng = ngt if num_gamma is None else num_gamma
+    cap = 1 << 16
+    while True:
+        out = np.zeros((cap, stride), np.uint64)
+        rows = lib().oracle_collect(n, stride, K, d, ng, g.ctypes.data_as(u64p), keep, out.ctypes.data_as(u64p),
+                                    cap, threads)
+        if rows <= cap:
+            return out[:rows]
+        cap = rows
+
+
+# ---------------------------------------------------------------- reference CPU library
+
+def ref_css_distance(Hx, Hz, which="M", threads=0) -> RefResult:
+    ax, az = _u8(Hx), _u8(Hz)
+    r = RefResult()
+    assert ref().distfind_css_distance(_p(ax), ax.shape[0], ax.shape[1], _p(az), az.shape[0], az.shape[1], b"bz",
+                                       which.encode()[:1], b"cpu", threads, 0, ctypes.byref(r)) == 0
+    return r
+
+
+def ref_classical_distance(H, threads=0) -> RefResult:
+    a = _u8(H)
+    r = RefResult()
+    assert ref().distfind_classical_distance(_p(a), a.shape[0], a.shape[1], b"bz", b"cpu", threads, 0,
+                                             ctypes.byref(r)) == 0
+    return r
+
+
+def ref_stabilizer_distance(S, which="M", threads=0) -> RefResult:
+    a = _u8(S)
+    r = RefResult()
+    assert ref().distfind_stabilizer_distance(_p(a), a.shape[0], a.shape[1], b"bz", which.encode()[:1], b"cpu",
+                                              threads, 0, ctypes.byref(r)) == 0
+    return r
+
+
+def ref_collect(G, p, keep, threads=0):
+    a = _u8(G)
+    L = ref()
+    h = L.codeaut_bz_collect(_p(a), a.shape[0], a.shape[1], p, keep, 0, threads)
+    try:
+        cnt = L.codeaut_bz_count(h)
+        out = np.zeros((cnt, a.shape[1]), np.uint8)
+        if cnt:
+            L.codeaut_bz_copy_rows(h, _p(out))
+        return out, L.codeaut_bz_combos(h)
+    finally:
+        L.codeaut_bz_free(h)

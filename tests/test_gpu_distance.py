@@ -1,0 +1,112 @@
+"""Distances through the distfind C ABI on the device vs the oracle and known values
+(the cases of the reference's tests/distfind/test_distance.py:30-207)."""
+import numpy as np
+import pytest
+
+from qubitserf_amd import codes
+from tests import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def css_cases():
+    cases = [("steane", codes.steane(), 3), ("shor", codes.shor(), 3)]
+    cases += [(f"toric{L}", codes.toric(L), L) for L in range(3, 8)]
+    cases += [(f"surface{L}", codes.surface(L), L) for L in range(3, 8)]
+    cases += [("bb72", codes.bb_family(6), 6)]
+    return cases
+
+
+CSS = css_cases()
+
+
+@pytest.mark.parametrize("name,HxHz,expected", CSS, ids=[c[0] for c in CSS])
+def test_css_matches_oracle_and_known(hip, name, HxHz, expected):
+    from qubitserf_amd import distfind as df
+
+    Hx, Hz = HxHz
+    for which in ("min", "z", "x"):
+        raw = df.api.css_distance_raw(Hx, Hz, "bz", {"min": "M", "z": "Z", "x": "X"}[which], "gpu")
+        o = O.css_distance(Hx, Hz, which={"min": "M", "z": "Z", "x": "X"}[which])
+        assert raw.backend.decode() == "hip"
+        assert (raw.distance, raw.proven) == (o.distance, o.proven), (name, which, raw.distance, o.distance)
+    vals = {df.css_distance(Hx, Hz, backend="gpu") for _ in range(3)}
+    assert vals == {expected}
+
+
+def test_zx_alternation(hip):
+    from qubitserf_amd import distfind as df
+
+    for L in (5, 6):
+        Hx, Hz = codes.toric(L)
+        for _ in range(6):
+            assert df.css_distance(Hx, Hz, which="z", backend="gpu") == L
+            assert df.css_distance(Hx, Hz, which="x", backend="gpu") == L
+
+
+@pytest.mark.parametrize("n", range(3, 9))
+def test_repetition(hip, n):
+    from qubitserf_amd import distfind as df
+
+    assert df.classical_distance(codes.repetition_parity(n), backend="gpu") == n
+
+
+def test_hamming(hip):
+    from qubitserf_amd import distfind as df
+
+    assert df.classical_distance(codes.hamming_parity(3), backend="gpu") == 3
+    assert df.classical_distance(codes.hamming_parity(4), backend="gpu") == 3
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_config1_random_64_32(hip, seed):
+    from qubitserf_amd import distfind as df
+
+    H = codes.config1_parity(seed)
+    raw = df.api.classical_distance_raw(H, "bz", "gpu")
+    o = O.classical_distance(H)
+    assert (raw.distance, raw.proven) == (o.distance, o.proven)
+
+
+def test_random_classical_sweep(hip):
+    from qubitserf_amd import distfind as df
+
+    rng = np.random.default_rng(21)
+    for _ in range(25):
+        H = codes.random_rate_code(rng, 40, 90)
+        raw = df.api.classical_distance_raw(H, "bz", "gpu")
+        o = O.classical_distance(H)
+        assert (raw.distance, raw.proven) == (o.distance, o.proven), H.shape
+
+
+@pytest.mark.parametrize("m", [7, 9, 10])
+def test_qrm_large_n(hip, m):
+    from qubitserf_amd import distfind as df
+
+    Hx, Hz = codes.quantum_reed_muller(1, m)
+    assert df.css_distance(Hx, Hz, backend="gpu") == 4
+
+
+def five_qubit():
+    # [[5,1,3]]: cyclic shifts of XZZXI in [z|x] order
+    z = np.array([0, 1, 1, 0, 0]); x = np.array([1, 0, 0, 1, 0])
+    rows = [np.concatenate([np.roll(z, s), np.roll(x, s)]) for s in range(4)]
+    return np.array(rows, dtype=np.uint8)
+
+
+def test_stabilizer_noncss(hip):
+    from qubitserf_amd import distfind as df
+
+    S = five_qubit()
+    assert df.stabilizer_distance(S, backend="gpu") == 3
+    assert O.stabilizer_distance(S).distance == 3
+
+
+def test_stabilizer_css_route(hip):
+    from qubitserf_amd import distfind as df
+
+    Hx, Hz = codes.toric(4)
+    n = Hx.shape[1]
+    S = np.vstack([np.hstack([np.zeros_like(Hx), Hx]), np.hstack([Hz, np.zeros_like(Hz)])])
+    assert df.stabilizer_distance(S, backend="gpu") == 4
+    assert df.stabilizer_distance(S, which="z", backend="gpu") == 4

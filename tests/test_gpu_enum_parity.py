@@ -1,0 +1,98 @@
+"""Device enumerator vs the oracle on randomized EnumPlans (the design of the reference's
+cpp/distfind/tests/cpu_enum_ref.cpp:27-127, with the device as the system under test and
+the oracle's CPU enumerator as the checker). Integer results: exact equality."""
+import numpy as np
+import pytest
+
+from tests import oracle as O
+
+pytestmark = pytest.mark.gpu
+NONE = 1 << 30
+
+
+def rand_plan(rng, K, stride, ng, kcheck, systematic=False):
+    gamma = rng.integers(0, 2**64, size=(ng, K, stride), dtype=np.uint64)
+    if systematic:   # identity on K columns spread over the row, like real information sets
+        for g in range(ng):
+            cols = rng.choice(stride * 64, size=K, replace=False)
+            for r, c in enumerate(cols):
+                w, b = divmod(int(c), 64)
+                gamma[g, :, w] &= ~np.uint64(1 << b)
+                gamma[g, r, w] |= np.uint64(1 << b)
+    check = rng.integers(0, 2**64, size=(kcheck, stride), dtype=np.uint64) if kcheck else None
+    return gamma, check
+
+
+def run_case(hip, rng, K, stride, d, ng, kcheck, best, systematic=False, num_gamma=None):
+    n = stride * 64
+    gamma, check = rand_plan(rng, K, stride, ng, kcheck, systematic)
+    got = hip.enumerate_level(gamma, n, d, check=check, current_best=best, num_gamma=num_gamma)
+    kern = hip.last_stats()["kernel"]
+    want = O.enumerate_plan(gamma, n, d, check=check, current_best=best, num_gamma=num_gamma)
+    assert got == want, f"K={K} stride={stride} d={d} ng={ng} kcheck={kcheck} best={best}: gpu={got} oracle={want} kernel={kern}"
+    return kern
+
+
+def test_small_random_plans(hip):
+    rng = np.random.default_rng(0xC0FFEE)
+    for _ in range(60):
+        K = int(rng.integers(4, 15)); stride = int(rng.integers(1, 4)); d = int(rng.integers(1, 6))
+        d = min(d, K)
+        ng = int(rng.integers(1, 4)); kcheck = int(rng.integers(0, 4))
+        roll = int(rng.integers(0, 4))
+        best = NONE if roll == 0 else int(rng.integers(0, stride * 64 + 1))
+        run_case(hip, rng, K, stride, d, ng, kcheck, best)
+
+
+def test_edge_plans(hip):
+    rng = np.random.default_rng(1)
+    run_case(hip, rng, 8, 1, 1, 1, 0, NONE)     # d == 1
+    run_case(hip, rng, 6, 2, 1, 2, 2, NONE)
+    run_case(hip, rng, 5, 1, 5, 1, 1, NONE)     # d == K: one combination
+    run_case(hip, rng, 7, 2, 4, 2, 0, NONE)     # every codeword logical
+    run_case(hip, rng, 10, 3, 3, 1, 2, 0)       # current_best 0: nothing lighter
+    run_case(hip, rng, 12, 1, 4, 3, 1, 1)       # very tight cap
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_tiled_kernel_plans(hip, seed):
+    """Levels big enough for the tiled kernel, random and systematic generators."""
+    rng = np.random.default_rng(100 + seed)
+    K = int(rng.integers(24, 60)); stride = int(rng.integers(1, 4)); d = int(rng.integers(3, 6))
+    ng = int(rng.integers(1, 3)); kcheck = int(rng.integers(0, 4))
+    best = [NONE, 40, 25, int(stride * 64 // 3)][seed % 4]
+    kern = run_case(hip, rng, K, stride, d, ng, kcheck, best, systematic=bool(seed % 2))
+    assert kern == 2, "expected the tiled kernel on this level"
+
+
+def test_tiled_masked_active_prefix(hip):
+    """num_gamma < num_gamma_total: only the active prefix is enumerated."""
+    rng = np.random.default_rng(5)
+    run_case(hip, rng, 40, 2, 4, 3, 2, 60, systematic=True, num_gamma=1)
+    run_case(hip, rng, 40, 2, 4, 3, 2, 60, systematic=True, num_gamma=2)
+
+
+def test_key_reuse_across_levels(hip):
+    """One buffers_key per solve: device tables cached across levels stay correct, and a
+    new solve with a new key (same host buffer) is never served stale data."""
+    rng = np.random.default_rng(9)
+    n = 128
+    for trial in range(3):
+        gamma, check = rand_plan(rng, 36, 2, 2, 2, systematic=True)
+        key = 0xABC000 + trial
+        for d in (1, 2, 3, 4):
+            got = hip.enumerate_level(gamma, n, d, check=check, current_best=NONE, key=key)
+            want = O.enumerate_plan(gamma, n, d, check=check, current_best=NONE)
+            assert got == want, (trial, d, got, want)
+
+
+def test_collect_matches_oracle(hip):
+    rng = np.random.default_rng(11)
+    for K, stride, d, keep in [(12, 1, 3, 30), (20, 2, 3, 50), (30, 1, 2, 28), (16, 3, 4, 80)]:
+        gamma, _ = rand_plan(rng, K, stride, 1, 0, systematic=True)
+        n = stride * 64
+        got = hip.collect_level(gamma, n, d, keep)
+        want = O.collect_plan(gamma, n, d, keep)
+        g = np.array(sorted(map(tuple, got.tolist())), dtype=np.uint64).reshape(-1, stride)
+        w = np.array(sorted(map(tuple, want.tolist())), dtype=np.uint64).reshape(-1, stride)
+        assert g.shape == w.shape and np.array_equal(g, w), (K, d, keep, g.shape, w.shape)
