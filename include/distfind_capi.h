@@ -81,10 +81,11 @@ const char* distfind_version(void);
 const char* distfind_last_error(void);
 
 /* Per-level trace of the last distfind_* solve on this thread: up to `cap` rows of
- * {side ('S','Z','X'), d, active sets, total sets, K, inner, outer} and the codeword
- * count and wall milliseconds of each level. Returns the number of levels recorded.
- * New (the reference prints the same fields under DISTFIND_PROFILE, bz.cpp:147-155). */
-int distfind_last_trace(int32_t* ints7, uint64_t* codewords, double* ms, int cap);
+ * ints8 = {side ('S','Z','X'), d, active sets, total sets, K, inner, outer, kernel
+ * (0 none, 1 row sweep, 2 tiled)}, the codeword count, and ms2 = {wall ms, device kernel
+ * ms} of each level. Returns the number of levels recorded. New (the reference prints
+ * the same fields under DISTFIND_PROFILE, bz.cpp:147-155). */
+int distfind_last_trace(int32_t* ints8, uint64_t* codewords, double* ms2, int cap);
 
 #ifdef __cplusplus
 }

@@ -868,6 +868,26 @@ int oracle_mitm_distance(int kind, const uint8_t* A, int ar, int ac, const uint8
     return orc::mitm(make_problem(kind, A, ar, ac, B2, br, bc), wmax);
 }
 
+// The information-set generators, ranks and detector of a subproblem, flattened as an
+// EnumPlan would carry them (bz.cpp:95-128). Returns the number of generators, or -1 if a
+// buffer is too small.
+int oracle_side_plan(int kind, const uint8_t* A, int ar, int ac, const uint8_t* B2, int br, int bc,
+                     uint64_t* gamma, int64_t gamma_cap, int* ranks, int ranks_cap, uint64_t* check,
+                     int64_t check_cap, int* K, int* kcheck, int* even) {
+    orc::Problem p = make_problem(kind, A, ar, ac, B2, br, bc);
+    std::vector<orc::Gamma> seq = orc::gamma_sequence(p.gen);
+    std::stable_sort(seq.begin(), seq.end(), [](const orc::Gamma& a, const orc::Gamma& b) { return a.rank > b.rank; });
+    const int k = p.gen.rows, st = orc::wfor(p.n), ng = (int)seq.size();
+    *K = k; *kcheck = p.count_all ? 0 : p.check.rows; *even = p.even;
+    if ((int64_t)ng * k * st > gamma_cap || ng > ranks_cap || (int64_t)*kcheck * st > check_cap) return -1;
+    for (int g = 0; g < ng; ++g) {
+        ranks[g] = seq[g].rank;
+        std::memcpy(gamma + (size_t)g * k * st, seq[g].g.w.data(), (size_t)k * st * 8);
+    }
+    if (*kcheck) std::memcpy(check, p.check.w.data(), (size_t)*kcheck * st * 8);
+    return ng;
+}
+
 // Upper bound from the problem's seed and 256 random information sets (bz.cpp:115-128).
 int oracle_upper_seed(int kind, const uint8_t* A, int ar, int ac, const uint8_t* B2, int br, int bc, int is_threads) {
     orc::Problem p = make_problem(kind, A, ar, ac, B2, br, bc);

@@ -26,6 +26,7 @@ SRCS = \
   $(REF)/cpp/codeaut/capi.cpp
 
 OBJS = $(patsubst $(REF)/cpp/%.cpp,$(OUT)/obj/%.o,$(SRCS))
+SHIM = $(OUT)/obj/ref_shim.o
 
 all: $(OUT)/libqubitserf_ref.so $(OUT)/cpu_enum_ref
 
@@ -33,8 +34,13 @@ $(OUT)/obj/%.o: $(REF)/cpp/%.cpp
 	@mkdir -p $(dir $@)
 	$(CXX) $(CXXFLAGS) -I$(REF)/cpp -I$(REF)/cpp/codeaut -c $< -o $@
 
-$(OUT)/libqubitserf_ref.so: $(OBJS)
-	$(CXX) $(CXXFLAGS) -shared $(OBJS) -o $@
+# ref_shim.cpp (ours) exposes the reference's CPU enumerator for single-level timing
+$(SHIM): oracle/ref_shim.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -I$(REF)/cpp -c $< -o $@
+
+$(OUT)/libqubitserf_ref.so: $(OBJS) $(SHIM)
+	$(CXX) $(CXXFLAGS) -shared $(OBJS) $(SHIM) -o $@
 
 # The reference's own randomized-plan self-check of its CPU enumerator.
 $(OUT)/cpu_enum_ref: $(REF)/cpp/distfind/tests/cpu_enum_ref.cpp $(OUT)/libqubitserf_ref.so

@@ -130,6 +130,7 @@ def load() -> ctypes.CDLL:
     sig("qsf_hip_free", None, [vp])
     sig("qsf_hip_last_stats", None, [ctypes.POINTER(HipStats)])
     sig("qsf_hip_last_error", c_char_p, [])
+    sig("qsf_hip_set_kernel_policy", c_int, [c_int])
     _lib = L
     return L
 

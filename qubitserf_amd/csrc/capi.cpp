@@ -167,16 +167,17 @@ const char* distfind_version(void) { return "0.1.0+gfx950"; }
 
 const char* distfind_last_error(void) { return g_err.c_str(); }
 
-int distfind_last_trace(int32_t* ints7, uint64_t* codewords, double* ms, int cap) {
+int distfind_last_trace(int32_t* ints8, uint64_t* codewords, double* ms2, int cap) {
     const int n = (int)g_trace.size();
     for (int i = 0; i < n && i < cap; ++i) {
         const LevelTrace& t = g_trace[i];
-        if (ints7) {
-            int32_t* o = ints7 + 7 * i;
+        if (ints8) {
+            int32_t* o = ints8 + 8 * i;
             o[0] = t.side; o[1] = t.d; o[2] = t.active; o[3] = t.ngamma; o[4] = t.K; o[5] = t.inner; o[6] = t.outer;
+            o[7] = t.kernel;
         }
         if (codewords) codewords[i] = t.codewords;
-        if (ms) ms[i] = t.ms;
+        if (ms2) { ms2[2 * i] = t.ms; ms2[2 * i + 1] = t.kernel_ms; }
     }
     return n;
 }

@@ -106,8 +106,8 @@ struct Side {
             std::fprintf(stderr, "[prof hip] d=%d K=%d ng=%d/%d stride=%d work=%llu kernel=%d -> %.3f ms (device %.3f ms)\n",
                          d, K, act, ng, stride, (unsigned long long)st.codewords, st.kernel, ms, st.kernel_ms);
         if (opt->trace)
-            opt->trace->push_back(LevelTrace{*tag ? tag[0] : 'S', d, act, ng, K,
-                                             (unsigned long long)st.codewords, ms, inner, outer});
+            opt->trace->push_back(LevelTrace{*tag ? tag[0] : 'S', d, act, ng, K, (unsigned long long)st.codewords,
+                                             ms, st.kernel_ms, st.kernel, inner, outer});
     }
 };
 

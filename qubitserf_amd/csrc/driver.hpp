@@ -19,7 +19,9 @@ struct LevelTrace {
     char side;             // 'S' single, 'Z'/'X' interleaved
     int d, active, ngamma, K;
     unsigned long long codewords;
-    double ms;
+    double ms;             // wall time of the level (host)
+    double kernel_ms;      // device time of the enumeration kernel (HIP events)
+    int kernel;            // 0 none, 1 row sweep, 2 tiled
     int inner, outer;      // bounds after the level
 };
 

@@ -402,6 +402,7 @@ struct Engine {
     DevBuf<u64> out;
     qsf_hip_stats stats{};
     std::string err;
+    int policy = 0;   // 0 auto, 1 row sweep, 2 tiled where it applies
 
     bool probe() {
         if (probed) return ok;
@@ -523,8 +524,8 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(best.p, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), stream));
 
-        const bool tiled = p.K <= kMaxTiledK && s.RW <= kMaxRW && p.d >= 2 && p.d <= 32 &&
-                           stats.codewords >= (1ull << 16);
+        const bool can_tile = p.K <= kMaxTiledK && s.RW <= kMaxRW && p.d <= 32;
+        const bool tiled = can_tile && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
         if (tiled) launch_tiled(s, p, floor);
         else launch_sweep(s, p, false, 0, 0);
 
@@ -573,9 +574,11 @@ struct Engine {
         stats.kernel = 2;
         stats.tiles = (int)std::min<u64>(tiles, 0x7fffffff);
         HIPCHK(hipEventRecord(ev0, stream));
-        if (FD == 1) hipLaunchKernelGGL((bz_tiled<1, 16>), dim3((unsigned)grid), dim3(64), 0, stream, a);
-        else hipLaunchKernelGGL((bz_tiled<2, 16>), dim3((unsigned)grid), dim3(64), 0, stream, a);
-        HIPCHK(hipGetLastError());
+        if (grid > 0) {
+            if (FD == 1) hipLaunchKernelGGL((bz_tiled<1, 16>), dim3((unsigned)grid), dim3(64), 0, stream, a);
+            else hipLaunchKernelGGL((bz_tiled<2, 16>), dim3((unsigned)grid), dim3(64), 0, stream, a);
+            HIPCHK(hipGetLastError());
+        }
         HIPCHK(hipEventRecord(ev1, stream));
         (void)RPL;
     }
@@ -596,8 +599,10 @@ struct Engine {
         const u64 blocks = (threads + 255) / 256;
         stats.kernel = 1;
         HIPCHK(hipEventRecord(ev0, stream));
-        hipLaunchKernelGGL(bz_rowsweep, dim3((unsigned)blocks), dim3(256), 0, stream, a);
-        HIPCHK(hipGetLastError());
+        if (blocks > 0) {
+            hipLaunchKernelGGL(bz_rowsweep, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+            HIPCHK(hipGetLastError());
+        }
         HIPCHK(hipEventRecord(ev1, stream));
     }
 
@@ -660,7 +665,9 @@ extern "C" {
 int qsf_hip_available(void) {
     auto& e = engine();
     std::lock_guard<std::mutex> lk(e.mu);
-    return e.probe() ? 1 : 0;
+    if (e.probe()) return 1;
+    e.err = "qsf_hip_available: " + e.probe_msg;
+    return 0;
 }
 
 static int run_guarded(const char* what, const std::function<int(qsa::Engine&)>& f) {
@@ -702,6 +709,14 @@ int qsf_hip_collect(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits,
 }
 
 void qsf_hip_free(void* p) { std::free(p); }
+
+int qsf_hip_set_kernel_policy(int policy) {
+    auto& e = engine();
+    std::lock_guard<std::mutex> lk(e.mu);
+    int old = e.policy;
+    e.policy = (policy >= 0 && policy <= 2) ? policy : 0;
+    return old;
+}
 
 void qsf_hip_last_stats(qsf_hip_stats* out) {
     if (out) *out = engine().stats;

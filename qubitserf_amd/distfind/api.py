@@ -138,16 +138,16 @@ def version() -> str:
 
 def last_trace():
     """Per-level records of the last solve on this thread: dicts with side, d, active,
-    ngamma, K, inner, outer, codewords, ms."""
+    ngamma, K, inner, outer, kernel, codewords, ms (wall) and kernel_ms (device)."""
     L = _native.load()
     n = L.distfind_last_trace(None, None, None, 0)
-    ints = (ctypes.c_int32 * (7 * max(n, 1)))()
+    ints = (ctypes.c_int32 * (8 * max(n, 1)))()
     cw = (ctypes.c_uint64 * max(n, 1))()
-    ms = (ctypes.c_double * max(n, 1))()
+    ms = (ctypes.c_double * (2 * max(n, 1)))()
     L.distfind_last_trace(ints, cw, ms, n)
     out = []
     for i in range(n):
-        s, d, act, ng, K, inner, outer = ints[7 * i:7 * i + 7]
-        out.append(dict(side=chr(s), d=d, active=act, ngamma=ng, K=K, inner=inner, outer=outer,
-                        codewords=int(cw[i]), ms=float(ms[i])))
+        s, d, act, ng, K, inner, outer, kern = ints[8 * i:8 * i + 8]
+        out.append(dict(side=chr(s), d=d, active=act, ngamma=ng, K=K, inner=inner, outer=outer, kernel=kern,
+                        codewords=int(cw[i]), ms=float(ms[2 * i]), kernel_ms=float(ms[2 * i + 1])))
     return out

@@ -80,6 +80,22 @@ def collect_level(gamma, n, d, keep_weight, *, num_gamma=None, key=None):
         del keep
 
 
+class kernel_policy:
+    """Context manager forcing the kernel choice: "auto", "sweep" or "tiled"."""
+
+    _codes = {"auto": 0, "sweep": 1, "tiled": 2}
+
+    def __init__(self, name: str):
+        self.code = self._codes[name]
+
+    def __enter__(self):
+        self.prev = _native.load().qsf_hip_set_kernel_policy(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        _native.load().qsf_hip_set_kernel_policy(self.prev)
+
+
 def last_stats() -> dict:
     s = _native.HipStats()
     _native.load().qsf_hip_last_stats(ctypes.byref(s))

@@ -209,3 +209,39 @@ def ref_collect(G, p, keep, threads=0):
         return out, L.codeaut_bz_combos(h)
     finally:
         L.codeaut_bz_free(h)
+
+
+def side_plan(kind, A, B=None):
+    """(gamma[ng,K,stride], ranks, check[kcheck,stride], even) of a subproblem's information sets."""
+    L = lib()
+    a, b = _kind_args(kind, A, B)
+    n = a.shape[1]
+    stride = (n + 63) // 64
+    cap = 64 * n * stride * 8 + 64
+    gamma = np.zeros(cap, np.uint64)
+    ranks = (ctypes.c_int * 256)()
+    check = np.zeros(n * stride + 1, np.uint64)
+    K, kc, ev = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.oracle_side_plan.restype = ctypes.c_int
+    L.oracle_side_plan.argtypes = [ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int,
+                                   u64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.c_int, u64p,
+                                   ctypes.c_int64] + [ctypes.POINTER(ctypes.c_int)] * 3
+    ng = L.oracle_side_plan(kind, _p(a), a.shape[0], a.shape[1], _p(b), b.shape[0], b.shape[1],
+                            gamma.ctypes.data_as(u64p), cap, ranks, 256, check.ctypes.data_as(u64p), check.size,
+                            ctypes.byref(K), ctypes.byref(kc), ctypes.byref(ev))
+    assert ng >= 0
+    k = K.value
+    return (gamma[: ng * k * stride].reshape(ng, k, stride).copy(), list(ranks[:ng]),
+            check[: kc.value * stride].reshape(kc.value, stride).copy(), bool(ev.value))
+
+
+def ref_enumerate_plan(gamma, n, d, check=None, num_gamma=None, current_best=WEIGHT_NONE, threads=0) -> int:
+    """The reference's own CPU enumerator on one level (oracle/ref_shim.cpp)."""
+    L = ref()
+    f = L.ref_enumerate_plan
+    f.restype = ctypes.c_int
+    i = ctypes.c_int
+    f.argtypes = [i, i, i, i, i, u64p, i, u64p, i, i]
+    g, c, cp, ngt, K, stride = _plan_arrays(gamma, check)
+    ng = ngt if num_gamma is None else num_gamma
+    return f(n, stride, K, d, ng, g.ctypes.data_as(u64p), c.shape[0], cp, current_best, threads)

@@ -33,8 +33,14 @@ def run_case(hip, rng, K, stride, d, ng, kcheck, best, systematic=False, num_gam
     return kern
 
 
-def test_small_random_plans(hip):
+@pytest.mark.parametrize("policy", ["auto", "sweep", "tiled"])
+def test_small_random_plans(hip, policy):
     rng = np.random.default_rng(0xC0FFEE)
+    with hip.kernel_policy(policy):
+        _small_random_plans(hip, rng)
+
+
+def _small_random_plans(hip, rng):
     for _ in range(60):
         K = int(rng.integers(4, 15)); stride = int(rng.integers(1, 4)); d = int(rng.integers(1, 6))
         d = min(d, K)
@@ -44,7 +50,13 @@ def test_small_random_plans(hip):
         run_case(hip, rng, K, stride, d, ng, kcheck, best)
 
 
-def test_edge_plans(hip):
+@pytest.mark.parametrize("policy", ["auto", "sweep", "tiled"])
+def test_edge_plans(hip, policy):
+    with hip.kernel_policy(policy):
+        _edge_plans(hip)
+
+
+def _edge_plans(hip):
     rng = np.random.default_rng(1)
     run_case(hip, rng, 8, 1, 1, 1, 0, NONE)     # d == 1
     run_case(hip, rng, 6, 2, 1, 2, 2, NONE)
@@ -61,7 +73,8 @@ def test_tiled_kernel_plans(hip, seed):
     K = int(rng.integers(24, 60)); stride = int(rng.integers(1, 4)); d = int(rng.integers(3, 6))
     ng = int(rng.integers(1, 3)); kcheck = int(rng.integers(0, 4))
     best = [NONE, 40, 25, int(stride * 64 // 3)][seed % 4]
-    kern = run_case(hip, rng, K, stride, d, ng, kcheck, best, systematic=bool(seed % 2))
+    with hip.kernel_policy("tiled"):
+        kern = run_case(hip, rng, K, stride, d, ng, kcheck, best, systematic=bool(seed % 2))
     assert kern == 2, "expected the tiled kernel on this level"
 
 
