@@ -83,6 +83,10 @@ typedef struct {
     double kernel_ms;
     int kernel;
     int tiles;
+    int fd;        /* tiled kernel: residual dwords in the first (fast) test */
+    uint64_t slow_hits;   /* tiled kernel: stream steps that entered the exact slow path */
+    int xd;               /* tiled kernel: residual dwords held per codeword (medium test) */
+    uint64_t medium_hits; /* tiled kernel: stream steps that passed the hot-loop test */
 } qsf_hip_stats;
 void qsf_hip_last_stats(qsf_hip_stats* out);
 

@@ -66,6 +66,10 @@ class HipStats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double),
         ("kernel", ctypes.c_int),
         ("tiles", ctypes.c_int),
+        ("fd", ctypes.c_int),
+        ("slow_hits", ctypes.c_uint64),
+        ("xd", ctypes.c_int),
+        ("medium_hits", ctypes.c_uint64),
     ]
 
 

@@ -1,0 +1,93 @@
+// Calibration probes for the measurements (a separate executable, not part of the library).
+//
+//   qsa_calib valu            integer-VALU peak: every lane runs 8 independent chains of
+//                             x = popcount(x ^ k) + x, i.e. one v_xor_b32 + one
+//                             v_bcnt_u32_b32 per step, over 2048 x 256 threads. Prints the
+//                             achieved int32 lane-ops/s.
+//   qsa_calib copy <MiB>      a plain device copy of <MiB> MiB (one read + one write of each
+//                             byte); used to scale the FETCH_SIZE / WRITE_SIZE counters.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#define CHECK(x)                                                                       \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            std::fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e_));               \
+            std::exit(1);                                                              \
+        }                                                                              \
+    } while (0)
+
+__global__ __launch_bounds__(256) void valu_chain(unsigned* out, unsigned key, int iters) {
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned x0 = t, x1 = t * 3u, x2 = t * 5u, x3 = t * 7u, x4 = t * 11u, x5 = t * 13u, x6 = t * 17u, x7 = t * 19u;
+    for (int i = 0; i < iters; ++i) {
+        x0 = __popc(x0 ^ key) + x0;
+        x1 = __popc(x1 ^ key) + x1;
+        x2 = __popc(x2 ^ key) + x2;
+        x3 = __popc(x3 ^ key) + x3;
+        x4 = __popc(x4 ^ key) + x4;
+        x5 = __popc(x5 ^ key) + x5;
+        x6 = __popc(x6 ^ key) + x6;
+        x7 = __popc(x7 ^ key) + x7;
+    }
+    out[t] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7;
+}
+
+__global__ __launch_bounds__(256) void copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: qsa_calib valu | copy <MiB>\n");
+        return 2;
+    }
+    hipEvent_t e0, e1;
+    CHECK(hipEventCreate(&e0));
+    CHECK(hipEventCreate(&e1));
+    if (std::strcmp(argv[1], "valu") == 0) {
+        const int blocks = 2048 * 4, threads = 256, iters = 20000;
+        unsigned* out;
+        CHECK(hipMalloc(&out, (size_t)blocks * threads * sizeof(unsigned)));
+        hipLaunchKernelGGL(valu_chain, dim3(blocks), dim3(threads), 0, 0, out, 0x9e3779b9u, 100);  // warm up
+        CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(e0, 0));
+        hipLaunchKernelGGL(valu_chain, dim3(blocks), dim3(threads), 0, 0, out, 0x9e3779b9u, iters);
+        CHECK(hipEventRecord(e1, 0));
+        CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double ops = (double)blocks * threads * iters * 8 * 2;
+        std::printf("{\"probe\": \"valu\", \"lane_ops\": %.6e, \"ms\": %.3f, \"tops\": %.4f}\n", ops, ms,
+                    ops / (ms * 1e-3) / 1e12);
+        CHECK(hipFree(out));
+        return 0;
+    }
+    if (std::strcmp(argv[1], "copy") == 0 && argc >= 3) {
+        const size_t mib = (size_t)std::atol(argv[2]);
+        const size_t bytes = mib << 20, n = bytes / sizeof(uint4);
+        uint4 *a, *b;
+        CHECK(hipMalloc(&a, bytes));
+        CHECK(hipMalloc(&b, bytes));
+        CHECK(hipMemset(a, 1, bytes));
+        CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(e0, 0));
+        hipLaunchKernelGGL(copy_u4, dim3(4096), dim3(256), 0, 0, a, b, n);
+        CHECK(hipEventRecord(e1, 0));
+        CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        CHECK(hipEventElapsedTime(&ms, e0, e1));
+        std::printf("{\"probe\": \"copy\", \"bytes\": %zu, \"ms\": %.3f, \"GBps\": %.1f}\n", bytes, ms,
+                    2.0 * bytes / (ms * 1e-3) / 1e9);
+        CHECK(hipFree(a));
+        CHECK(hipFree(b));
+        return 0;
+    }
+    std::fprintf(stderr, "unknown probe\n");
+    return 2;
+}

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -52,6 +54,12 @@ __device__ __forceinline__ u64 colex_next(u64 x) {
     return (((r ^ x) >> 2) >> __builtin_ctzll(c)) | r;
 }
 
+// Bits [base, base + m) of the 128-bit mask (lo, hi), shifted down to bit 0 (m <= 64).
+__device__ __forceinline__ u64 range_bits(u64 lo, u64 hi, int base, int m) {
+    u64 v = base == 0 ? lo : base < 64 ? (lo >> base) | (hi << (64 - base)) : hi >> (base - 64);
+    return m >= 64 ? v : v & ((1ull << m) - 1ull);
+}
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
@@ -68,27 +76,33 @@ struct SElem {
 // bz_tiled: one workgroup = one wavefront; persistent, tiles pulled from a global counter
 // =====================================================================================
 
-template <int FD, int RPL>
+// FD residual dwords are tested in the hot loop; XD >= FD dwords are held per codeword and
+// re-tested (the "medium" test) before a stream step drops into the exact slow path.
+template <int FD, int XD, int RPL>
 __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
-    __shared__ u32 rows_fast[kMaxTiledK * FD];
-    __shared__ SElem<FD> stream[kStreamMax];
+    __shared__ u32 rows_fast[kMaxTiledK * XD];
+    __shared__ SElem<XD> stream[kStreamMax];
     __shared__ u32 sfull[kMaxRW];
+    __shared__ u64 run_start[64];            // colex mask of the first stream element of each lane's run
+    extern __shared__ u32 rows_full[];       // whole residual+syndrome table when it fits (dynamic size)
 
     const int lane = threadIdx.x;
-    const int K = a.K, h = a.h, RW = a.RW, RD = a.RD, SD = a.SD;
-    int bound = __builtin_amdgcn_readfirstlane(__hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int K = a.K, RW = a.RW, RD = a.RD, SD = a.SD;
+    int bound = a.collect ? a.keep + 1
+                          : __builtin_amdgcn_readfirstlane(__hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     int cur_g = -1;
+    unsigned slow_entries = 0, medium_entries = 0;
 
     for (;;) {
         unsigned long long t = 0;
         if (lane == 0) t = atomicAdd(a.next_tile, 1ull);
         t = __shfl(t, 0, 64);
         if (t >= a.total_tiles) break;
-        {
+        if (!a.collect) {
             int gb = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bound = min(bound, __builtin_amdgcn_readfirstlane(gb));
+            if (bound <= a.floor) break;
         }
-        if (bound <= a.floor) break;
 
         // locate the rectangle holding tile t (rects sorted by tile_base)
         int lo = 0, hi = a.nrects - 1;
@@ -98,72 +112,52 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         }
         const Rect R = a.rects[lo];
         const u64 local = t - R.tile_base;
-        const u64 bx = local / R.stream_chunks;
-        const u64 by = local - bx * R.stream_chunks;
+        const u64 bx = local / R.stream_groups;
+        const u64 bg = local - bx * R.stream_groups;
         const int g = R.g;
 
-        const int lk = R.lanes_on_a ? R.ka : R.kb;
-        const int lbase = R.lanes_on_a ? 0 : h;
-        const int lm = R.lanes_on_a ? h : K - h;
-        const u64 lflags = a.flags[2 * g + (R.lanes_on_a ? 0 : 1)];
-        const int sk = R.lanes_on_a ? R.kb : R.ka;
-        const int sbase = R.lanes_on_a ? h : 0;
-        const int sm = R.lanes_on_a ? K - h : h;
-        const u64 sflags = a.flags[2 * g + (R.lanes_on_a ? 1 : 0)];
+        const int lk = R.lk, lbase = R.lbase, lm = R.lm;
+        const int sk = R.sk, sbase = R.sbase, sm = R.sm;
+        const u64 flo = a.flags[2 * g], fhi = a.flags[2 * g + 1];
+        const u64 lflags = range_bits(flo, fhi, lbase, lm);
+        const u64 sflags = range_bits(flo, fhi, sbase, sm);
+        const u32* fx = R.fixed >= 0 ? a.fixed + (size_t)R.fixed * (RW + 1) : nullptr;
+        u32 fw[XD];
+#pragma unroll
+        for (int f = 0; f < XD; ++f) fw[f] = (fx && f < RD) ? fx[f] : 0u;
+        const int foff = fx ? (int)fx[RW] : 0;
         const u32* gtbl = a.tbl + (size_t)g * K * RW;
 
+        const bool full_in_lds = K * RW <= kLdsFullRows;
+        const u32* slow_tbl = full_in_lds ? rows_full : gtbl;
         __syncthreads();   // previous tile's readers of LDS are done
         if (g != cur_g) {
-            for (int i = lane; i < K * FD; i += 64) rows_fast[i] = gtbl[(i / FD) * RW + (i % FD)];
+            for (int i = lane; i < K * XD; i += 64) rows_fast[i] = (i % XD) < RD ? gtbl[(i / XD) * RW + (i % XD)] : 0u;
+            if (full_in_lds)
+                for (int i = lane; i < K * RW; i += 64) rows_full[i] = gtbl[i];
             cur_g = g;
         }
-
-        // ---- stream chunk -> LDS (each lane a contiguous run, colex successor steps)
-        const u64 c0 = by * (u64)a.S;
-        const int len = (int)min((u64)a.S, R.n_stream - c0);
         __syncthreads();   // rows_fast visible
-        {
-            const int per = (len + 63) >> 6;
-            const int i0 = lane * per;
-            const int i1 = min(len, i0 + per);
-            if (i0 < i1) {
-                u64 m = colex_mask(c0 + (u64)i0, sk, sm, a.binom, a.bk);
-                for (int i = i0; i < i1; ++i) {
-                    SElem<FD> e;
-#pragma unroll
-                    for (int f = 0; f < FD; ++f) e.w[f] = 0;
-                    u64 mm = m;
-                    while (mm) {
-                        int b = __builtin_ctzll(mm);
-                        mm &= mm - 1;
-#pragma unroll
-                        for (int f = 0; f < FD; ++f) e.w[f] ^= rows_fast[(sbase + b) * FD + f];
-                    }
-                    e.off = __popcll(m & sflags);
-                    stream[i] = e;
-                    if (i + 1 < i1) m = colex_next(m);
-                }
-            }
-        }
 
-        // ---- register tile: RPL consecutive list elements per lane
-        u32 xw[RPL][FD];
+        // ---- register tile: RPL consecutive list elements per lane, built once per tile
+        u32 xw[RPL][XD];
         int xoff[RPL];
         const u64 b0 = bx * (u64)(64 * RPL) + (u64)lane * RPL;
+        const u64 m0 = (b0 < R.n_lane) ? colex_mask(b0, lk, lm, a.binom, a.bk) : 0ull;
         {
-            u64 m = (b0 < R.n_lane) ? colex_mask(b0, lk, lm, a.binom, a.bk) : 0ull;
+            u64 m = m0;
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
                 const bool valid = b0 + k < R.n_lane;
 #pragma unroll
-                for (int f = 0; f < FD; ++f) xw[k][f] = 0;
+                for (int f = 0; f < XD; ++f) xw[k][f] = 0;
                 if (valid) {
                     u64 mm = m;
                     while (mm) {
                         int b = __builtin_ctzll(mm);
                         mm &= mm - 1;
 #pragma unroll
-                        for (int f = 0; f < FD; ++f) xw[k][f] ^= rows_fast[(lbase + b) * FD + f];
+                        for (int f = 0; f < XD; ++f) xw[k][f] ^= rows_fast[(lbase + b) * XD + f];
                     }
                     xoff[k] = __popcll(m & lflags);
                     if (b0 + k + 1 < R.n_lane && lk > 0) m = colex_next(m);
@@ -172,83 +166,159 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 }
             }
         }
-        __syncthreads();   // stream chunk visible
 
-        // ---- hot loop: every (lane element, stream element) pair
-        for (int s = 0; s < len; ++s) {
-            const SElem<FD> e = stream[s];
-            int acc = kBigOffset;
+        const u64 ch0 = bg * (u64)a.G;
+        const u64 ch1 = min(ch0 + (u64)a.G, R.stream_chunks);
+        for (u64 ch = ch0; ch < ch1; ++ch) {
+            // ---- stream chunk -> LDS (each lane a contiguous run, colex successor steps)
+            const u64 c0 = ch * (u64)a.S;
+            const int len = (int)min((u64)a.S, R.n_stream - c0);
+            if (ch != ch0) __syncthreads();   // previous chunk's readers are done
+            const int per = (len + 63) >> 6;
+            {
+                const int i0 = lane * per;
+                const int i1 = min(len, i0 + per);
+                if (i0 < i1) {
+                    u64 m = colex_mask(c0 + (u64)i0, sk, sm, a.binom, a.bk);
+                    run_start[lane] = m;
+                    for (int i = i0; i < i1; ++i) {
+                        SElem<XD> e;
 #pragma unroll
-            for (int k = 0; k < RPL; ++k) {
-                int tk = xoff[k];
+                        for (int f = 0; f < XD; ++f) e.w[f] = fw[f];
+                        u64 mm = m;
+                        while (mm) {
+                            int b = __builtin_ctzll(mm);
+                            mm &= mm - 1;
 #pragma unroll
-                for (int f = 0; f < FD; ++f) tk += __popc(xw[k][f] ^ e.w[f]);
-                acc = min(acc, tk);
-            }
-            const int thr = bound - e.off;
-            if (__builtin_expect(__ballot(acc < thr) != 0ull, 0)) {
-                // ---- slow path: exact weight and logical test of the light candidates
-                const u64 smask = colex_mask(c0 + (u64)s, sk, sm, a.binom, a.bk);
-                if (lane < RW) {
-                    u32 v = 0;
-                    u64 mm = smask;
-                    while (mm) {
-                        int b = __builtin_ctzll(mm);
-                        mm &= mm - 1;
-                        v ^= gtbl[(sbase + b) * RW + lane];
+                            for (int f = 0; f < XD; ++f) e.w[f] ^= rows_fast[(sbase + b) * XD + f];
+                        }
+                        e.off = foff + __popcll(m & sflags);
+                        stream[i] = e;
+                        if (i + 1 < i1) m = colex_next(m);
                     }
-                    sfull[lane] = v;
                 }
-                __syncthreads();
-                u32 cand = 0;
+            }
+            __syncthreads();   // stream chunk visible
+
+            // ---- hot loop: every (lane element, stream element) pair
+            u32 nw[FD];
+            int noff = 0;
+#pragma unroll
+            for (int f = 0; f < FD; ++f) nw[f] = stream[0].w[f];
+            noff = stream[0].off;
+            for (int s = 0; s < len; ++s) {
+                u32 ew[FD];
+#pragma unroll
+                for (int f = 0; f < FD; ++f) ew[f] = nw[f];
+                const int eoff = noff;
+                if (s + 1 < len) {   // next element's LDS read overlaps this step's VALU work
+#pragma unroll
+                    for (int f = 0; f < FD; ++f) nw[f] = stream[s + 1].w[f];
+                    noff = stream[s + 1].off;
+                }
+                int acc = kBigOffset;
 #pragma unroll
                 for (int k = 0; k < RPL; ++k) {
                     int tk = xoff[k];
 #pragma unroll
-                    for (int f = 0; f < FD; ++f) tk += __popc(xw[k][f] ^ e.w[f]);
-                    if (tk < thr) cand |= 1u << k;
+                    for (int f = 0; f < FD; ++f) tk += __popc(xw[k][f] ^ ew[f]);
+                    acc = min(acc, tk);
                 }
-                int mine = QSF_WEIGHT_NONE;
-                while (cand) {
-                    const int k = __builtin_ctz(cand);
-                    cand &= cand - 1;
-                    const u64 lmask = colex_mask(b0 + k, lk, lm, a.binom, a.bk);
-                    int w = __popcll(lmask & lflags) + e.off;
-                    for (int f = 0; f < RD && w < bound; ++f) {
-                        u32 v = sfull[f];
-                        u64 mm = lmask;
+                const int thr = bound - eoff;
+                if (__builtin_expect(__ballot(acc < thr) != 0ull, 0)) {
+                    // ---- medium test: all XD register dwords
+                    u32 ex[XD];
+#pragma unroll
+                    for (int f = 0; f < XD; ++f) ex[f] = stream[s].w[f];
+                    u32 cand = 0;
+#pragma unroll
+                    for (int k = 0; k < RPL; ++k) {
+                        int tk = xoff[k];
+#pragma unroll
+                        for (int f = 0; f < XD; ++f) tk += __popc(xw[k][f] ^ ex[f]);
+                        if (tk < thr) cand |= 1u << k;
+                    }
+                    ++medium_entries;
+                    if (__ballot(cand != 0) == 0ull) continue;
+                    // ---- slow path: exact weight and logical test of the light candidates
+                    ++slow_entries;
+                    const int owner = s / per;
+                    u64 smask = run_start[owner];
+                    for (int j = owner * per; j < s; ++j) smask = colex_next(smask);
+                    if (lane < RW) {
+                        u32 v = fx ? fx[lane] : 0u;
+                        u64 mm = smask;
                         while (mm) {
                             int b = __builtin_ctzll(mm);
                             mm &= mm - 1;
-                            v ^= gtbl[(lbase + b) * RW + f];
+                            v ^= slow_tbl[(sbase + b) * RW + lane];
                         }
-                        w += __popc(v);
+                        sfull[lane] = v;
                     }
-                    if (w < bound && w < mine) {
-                        bool logical = (SD == 0);
-                        for (int f = 0; f < SD && !logical; ++f) {
-                            u32 v = sfull[RD + f];
+                    __syncthreads();
+                    int mine = QSF_WEIGHT_NONE;
+                    while (cand) {
+                        const int k = __builtin_ctz(cand);
+                        cand &= cand - 1;
+                        u64 lmask = m0;
+                        for (int j = 0; j < k; ++j) lmask = colex_next(lmask);
+                        int w = __popcll(lmask & lflags) + eoff;
+                        for (int f = 0; f < RD && w < bound; ++f) {
+                            u32 v = sfull[f];
                             u64 mm = lmask;
                             while (mm) {
                                 int b = __builtin_ctzll(mm);
                                 mm &= mm - 1;
-                                v ^= gtbl[(lbase + b) * RW + RD + f];
+                                v ^= slow_tbl[(lbase + b) * RW + f];
                             }
-                            logical = v != 0;
+                            w += __popc(v);
                         }
-                        if (logical) mine = w;
+                        if (a.collect) {
+                            if (w >= 1 && w < bound) {   // bound = keep + 1
+                                const unsigned long long slot = atomicAdd(a.out_rows, 1ull);
+                                if (slot < a.out_cap) {
+                                    const u64* G = a.gamma + (size_t)g * K * a.stride;
+                                    for (int wd = 0; wd < a.stride; ++wd) {
+                                        u64 v = 0;
+                                        for (u64 mm = R.f_lo; mm; mm &= mm - 1) v ^= G[(size_t)__builtin_ctzll(mm) * a.stride + wd];
+                                        for (u64 mm = R.f_hi; mm; mm &= mm - 1) v ^= G[(size_t)(64 + __builtin_ctzll(mm)) * a.stride + wd];
+                                        for (u64 mm = smask; mm; mm &= mm - 1) v ^= G[(size_t)(sbase + __builtin_ctzll(mm)) * a.stride + wd];
+                                        for (u64 mm = lmask; mm; mm &= mm - 1) v ^= G[(size_t)(lbase + __builtin_ctzll(mm)) * a.stride + wd];
+                                        a.out[slot * a.stride + wd] = v;
+                                    }
+                                }
+                            }
+                            continue;
+                        }
+                        if (w < bound && w < mine) {
+                            bool logical = (SD == 0);
+                            for (int f = 0; f < SD && !logical; ++f) {
+                                u32 v = sfull[RD + f];
+                                u64 mm = lmask;
+                                while (mm) {
+                                    int b = __builtin_ctzll(mm);
+                                    mm &= mm - 1;
+                                    v ^= slow_tbl[(lbase + b) * RW + RD + f];
+                                }
+                                logical = v != 0;
+                            }
+                            if (logical) mine = w;
+                        }
                     }
+                    const int wm = __builtin_amdgcn_readfirstlane(wave_min(mine));
+                    if (!a.collect && wm < bound) {
+                        bound = wm;
+                        if (lane == 0) atomicMin(a.best, wm);
+                    }
+                    __syncthreads();
                 }
-                const int wm = __builtin_amdgcn_readfirstlane(wave_min(mine));
-                if (wm < bound) {
-                    bound = wm;
-                    if (lane == 0) atomicMin(a.best, wm);
-                }
-                __syncthreads();
             }
+            if (bound <= a.floor) break;
         }
         if (bound <= a.floor) break;
     }
+    if (lane == 0 && slow_entries) atomicAdd(a.slow_hits, (unsigned long long)slow_entries);
+    if (lane == 0 && medium_entries) atomicAdd(a.slow_hits + 1, (unsigned long long)medium_entries);
 }
 
 // =====================================================================================
@@ -377,7 +447,10 @@ u64 sat_mul(u64 a, u64 b) {
 struct Solve {
     u64 key = 0;
     int n = 0, stride = 0, K = 0, ngt = 0, kcheck = 0;
-    int h = 0, RD = 0, SD = 0, RW = 0, bk = 0;
+    int h = 0, RD = 0, SD = 0, RW = 0, bk = 0, maxres = 0;
+    std::vector<int> nflag;       // flagged rows per generator
+    std::vector<u32> htbl;        // host copy of the residual+syndrome table
+    std::vector<u64> hflags;      // host copy of the flag masks
     std::vector<u64> binom;       // (K+1) x bk
     DevBuf<u64> gamma, check, dbinom, flags;
     DevBuf<u32> tbl;
@@ -397,8 +470,9 @@ struct Engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
     DevBuf<Rect> rects;
+    DevBuf<u32> fixed;
     DevBuf<int> best;
-    DevBuf<unsigned long long> counter;
+    DevBuf<unsigned long long> counter;   // [0] tile counter, [1] slow-path entries
     DevBuf<u64> out;
     qsf_hip_stats stats{};
     std::string err;
@@ -469,8 +543,20 @@ struct Engine {
                 if (cnt[j] == 1 && !flagged[g][who[j]]) flagged[g][who[j]] = 1;
                 else resid[g].push_back(j);
             }
+            // Densest-first residual order: a column with c ones among the K rows is 1 in
+            // the XOR of d random rows with probability (1 - (1 - 2c/K)^d) / 2, closest to
+            // 1/2 when c is near K/2. Those columns go into the first (fast-test) dwords, so
+            // the fast test rejects as many codewords as a test on fair coins would. The
+            // weight is a column-order invariant, so results do not depend on the order.
+            std::stable_sort(resid[g].begin(), resid[g].end(), [&](int x, int y) {
+                return std::abs(K - 2 * cnt[x]) < std::abs(K - 2 * cnt[y]);
+            });
             maxres = std::max(maxres, (int)resid[g].size());
         }
+        s.maxres = maxres;
+        s.nflag.assign(s.ngt, 0);
+        for (int g = 0; g < s.ngt; ++g)
+            for (int r = 0; r < K; ++r) s.nflag[g] += flagged[g][r];
         s.RD = std::max(1, (maxres + 31) / 32);
         s.SD = (p.kcheck + 31) / 32;
         s.RW = s.RD + s.SD;
@@ -491,10 +577,8 @@ struct Engine {
                     for (int w = 0; w < st; ++w) acc ^= ch[w] & row[w];
                     if (__builtin_popcountll(acc) & 1) out[s.RD + (c >> 5)] |= 1u << (c & 31);
                 }
-                if (flagged[g][r]) {
-                    if (r < s.h) fl[2 * g] |= 1ull << r;
-                    else fl[2 * g + 1] |= 1ull << (r - s.h);
-                }
+                // two mask words per generator cover rows 0..127, the tiled kernel's K range
+                if (flagged[g][r] && r < kMaxTiledK) fl[2 * g + (r >> 6)] |= 1ull << (r & 63);
             }
         }
         s.gamma.reserve((size_t)s.ngt * K * st);
@@ -509,6 +593,8 @@ struct Engine {
         s.flags.reserve(fl.size());
         HIPCHK(hipMemcpyAsync(s.flags.p, fl.data(), fl.size() * 8, hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));   // host vectors above go out of scope
+        s.htbl = std::move(tbl);
+        s.hflags = std::move(fl);
     }
 
     // ---- min mode -------------------------------------------------------------------
@@ -520,9 +606,9 @@ struct Engine {
         if (per_g == 0 || p.num_gamma == 0) { *best_out = p.current_best; return 0; }
         if (stats.codewords > (1ull << 62)) throw HipError{4, "level has more than 2^62 codewords"};
         best.reserve(1);
-        counter.reserve(1);
+        counter.reserve(4);
         HIPCHK(hipMemcpyAsync(best.p, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), stream));
+        HIPCHK(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned long long), stream));
 
         const bool can_tile = p.K <= kMaxTiledK && s.RW <= kMaxRW && p.d <= 32;
         const bool tiled = can_tile && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
@@ -530,8 +616,11 @@ struct Engine {
         else launch_sweep(s, p, false, 0, 0);
 
         int result = 0;
+        unsigned long long ctr[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(&result, best.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(ctr, counter.p, sizeof(ctr), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
+        if (tiled) { stats.slow_hits = ctr[1]; stats.medium_hits = ctr[2]; }
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         stats.kernel_ms = ms;
@@ -539,48 +628,189 @@ struct Engine {
         return 0;
     }
 
-    void launch_tiled(Solve& s, const qsf_enum_plan& p, int floor) {
-        const int K = p.K, d = p.d, h = s.h;
-        const int FD = s.RD >= 2 ? 2 : 1;
-        const int RPL = 16;
-        const int S = kStreamMax;
-        const u64 lane_tile = 64ull * RPL;
+    // Kernel variant: FD residual dwords tested in the hot loop, XD held for the medium
+    // test. The hot-loop test is chosen so that on fair-coin residual bits at most ~5% of
+    // stream steps would reach the medium test; the medium test (2 FD dwords when
+    // available) absorbs the extra light codewords of structured codes (stabilizer
+    // elements), which would otherwise reach the slow path. thr = bound - (flagged rows
+    // that any d-subset must contain).
+    struct Variant { int fd, xd, rpl; };
+    static double binom_cdf_below(int bits, long thr) {   // P(Binomial(bits, 1/2) < thr)
+        if (thr <= 0) return 0.0;
+        if (thr > bits) return 1.0;
+        double pmf = std::pow(0.5, bits), cdf = 0.0;
+        for (long k = 0; k < thr; ++k) { cdf += pmf; pmf = pmf * (bits - k) / (k + 1); }
+        return cdf;
+    }
+    static Variant choose_variant(const Solve& s, const qsf_enum_plan& p) {
+        static const Variant table[] = {{1, 1, 32}, {1, 2, 16}, {2, 2, 16}, {2, 4, 12}, {3, 4, 12}, {4, 4, 12},
+                                        {1, 2, 24}, {1, 2, 32}};
+        if (const char* env = std::getenv("QSA_TILED_VARIANT")) {   // tuning: "fd,xd,rpl"
+            int f = 0, x = 0, r = 0;
+            if (std::sscanf(env, "%d,%d,%d", &f, &x, &r) == 3)
+                for (const Variant& v : table)
+                    if (v.fd == f && v.xd == x && v.rpl == r && (x <= s.RD || x == 4)) return v;
+        }
+        if (const char* env = std::getenv("QSA_TILED_FD")) {   // tuning/testing override
+            int f = std::atoi(env);
+            if (f == 1) return s.RD == 1 ? table[0] : table[7];
+            if (f == 2) return s.RD == 2 ? table[2] : table[3];
+            if (f == 3) return table[4];
+            if (f == 4) return table[5];
+        }
+        if (s.RD == 1) return table[0];
+        int min_off = p.d;
+        for (int g = 0; g < p.num_gamma; ++g) min_off = std::min(min_off, std::max(0, p.d - (p.K - s.nflag[g])));
+        const long thr = (long)p.current_best - min_off;
+        const int res = std::max(1, s.maxres);
+        auto rate = [&](int fd, int rpl) { return 64.0 * rpl * binom_cdf_below(std::min(32 * fd, res), thr); };
+        if (rate(1, 32) < 0.05) return table[7];   // {1, 2, 32}
+        if (s.RD == 2) return table[2];
+        if (rate(2, 12) < 0.05) return table[3];
+        if (s.RD >= 3 && rate(3, 12) < 0.05) return table[4];
+        return table[5];
+    }
+
+    // Decomposition of one level into products  F + subsets(L, lk) x subsets(Sr, sk).
+    // The level's d-subsets of [0, K) split by how many rows fall in [0, h) and [h, K).
+    // A product whose shorter list has fewer than kExpandBelow elements would rebuild the
+    // 64 x RPL register tile for only a few stream steps; each element of that short list
+    // is then made a fixed row set F and the long list's range is cut in two again,
+    // which turns it into balanced products. Exact cover: every d-subset lies in exactly
+    // one product.
+    struct RectBuilder {
+        const Solve& s;
+        int g, RW, S;
+        u64 lane_tile;
         std::vector<Rect> rv;
-        u64 tiles = 0;
-        for (int g = 0; g < p.num_gamma; ++g)
-            for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka) {
-                const int kb = d - ka;
-                const u64 na = s.C(h, ka), nb = s.C(K - h, kb);
-                if (na == 0 || nb == 0) continue;
-                Rect r{};
-                r.g = g; r.ka = ka; r.kb = kb;
-                r.lanes_on_a = na >= nb ? 1 : 0;
-                r.n_lane = r.lanes_on_a ? na : nb;
-                r.n_stream = r.lanes_on_a ? nb : na;
-                r.lane_blocks = (r.n_lane + lane_tile - 1) / lane_tile;
-                r.stream_chunks = (r.n_stream + S - 1) / S;
-                r.tile_base = tiles;
-                tiles += r.lane_blocks * r.stream_chunks;
-                rv.push_back(r);
+        std::vector<u32> fixed;
+        static constexpr u64 kExpandBelow = 64;
+
+        void emit(u64 f_lo, u64 f_hi, int xb, int xm, int xk, int yb, int ym, int yk, u64 nx, u64 ny) {
+            Rect r{};
+            r.g = g;
+            const bool x_lanes = nx >= ny;
+            r.lbase = x_lanes ? xb : yb; r.lm = x_lanes ? xm : ym; r.lk = x_lanes ? xk : yk;
+            r.sbase = x_lanes ? yb : xb; r.sm = x_lanes ? ym : xm; r.sk = x_lanes ? yk : xk;
+            r.n_lane = x_lanes ? nx : ny;
+            r.n_stream = x_lanes ? ny : nx;
+            r.lane_blocks = (r.n_lane + lane_tile - 1) / lane_tile;
+            r.stream_chunks = (r.n_stream + S - 1) / S;
+            r.fixed = -1;
+            r.f_lo = f_lo;
+            r.f_hi = f_hi;
+            if (f_lo | f_hi) {
+                r.fixed = (int)(fixed.size() / (RW + 1));
+                const size_t at = fixed.size();
+                fixed.resize(at + RW + 1, 0);
+                int off = 0;
+                for (int row = 0; row < s.K; ++row) {
+                    const bool in = row < 64 ? (f_lo >> row) & 1 : (f_hi >> (row - 64)) & 1;
+                    if (!in) continue;
+                    const u32* src = s.htbl.data() + ((size_t)g * s.K + row) * RW;
+                    for (int w = 0; w < RW; ++w) fixed[at + w] ^= src[w];
+                    off += (int)((s.hflags[2 * g + (row >> 6)] >> (row & 63)) & 1);
+                }
+                fixed[at + RW] = (u32)off;
             }
+            rv.push_back(r);
+        }
+
+        void split(u64 f_lo, u64 f_hi, int xb, int xm, int xk, int yb, int ym, int yk, int depth) {
+            const u64 nx = s.C(xm, xk), ny = s.C(ym, yk);
+            if (nx == 0 || ny == 0) return;
+            const bool x_small = nx <= ny;
+            const u64 ns = x_small ? nx : ny;
+            const int bb = x_small ? yb : xb, bm = x_small ? ym : xm, bk = x_small ? yk : xk;
+            if (ns >= kExpandBelow || bk == 0 || bm < 2 || depth >= 3 || sat_mul(nx, ny) < (1ull << 22)) {
+                emit(f_lo, f_hi, xb, xm, xk, yb, ym, yk, nx, ny);
+                return;
+            }
+            const int sb = x_small ? xb : yb, sm = x_small ? xm : ym, sk = x_small ? xk : yk;
+            const int mid = bm / 2;
+            u64 m = sk == 0 ? 0 : (sk >= 64 ? ~0ull : ((1ull << sk) - 1));
+            for (u64 e = 0; e < ns; ++e) {
+                u64 lo = f_lo, hi = f_hi;
+                for (u64 mm = m; mm; mm &= mm - 1) {
+                    const int row = sb + __builtin_ctzll(mm);
+                    if (row < 64) lo |= 1ull << row; else hi |= 1ull << (row - 64);
+                }
+                for (int j = std::max(0, bk - (bm - mid)); j <= std::min(bk, mid); ++j)
+                    split(lo, hi, bb, mid, j, bb + mid, bm - mid, bk - j, depth + 1);
+                if (e + 1 < ns) {   // next sk-subset of [sb, sb + sm) in colex order
+                    const u64 c = m & (~m + 1ull), r = m + c;
+                    m = (((r ^ m) >> 2) >> __builtin_ctzll(c)) | r;
+                }
+            }
+        }
+    };
+
+    std::vector<Rect> h_rects;   // host staging, alive until the copies complete
+    std::vector<u32> h_fixed;
+
+    void launch_tiled(Solve& s, const qsf_enum_plan& p, int floor, bool collect = false, int keep = 0, u64 cap = 0) {
+        const int K = p.K, d = p.d, h = s.h;
+        qsf_enum_plan pv = p;
+        if (collect) pv.current_best = keep + 1;
+        const Variant V = choose_variant(s, pv);
+        const int RPL = V.rpl;
+        const int S = kStreamMax;
+        RectBuilder B{s, 0, s.RW, S, 64ull * RPL, {}, {}};
+        for (int g = 0; g < p.num_gamma; ++g) {
+            B.g = g;
+            for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
+                B.split(0, 0, 0, h, ka, h, K - h, d - ka, 0);
+        }
+        h_rects = std::move(B.rv);
+        h_fixed = std::move(B.fixed);
+        std::vector<Rect>& rv = h_rects;
+        u64 chunks = 0;
+        for (const Rect& r : rv) chunks += r.lane_blocks * r.stream_chunks;
+        // chunks per tile: large enough to amortise the register tile, small enough to
+        // leave >= 8 tiles per resident wave for load balance
+        const u64 waves = (u64)std::max(1, cus) * 16;
+        const int G = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * waves)));
+        u64 tiles = 0;
+        for (Rect& r : rv) {
+            r.stream_groups = (r.stream_chunks + G - 1) / G;
+            r.tile_base = tiles;
+            tiles += r.lane_blocks * r.stream_groups;
+        }
         rects.reserve(rv.size());
         HIPCHK(hipMemcpyAsync(rects.p, rv.data(), rv.size() * sizeof(Rect), hipMemcpyHostToDevice, stream));
+        fixed.reserve(std::max<size_t>(1, h_fixed.size()));
+        if (!h_fixed.empty())
+            HIPCHK(hipMemcpyAsync(fixed.p, h_fixed.data(), h_fixed.size() * 4, hipMemcpyHostToDevice, stream));
         TiledArgs a{};
-        a.tbl = s.tbl.p; a.flags = s.flags.p; a.rects = rects.p; a.nrects = (int)rv.size();
+        a.tbl = s.tbl.p; a.flags = s.flags.p; a.fixed = fixed.p; a.rects = rects.p; a.nrects = (int)rv.size();
         a.total_tiles = tiles; a.binom = s.dbinom.p; a.bk = s.bk;
-        a.K = K; a.h = h; a.RD = s.RD; a.SD = s.SD; a.RW = s.RW; a.S = S;
-        a.best = best.p; a.next_tile = counter.p; a.floor = floor;
-        const u64 grid = std::min<u64>(tiles, (u64)std::max(1, cus) * 16);
+        a.K = K; a.h = h; a.RD = s.RD; a.SD = s.SD; a.RW = s.RW; a.S = S; a.G = G;
+        a.best = best.p; a.next_tile = counter.p; a.slow_hits = counter.p + 1; a.floor = floor;
+        a.collect = collect ? 1 : 0; a.keep = keep; a.stride = p.stride; a.gamma = s.gamma.p;
+        a.out_rows = counter.p + 3; a.out = out.p; a.out_cap = cap;
+        const u64 grid = std::min<u64>(tiles, waves);
         stats.kernel = 2;
         stats.tiles = (int)std::min<u64>(tiles, 0x7fffffff);
+        stats.fd = V.fd;
+        stats.xd = V.xd;
         HIPCHK(hipEventRecord(ev0, stream));
         if (grid > 0) {
-            if (FD == 1) hipLaunchKernelGGL((bz_tiled<1, 16>), dim3((unsigned)grid), dim3(64), 0, stream, a);
-            else hipLaunchKernelGGL((bz_tiled<2, 16>), dim3((unsigned)grid), dim3(64), 0, stream, a);
+            const dim3 gd((unsigned)grid), bd(64);
+            const size_t lds = (size_t)(p.K * s.RW <= kLdsFullRows ? p.K * s.RW : 0) * sizeof(u32);
+            const int code = V.fd * 1000 + V.xd * 100 + V.rpl;
+            switch (code) {
+                case 1132: hipLaunchKernelGGL((bz_tiled<1, 1, 32>), gd, bd, lds, stream, a); break;
+                case 1216: hipLaunchKernelGGL((bz_tiled<1, 2, 16>), gd, bd, lds, stream, a); break;
+                case 1224: hipLaunchKernelGGL((bz_tiled<1, 2, 24>), gd, bd, lds, stream, a); break;
+                case 1232: hipLaunchKernelGGL((bz_tiled<1, 2, 32>), gd, bd, lds, stream, a); break;
+                case 2216: hipLaunchKernelGGL((bz_tiled<2, 2, 16>), gd, bd, lds, stream, a); break;
+                case 2412: hipLaunchKernelGGL((bz_tiled<2, 4, 12>), gd, bd, lds, stream, a); break;
+                case 3412: hipLaunchKernelGGL((bz_tiled<3, 4, 12>), gd, bd, lds, stream, a); break;
+                default: hipLaunchKernelGGL((bz_tiled<4, 4, 12>), gd, bd, lds, stream, a); break;
+            }
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(ev1, stream));
-        (void)RPL;
     }
 
     void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap) {
@@ -594,7 +824,7 @@ struct Engine {
         u64 chunk = std::max<u64>(64, (a.total * (u64)p.num_gamma + target_threads - 1) / target_threads);
         a.chunk = chunk;
         a.best = best.p; a.collect = collect ? 1 : 0; a.keep = keep;
-        a.out_rows = counter.p; a.out = out.p; a.out_cap = cap;
+        a.out_rows = counter.p + 3; a.out = out.p; a.out_cap = cap;
         const u64 threads = ((a.total + chunk - 1) / chunk) * (u64)p.num_gamma;
         const u64 blocks = (threads + 255) / 256;
         stats.kernel = 1;
@@ -617,14 +847,17 @@ struct Engine {
         if (per_g == 0 || p.num_gamma == 0) return 0;
         if (stats.codewords > (1ull << 62)) throw HipError{4, "level has more than 2^62 codewords"};
         best.reserve(1);
-        counter.reserve(1);
+        counter.reserve(4);
+        const bool can_tile = p.K <= kMaxTiledK && s.RW <= kMaxRW && p.d <= 32;
+        const bool tiled = can_tile && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
         u64 cap = 1 << 16;
         for (;;) {
             out.reserve(cap * (u64)p.stride);
-            HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), stream));
-            launch_sweep(s, p, true, keep, cap);
+            HIPCHK(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned long long), stream));
+            if (tiled) launch_tiled(s, p, -1, true, keep, cap);
+            else launch_sweep(s, p, true, keep, cap);
             unsigned long long got = 0;
-            HIPCHK(hipMemcpyAsync(&got, counter.p, sizeof(got), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(&got, counter.p + 3, sizeof(got), hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
             if (got > cap) { cap = got; continue; }   // overflow: rerun with room for all
             u64* h = (u64*)std::malloc(std::max<u64>(1, got) * p.stride * 8);

@@ -36,22 +36,30 @@ constexpr int kMaxTiledK = 128;     // both halves must fit a 64-bit subset mask
 constexpr int kMaxD = 64;           // binomial table columns
 constexpr int kStreamMax = 512;     // LDS stream chunk capacity (elements)
 constexpr int kMaxRW = 64;          // residual + syndrome dwords per row in the tiled kernel
+constexpr int kLdsFullRows = 1024;  // dwords: the whole K x RW table is kept in LDS up to this size
 constexpr int kBigOffset = (1 << 30) + (1 << 29);  // masks lanes past the end of a list
 
-// One complete product subsets(A, ka) x subsets(B, kb) of one generator.
+// One complete product  F + subsets(L, lk) x subsets(Sr, sk)  of one generator: L and Sr
+// are disjoint row ranges of at most 64 rows, F a fixed set of rows outside both (its
+// codeword is XORed into every stream element). The register tile holds subsets of L,
+// the stream subsets of Sr.
 struct Rect {
     int g;
-    int ka, kb;
-    int lanes_on_a;        // 1: the register tile holds subsets of A, the stream subsets of B
-    u64 n_lane, n_stream;  // list lengths on each side
+    int lbase, lm, lk;     // lane list: lk-subsets of rows [lbase, lbase + lm)
+    int sbase, sm, sk;     // stream list: sk-subsets of rows [sbase, sbase + sm)
+    int fixed;             // index into TiledArgs::fixed, or -1 for no fixed rows
+    u64 f_lo, f_hi;        // the fixed rows as a 128-bit row mask
+    u64 n_lane, n_stream;  // list lengths C(lm, lk), C(sm, sk)
     u64 lane_blocks;       // ceil(n_lane / (64 * RPL))
     u64 stream_chunks;     // ceil(n_stream / S)
+    u64 stream_groups;     // ceil(stream_chunks / G): a tile is one lane block x G chunks
     u64 tile_base;         // index of this rectangle's first tile
 };
 
 struct TiledArgs {
     const u32* tbl;          // [g][row][RW]: RD residual dwords, then SD syndrome dwords
-    const u64* flags;        // [g][2]: flagged rows of A (bit i = row i) and of B (bit i = row h+i)
+    const u64* flags;        // [g][2]: flagged rows 0..63 and 64..127 as bit masks
+    const u32* fixed;        // [f][RW + 1]: residual+syndrome of a fixed row set, then its flagged count
     const Rect* rects;
     int nrects;
     u64 total_tiles;
@@ -59,9 +67,17 @@ struct TiledArgs {
     int bk;
     int K, h, RD, SD, RW;
     int S;                   // stream chunk length (<= kStreamMax)
+    int G;                   // stream chunks per tile (the register tile is built once per tile)
     int* best;               // running bound; starts at current_best
     unsigned long long* next_tile;
+    unsigned long long* slow_hits;   // slow-path entries (diagnostics)
     int floor;               // stop once best <= floor (floor < 0: never)
+    // collect mode: every codeword of weight in [1, keep] is written out in full
+    int collect, keep, stride;
+    const u64* gamma;        // [g][row][stride] full rows
+    unsigned long long* out_rows;
+    u64* out;
+    u64 out_cap;             // rows
 };
 
 struct SweepArgs {

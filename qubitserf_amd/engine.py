@@ -99,7 +99,8 @@ class kernel_policy:
 def last_stats() -> dict:
     s = _native.HipStats()
     _native.load().qsf_hip_last_stats(ctypes.byref(s))
-    return dict(codewords=int(s.codewords), kernel_ms=float(s.kernel_ms), kernel=int(s.kernel), tiles=int(s.tiles))
+    return dict(codewords=int(s.codewords), kernel_ms=float(s.kernel_ms), kernel=int(s.kernel), tiles=int(s.tiles),
+                fd=int(s.fd), xd=int(s.xd), slow_hits=int(s.slow_hits), medium_hits=int(s.medium_hits))
 
 
 def pack_rows(M: np.ndarray) -> np.ndarray:

@@ -99,13 +99,74 @@ def test_key_reuse_across_levels(hip):
             assert got == want, (trial, d, got, want)
 
 
-def test_collect_matches_oracle(hip):
+def _sorted_rows(a, stride):
+    a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, stride)
+    if a.shape[0] == 0:
+        return a
+    return a[np.lexsort(a.T[::-1])]
+
+
+def _check_collect(hip, gamma, n, d, keep):
+    got = hip.collect_level(gamma, n, d, keep)
+    want = O.collect_plan(gamma, n, d, keep)
+    stride = gamma.shape[2]
+    g, w = _sorted_rows(got, stride), _sorted_rows(want, stride)
+    assert g.shape == w.shape and np.array_equal(g, w), (gamma.shape, d, keep, g.shape, w.shape)
+    return g.shape[0]
+
+
+@pytest.mark.parametrize("policy", ["sweep", "tiled"])
+def test_collect_matches_oracle(hip, policy):
+    """Collect mode: the multiset of emitted codewords equals the oracle's (each
+    combination exactly once, duplicates across generators kept)."""
     rng = np.random.default_rng(11)
-    for K, stride, d, keep in [(12, 1, 3, 30), (20, 2, 3, 50), (30, 1, 2, 28), (16, 3, 4, 80)]:
-        gamma, _ = rand_plan(rng, K, stride, 1, 0, systematic=True)
-        n = stride * 64
-        got = hip.collect_level(gamma, n, d, keep)
-        want = O.collect_plan(gamma, n, d, keep)
-        g = np.array(sorted(map(tuple, got.tolist())), dtype=np.uint64).reshape(-1, stride)
-        w = np.array(sorted(map(tuple, want.tolist())), dtype=np.uint64).reshape(-1, stride)
-        assert g.shape == w.shape and np.array_equal(g, w), (K, d, keep, g.shape, w.shape)
+    with hip.kernel_policy(policy):
+        for K, stride, d, keep, ng in [(12, 1, 3, 30, 1), (20, 2, 3, 50, 2), (30, 1, 2, 28, 1), (16, 3, 4, 80, 1),
+                                       (40, 2, 4, 50, 2)]:
+            gamma, _ = rand_plan(rng, K, stride, ng, 0, systematic=True)
+            _check_collect(hip, gamma, stride * 64, d, keep)
+
+
+@pytest.mark.parametrize("K,d,stride,keep", [(64, 6, 2, 26), (90, 5, 3, 41), (120, 5, 3, 27)])
+def test_tiled_split_levels(hip, K, d, stride, keep):
+    """Levels whose skewed products are re-split with fixed rows: min mode against the
+    oracle, and collect mode as an exact-cover check (every light codeword of the level
+    appears exactly as often as the oracle emits it)."""
+    rng = np.random.default_rng(K * 100 + d)
+    gamma, check = rand_plan(rng, K, stride, 1, 2, systematic=True)
+    n = stride * 64
+    best = keep + 4
+    got = hip.enumerate_level(gamma, n, d, check=check, current_best=best)
+    assert hip.last_stats()["kernel"] == 2
+    want = O.enumerate_plan(gamma, n, d, check=check, current_best=best)
+    assert got == want, (K, d, got, want)
+    rows = _check_collect(hip, gamma, n, d, keep)
+    assert rows > 0
+
+
+@pytest.mark.parametrize("fd", [1, 2, 3, 4])
+def test_tiled_fast_width(hip, fd, monkeypatch):
+    """Every fast-test width of the tiled kernel (dwords tested before the exact slow path)."""
+    monkeypatch.setenv("QSA_TILED_FD", str(fd))
+    rng = np.random.default_rng(500 + fd)
+    with hip.kernel_policy("tiled"):
+        for i in range(6):
+            # stride 3 and K <= 60 leave >= 132 residual bits, so every width is available
+            K = int(rng.integers(30, 61)); stride = 3; d = int(rng.integers(3, 6))
+            kcheck = int(rng.integers(0, 3))
+            best = [NONE, 60, 35, 20, 12, 90][i]
+            run_case(hip, rng, K, stride, d, 1 + i % 2, kcheck, best, systematic=True)
+            assert hip.last_stats()["fd"] == fd
+
+
+def test_tiled_large_levels_with_groups(hip):
+    """Levels with many tiles, so several stream chunks share one register tile."""
+    rng = np.random.default_rng(77)
+    for K, d, best in [(64, 5, 30), (80, 5, 40), (100, 4, 50), (128, 4, 60)]:
+        stride = 3
+        gamma, check = rand_plan(rng, K, stride, 1, 2, systematic=True)
+        got = hip.enumerate_level(gamma, stride * 64, d, check=check, current_best=best)
+        st = hip.last_stats()
+        want = O.enumerate_plan(gamma, stride * 64, d, check=check, current_best=best)
+        assert got == want, (K, d, got, want, st)
+        assert st["kernel"] == 2
