@@ -80,12 +80,28 @@ const char* distfind_version(void);
 /* Text for the last nonzero return code of any distfind_* call (new; no reference counterpart). */
 const char* distfind_last_error(void);
 
-/* Per-level trace of the last distfind_* solve on this thread: up to `cap` rows of
- * ints8 = {side ('S','Z','X'), d, active sets, total sets, K, inner, outer, kernel
- * (0 none, 1 row sweep, 2 tiled)}, the codeword count, and ms2 = {wall ms, device kernel
- * ms} of each level. Returns the number of levels recorded. New (the reference prints
- * the same fields under DISTFIND_PROFILE, bz.cpp:147-155). */
-int distfind_last_trace(int32_t* ints8, uint64_t* codewords, double* ms2, int cap);
+/* One enumerated weight level of the last distfind_* solve on the calling thread (new; the
+ * reference prints the same fields under DISTFIND_PROFILE, cpp/distfind/bz.cpp:147-155). */
+typedef struct {
+    int32_t side;          /* 'S' single subproblem, 'Z' / 'X' interleaved CSS sides */
+    int32_t d;             /* weight level */
+    int32_t active;        /* information sets enumerated at this level */
+    int32_t ngamma;        /* information sets in total */
+    int32_t K;             /* rows per generator */
+    int32_t inner, outer;  /* upper and lower bound after the level */
+    int32_t kernel;        /* 0 none, 1 row sweep, 2 tiled */
+    int32_t fd, xd, rpl;   /* tiled kernel variant (hot-test dwords, held dwords, codewords/lane) */
+    int32_t pad;
+    uint64_t codewords;    /* C(K, d) x active */
+    uint64_t medium_hits;  /* stream steps past the hot-loop test */
+    uint64_t slow_hits;    /* stream steps into the exact slow path */
+    double ms;             /* wall time of the level */
+    double kernel_ms;      /* device time of the enumeration kernel (HIP events) */
+    double ops_per_codeword; /* hot-loop int32 VALU lane-ops per codeword of the variant */
+} DistfindLevel;
+
+/* Copies up to `cap` levels into `out` (may be NULL); returns the number recorded. */
+int distfind_last_trace(DistfindLevel* out, int cap);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,10 @@ typedef struct {
     uint64_t slow_hits;   /* tiled kernel: stream steps that entered the exact slow path */
     int xd;               /* tiled kernel: residual dwords held per codeword (medium test) */
     uint64_t medium_hits; /* tiled kernel: stream steps that passed the hot-loop test */
+    int rpl;              /* tiled kernel: codewords held per lane */
+    double ops_per_codeword; /* tiled kernel: hot-loop int32 VALU lane-ops per codeword,
+                                (2 FD RPL + RPL/2 + 3) / RPL: FD xor+bcnt pairs per codeword,
+                                a v_min3 per two codewords, 3 per stream step */
 } qsf_hip_stats;
 void qsf_hip_last_stats(qsf_hip_stats* out);
 

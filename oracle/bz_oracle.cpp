@@ -3,9 +3,9 @@
 // cpu_baseline leg as the CHECKER. Nothing in the product links, loads or calls it.
 //
 // Every function cites the reference file:line it restates (paths relative to the
-// reference repository root). The restatement is pinned by the checks listed in
-// oracle/README.md (golden vectors from the reference's own CPU library, brute force over
-// all 2^K messages, closed forms and textbook distances).
+// reference repository root). tests/test_cpu_oracle.py pins the restatement: golden
+// vectors from the reference's own CPU library, brute force over all 2^K messages, a
+// meet-in-the-middle search, closed forms and textbook distances.
 //
 // Build:  g++ -O3 -std=c++17 -fPIC -shared -pthread oracle/bz_oracle.cpp -o oracle/_build/liboracle.so
 #include <algorithm>

@@ -70,7 +70,16 @@ class HipStats(ctypes.Structure):
         ("slow_hits", ctypes.c_uint64),
         ("xd", ctypes.c_int),
         ("medium_hits", ctypes.c_uint64),
+        ("rpl", ctypes.c_int),
+        ("ops_per_codeword", ctypes.c_double),
     ]
+
+
+class DistfindLevel(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int32) for f in ("side", "d", "active", "ngamma", "K", "inner", "outer", "kernel",
+                                              "fd", "xd", "rpl", "pad")] + [
+        ("codewords", ctypes.c_uint64), ("medium_hits", ctypes.c_uint64), ("slow_hits", ctypes.c_uint64),
+        ("ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("ops_per_codeword", ctypes.c_double)]
 
 
 _lib = None
@@ -111,8 +120,7 @@ def load() -> ctypes.CDLL:
     sig("distfind_backend_available", c_int, [c_char_p])
     sig("distfind_version", c_char_p, [])
     sig("distfind_last_error", c_char_p, [])
-    sig("distfind_last_trace", c_int,
-        [ctypes.POINTER(ctypes.c_int32), u64p, ctypes.POINTER(ctypes.c_double), c_int])
+    sig("distfind_last_trace", c_int, [ctypes.POINTER(DistfindLevel), c_int])
 
     vp = ctypes.c_void_p
     sig("codeaut_bz_collect", vp,

@@ -167,17 +167,17 @@ const char* distfind_version(void) { return "0.1.0+gfx950"; }
 
 const char* distfind_last_error(void) { return g_err.c_str(); }
 
-int distfind_last_trace(int32_t* ints8, uint64_t* codewords, double* ms2, int cap) {
+int distfind_last_trace(DistfindLevel* out, int cap) {
     const int n = (int)g_trace.size();
-    for (int i = 0; i < n && i < cap; ++i) {
+    for (int i = 0; out && i < n && i < cap; ++i) {
         const LevelTrace& t = g_trace[i];
-        if (ints8) {
-            int32_t* o = ints8 + 8 * i;
-            o[0] = t.side; o[1] = t.d; o[2] = t.active; o[3] = t.ngamma; o[4] = t.K; o[5] = t.inner; o[6] = t.outer;
-            o[7] = t.kernel;
-        }
-        if (codewords) codewords[i] = t.codewords;
-        if (ms2) { ms2[2 * i] = t.ms; ms2[2 * i + 1] = t.kernel_ms; }
+        DistfindLevel& o = out[i];
+        std::memset(&o, 0, sizeof(o));
+        o.side = t.side; o.d = t.d; o.active = t.active; o.ngamma = t.ngamma; o.K = t.K;
+        o.inner = t.inner; o.outer = t.outer; o.kernel = t.st.kernel;
+        o.fd = t.st.fd; o.xd = t.st.xd; o.rpl = t.st.rpl;
+        o.codewords = t.st.codewords; o.medium_hits = t.st.medium_hits; o.slow_hits = t.st.slow_hits;
+        o.ms = t.ms; o.kernel_ms = t.st.kernel_ms; o.ops_per_codeword = t.st.ops_per_codeword;
     }
     return n;
 }

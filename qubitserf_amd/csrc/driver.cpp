@@ -106,9 +106,7 @@ struct Side {
             std::fprintf(stderr, "[prof hip] d=%d K=%d ng=%d/%d stride=%d work=%llu kernel=%d fd=%d/%d medium=%llu slow=%llu -> %.3f ms (device %.3f ms)\n",
                          d, K, act, ng, stride, (unsigned long long)st.codewords, st.kernel, st.fd, st.xd,
                          (unsigned long long)st.medium_hits, (unsigned long long)st.slow_hits, ms, st.kernel_ms);
-        if (opt->trace)
-            opt->trace->push_back(LevelTrace{*tag ? tag[0] : 'S', d, act, ng, K, (unsigned long long)st.codewords,
-                                             ms, st.kernel_ms, st.kernel, inner, outer});
+        if (opt->trace) opt->trace->push_back(LevelTrace{*tag ? tag[0] : 'S', d, act, ng, K, inner, outer, ms, st});
     }
 };
 

@@ -3,6 +3,7 @@
 #include <string>
 #include <vector>
 #include "problem.hpp"
+#include "../../include/qsf_hip.h"
 
 namespace qsa {
 
@@ -18,11 +19,9 @@ struct BzResult {
 struct LevelTrace {
     char side;             // 'S' single, 'Z'/'X' interleaved
     int d, active, ngamma, K;
-    unsigned long long codewords;
-    double ms;             // wall time of the level (host)
-    double kernel_ms;      // device time of the enumeration kernel (HIP events)
-    int kernel;            // 0 none, 1 row sweep, 2 tiled
     int inner, outer;      // bounds after the level
+    double ms;             // wall time of the level (host)
+    qsf_hip_stats st;      // engine statistics of the level (zero when nothing was enumerated)
 };
 
 struct BzOptions {

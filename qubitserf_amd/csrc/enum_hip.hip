@@ -793,6 +793,8 @@ struct Engine {
         stats.tiles = (int)std::min<u64>(tiles, 0x7fffffff);
         stats.fd = V.fd;
         stats.xd = V.xd;
+        stats.rpl = V.rpl;
+        stats.ops_per_codeword = (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
         HIPCHK(hipEventRecord(ev0, stream));
         if (grid > 0) {
             const dim3 gd((unsigned)grid), bd(64);

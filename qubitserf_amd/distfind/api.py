@@ -137,17 +137,17 @@ def version() -> str:
 
 
 def last_trace():
-    """Per-level records of the last solve on this thread: dicts with side, d, active,
-    ngamma, K, inner, outer, kernel, codewords, ms (wall) and kernel_ms (device)."""
+    """Per-level records of the last solve on this thread, as dicts with the fields of
+    DistfindLevel (include/distfind_capi.h): side, d, active, ngamma, K, inner, outer,
+    kernel, fd, xd, rpl, codewords, medium_hits, slow_hits, ms, kernel_ms, ops_per_codeword."""
     L = _native.load()
-    n = L.distfind_last_trace(None, None, None, 0)
-    ints = (ctypes.c_int32 * (8 * max(n, 1)))()
-    cw = (ctypes.c_uint64 * max(n, 1))()
-    ms = (ctypes.c_double * (2 * max(n, 1)))()
-    L.distfind_last_trace(ints, cw, ms, n)
+    n = L.distfind_last_trace(None, 0)
+    buf = (_native.DistfindLevel * max(n, 1))()
+    L.distfind_last_trace(buf, n)
     out = []
     for i in range(n):
-        s, d, act, ng, K, inner, outer, kern = ints[8 * i:8 * i + 8]
-        out.append(dict(side=chr(s), d=d, active=act, ngamma=ng, K=K, inner=inner, outer=outer, kernel=kern,
-                        codewords=int(cw[i]), ms=float(ms[2 * i]), kernel_ms=float(ms[2 * i + 1])))
+        r = buf[i]
+        rec = {f: getattr(r, f) for f, _ in _native.DistfindLevel._fields_ if f != "pad"}
+        rec["side"] = chr(rec["side"])
+        out.append(rec)
     return out

@@ -1,0 +1,268 @@
+"""Pins the oracle (the CPU restatement in oracle/bz_oracle.cpp) before it is trusted as
+the checker of the device engine:
+
+* against the reference's own answers (tests/golden/reference_cases.json, produced by the
+  reference CPU library compiled from its sources),
+* against brute force over all 2^K messages and an independent meet-in-the-middle search,
+* against textbook distances and the closed form of the BZ lower bound,
+* its level enumerator against a plain per-combination enumerator and a numpy one.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from qubitserf_amd import codes
+from tests import oracle as O
+from tests.golden.fixtures import dec, load
+
+GOLD = load()
+HW = GOLD["meta"]["hw_threads"]
+NONE = 1 << 30
+
+
+def rec(r):
+    return dict(distance=r.distance, lower_bound=r.lower_bound, proven=r.proven, levels=r.levels)
+
+
+# ---------------------------------------------------------------- golden vectors
+
+@pytest.mark.parametrize("case", GOLD["classical"], ids=[c["name"] for c in GOLD["classical"]])
+def test_classical_matches_reference(case):
+    # same random-information-set worker count as the reference run => same level trace
+    got = rec(O.classical_distance(dec(case["H"]), is_threads=HW))
+    assert got == case["ref"]
+
+
+@pytest.mark.parametrize("case", GOLD["css"], ids=[c["name"] for c in GOLD["css"]])
+def test_css_matches_reference(case):
+    Hx, Hz = dec(case["Hx"]), dec(case["Hz"])
+    for w in ("M", "Z", "X"):
+        assert rec(O.css_distance(Hx, Hz, which=w, is_threads=HW)) == case["ref"][w], w
+
+
+@pytest.mark.parametrize("case", GOLD["stabilizer"], ids=[c["name"] for c in GOLD["stabilizer"]])
+def test_stabilizer_matches_reference(case):
+    r = O.stabilizer_distance(dec(case["S"]), is_threads=HW)
+    assert (r.distance, r.proven) == (case["ref"]["distance"], case["ref"]["proven"])
+
+
+@pytest.mark.parametrize("case", GOLD["collect"], ids=[c["name"] for c in GOLD["collect"]])
+def test_collect_matches_reference(case):
+    """The reference dedups and sorts collected codewords; the oracle's level collector
+    plus the same dedup must give the same set."""
+    G = dec(case["G"])
+    m, n = G.shape
+    gamma = pack(G)[None]
+    seen = set()
+    for sw in range(1, min(case["p"], m) + 1):
+        for row in O.collect_plan(gamma, n, sw, case["keep"]):
+            seen.add(tuple(int(x) for x in row))
+    assert len(seen) == case["ref"]["count"]
+    if case["ref"]["rows"] is not None:
+        want = {tuple(int(x) for x in r) for r in pack(dec(case["ref"]["rows"]))}
+        assert seen == want
+
+
+# ---------------------------------------------------------------- brute force / MITM
+
+def pack(M):
+    from qubitserf_amd.engine import pack_rows
+    return pack_rows(M)
+
+
+def numpy_min_weight(gen, det=None):
+    """Min weight over nonzero messages (logical when det is given), by direct expansion."""
+    K, n = gen.shape
+    best = NONE
+    for bits in range(1, 1 << K):
+        msg = np.array([(bits >> i) & 1 for i in range(K)], dtype=np.uint8)
+        cw = (msg @ gen) % 2
+        if det is not None and not ((det @ cw) % 2).any():
+            continue
+        best = min(best, int(cw.sum()))
+    return best
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_bz_equals_bruteforce_and_mitm_classical(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(12, 26))
+    k = int(rng.integers(3, min(12, n - 2)))
+    H = codes.random_full_rank(rng, n - k, n)
+    bz = O.classical_distance(H).distance
+    assert bz == O.bruteforce_distance(0, H) == O.mitm_distance(0, H)
+
+
+@pytest.mark.parametrize("name,HxHz", [("steane", codes.steane()), ("shor", codes.shor()),
+                                       ("toric3", codes.toric(3)), ("surface3", codes.surface(3)),
+                                       ("surface4", codes.surface(4)), ("bb72", codes.bb_family(6))])
+def test_bz_equals_bruteforce_and_mitm_css(name, HxHz):
+    Hx, Hz = HxHz
+    for kind, w in ((1, "Z"), (2, "X")):
+        bz = O.css_distance(Hx, Hz, which=w).distance
+        assert bz == O.mitm_distance(kind, Hx, Hz, 16)
+        if O.problem_info(kind, Hx, Hz)["K"] <= 22:
+            assert bz == O.bruteforce_distance(kind, Hx, Hz)
+
+
+def test_bruteforce_is_independent_of_bz():
+    """The brute-force checker itself, on a code computed by hand: the [7,4] Hamming code
+    has 7 codewords of weight 3, so min weight 3; numpy expansion agrees."""
+    H = codes.hamming_parity(3)
+    assert O.bruteforce_distance(0, H) == 3
+    G = np.array([[1, 0, 0, 0, 0, 1, 1], [0, 1, 0, 0, 1, 0, 1], [0, 0, 1, 0, 1, 1, 0], [0, 0, 0, 1, 1, 1, 1]],
+                 dtype=np.uint8)
+    assert not ((H @ G.T) % 2).any()
+    assert numpy_min_weight(G) == 3
+
+
+# ---------------------------------------------------------------- textbook distances
+
+@pytest.mark.parametrize("L", range(3, 8))
+def test_toric_and_surface(L):
+    assert O.css_distance(*codes.toric(L)).distance == L
+    assert O.css_distance(*codes.surface(L)).distance == L
+
+
+@pytest.mark.parametrize("n", range(3, 10))
+def test_repetition(n):
+    assert O.classical_distance(codes.repetition_parity(n)).distance == n
+
+
+def test_small_known_codes():
+    assert O.classical_distance(codes.hamming_parity(3)).distance == 3
+    assert O.classical_distance(codes.hamming_parity(4)).distance == 3
+    ext = np.vstack([np.hstack([codes.hamming_parity(3), np.zeros((3, 1), np.uint8)]), np.ones((1, 8), np.uint8)])
+    assert O.classical_distance(ext).distance == 4           # extended Hamming [8,4,4]
+    assert O.css_distance(*codes.steane()).distance == 3
+    assert O.css_distance(*codes.shor()).distance == 3
+    assert O.css_distance(*codes.bb_family(6)).distance == 6  # [[72,12,6]]
+    for m in (5, 6, 7):
+        assert O.css_distance(*codes.quantum_reed_muller(1, m)).distance == 4
+
+
+def test_five_qubit_code():
+    z = np.array([0, 1, 1, 0, 0]); x = np.array([1, 0, 0, 1, 0])
+    S = np.array([np.concatenate([np.roll(z, s), np.roll(x, s)]) for s in range(4)], dtype=np.uint8)
+    assert O.stabilizer_distance(S).distance == 3
+
+
+def test_min_is_min_of_sides():
+    for Hx, Hz in (codes.toric(4), codes.shor(),
+                   codes.hypergraph_product(codes.repetition_parity(3), codes.repetition_parity(4))):
+        m = O.css_distance(Hx, Hz, "M").distance
+        z = O.css_distance(Hx, Hz, "Z").distance
+        x = O.css_distance(Hx, Hz, "X").distance
+        assert m == min(z, x)
+
+
+# ---------------------------------------------------------------- closed forms
+
+@pytest.mark.parametrize("name,kind,mats", [
+    ("config1", 0, (codes.config1_parity(0), None)),
+    ("toric6Z", 1, codes.toric(6)),
+    ("bb72Z", 1, codes.bb_family(6)),
+    ("surface5X", 2, codes.surface(5)),
+])
+def test_lower_bound_closed_form(name, kind, mats):
+    """After the last level d the reported lower bound is sum_g max(0, d+1-(K-r_g)),
+    rounded up to even for an even code (cpp/distfind/bz.cpp:159-162)."""
+    A, B = mats
+    info = O.problem_info(kind, A, B)
+    if kind == 0:
+        r = O.classical_distance(A)
+    else:
+        r = O.css_distance(A, B, which="Z" if kind == 1 else "X")
+    d = r.levels
+    lb = sum(max(0, d + 1 - (info["K"] - rk)) for rk in info["ranks"])
+    if info["even"]:
+        lb += lb & 1
+    assert r.lower_bound == lb
+    assert r.proven and r.distance <= lb
+
+
+def test_work_closed_form():
+    """Level work is C(K, d) x active information sets: a systematic generator's
+    enumeration at level d visits C(K, d) codewords, each once."""
+    rng = np.random.default_rng(2)
+    K, n, d = 9, 40, 3
+    G = np.concatenate([np.eye(K, dtype=np.uint8), rng.integers(0, 2, (K, n - K), dtype=np.uint8)], axis=1)
+    rows = O.collect_plan(pack(G)[None], n, d, n)   # keep = n: every nonzero codeword
+    assert rows.shape[0] == math.comb(K, d)
+    assert len({tuple(r) for r in rows.tolist()}) == math.comb(K, d)
+
+
+# ---------------------------------------------------------------- the level enumerator
+
+def rand_plan(rng, K, stride, ng, kcheck):
+    gamma = rng.integers(0, 2**64, size=(ng, K, stride), dtype=np.uint64)
+    check = rng.integers(0, 2**64, size=(kcheck, stride), dtype=np.uint64) if kcheck else None
+    return gamma, check
+
+
+def numpy_level(gamma, d, check, best, num_gamma):
+    ng, K, stride = gamma.shape
+    out = best
+    for g in range(num_gamma):
+        for comb in itertools.combinations(range(K), d):
+            cw = np.bitwise_xor.reduce(gamma[g, list(comb)], axis=0)
+            if check is not None:
+                logical = any(bin(int(np.bitwise_xor.reduce(np.bitwise_and(c, cw)))).count("1") & 1 for c in check)
+                if not logical:
+                    continue
+            w = sum(bin(int(x)).count("1") for x in cw)
+            out = min(out, w)
+    return out
+
+
+def test_enumerator_vs_plain_and_numpy():
+    rng = np.random.default_rng(0xC0FFEE)
+    for i in range(60):
+        K = int(rng.integers(4, 15)); stride = int(rng.integers(1, 4)); d = min(int(rng.integers(1, 6)), K)
+        ng = int(rng.integers(1, 4)); kcheck = int(rng.integers(0, 4))
+        best = NONE if i % 4 == 0 else int(rng.integers(0, stride * 64 + 1))
+        gamma, check = rand_plan(rng, K, stride, ng, kcheck)
+        n = stride * 64
+        a = O.enumerate_plan(gamma, n, d, check=check, current_best=best)
+        b = O.enumerate_brute(gamma, n, d, check=check, current_best=best)
+        assert a == b, (K, stride, d, ng, kcheck, best)
+        if i < 15:
+            assert a == numpy_level(gamma, d, check, best, ng)
+
+
+@pytest.mark.parametrize("threads", [1, 2, 3, 8])
+def test_enumerator_thread_split(threads):
+    rng = np.random.default_rng(threads)
+    gamma, check = rand_plan(rng, 18, 2, 2, 2)
+    a = O.enumerate_plan(gamma, 128, 5, check=check, current_best=NONE, threads=threads)
+    b = O.enumerate_brute(gamma, 128, 5, check=check, current_best=NONE)
+    assert a == b
+
+
+def test_collect_vs_numpy():
+    rng = np.random.default_rng(4)
+    for K, stride, d, keep in [(8, 1, 2, 40), (10, 2, 3, 70), (7, 1, 4, 34)]:
+        gamma, _ = rand_plan(rng, K, stride, 1, 0)
+        got = sorted(tuple(int(x) for x in r) for r in O.collect_plan(gamma, stride * 64, d, keep))
+        want = []
+        for comb in itertools.combinations(range(K), d):
+            cw = np.bitwise_xor.reduce(gamma[0, list(comb)], axis=0)
+            w = sum(bin(int(x)).count("1") for x in cw)
+            if 1 <= w <= keep:
+                want.append(tuple(int(x) for x in cw))
+        assert got == sorted(want)
+
+
+def test_reference_enumerator_agrees_with_oracle():
+    """Where the reference library is built, its own CPU enumerator on single levels."""
+    if not O.ref_available():
+        pytest.skip("reference library not built here")
+    rng = np.random.default_rng(8)
+    for i in range(20):
+        K = int(rng.integers(6, 20)); stride = int(rng.integers(1, 4)); d = min(int(rng.integers(1, 6)), K)
+        gamma, check = rand_plan(rng, K, stride, int(rng.integers(1, 3)), int(rng.integers(0, 3)))
+        best = NONE if i % 3 == 0 else int(rng.integers(10, 60))
+        assert O.ref_enumerate_plan(gamma, stride * 64, d, check=check, current_best=best) == \
+            O.enumerate_plan(gamma, stride * 64, d, check=check, current_best=best)
