@@ -60,6 +60,15 @@ __device__ __forceinline__ u64 range_bits(u64 lo, u64 hi, int base, int m) {
     return m >= 64 ? v : v & ((1ull << m) - 1ull);
 }
 
+// Values every lane already agrees on, moved to scalar registers so that loops and branches
+// on them are scalar (the compiler cannot see uniformity through vector loads).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// (readfirstlane returns int: each half goes through u32 so the low half is not sign-extended)
+__device__ __forceinline__ u64 uni64(u64 v) {
+    return ((u64)(u32)__builtin_amdgcn_readfirstlane((u32)(v >> 32)) << 32) |
+           (u64)(u32)__builtin_amdgcn_readfirstlane((u32)v);
+}
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
@@ -94,9 +103,11 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
     unsigned slow_entries = 0, medium_entries = 0;
 
     for (;;) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(a.next_tile, 1ull);
-        t = __shfl(t, 0, 64);
+        unsigned long long t0 = 0;
+        if (lane == 0) t0 = atomicAdd(a.next_tile, 1ull);
+        // readfirstlane (lane 0 is active) makes the tile index, and everything derived from
+        // it, provably wave-uniform: scalar registers and scalar loops, no exec masking
+        const u64 t = uni64(t0);
         if (t >= a.total_tiles) break;
         if (!a.collect) {
             int gb = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -110,14 +121,19 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
             int mid = (lo + hi + 1) >> 1;
             if (a.rects[mid].tile_base <= t) lo = mid; else hi = mid - 1;
         }
-        const Rect R = a.rects[lo];
+        const Rect& Rr = a.rects[uni(lo)];
+        struct {
+            u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream, f_lo, f_hi;
+            int fixed;
+        } R = {uni64(Rr.tile_base), uni64(Rr.stream_groups), uni64(Rr.stream_chunks), uni64(Rr.n_lane),
+               uni64(Rr.n_stream), uni64(Rr.f_lo), uni64(Rr.f_hi), uni(Rr.fixed)};
         const u64 local = t - R.tile_base;
         const u64 bx = local / R.stream_groups;
         const u64 bg = local - bx * R.stream_groups;
-        const int g = R.g;
+        const int g = uni(Rr.g);
 
-        const int lk = R.lk, lbase = R.lbase, lm = R.lm;
-        const int sk = R.sk, sbase = R.sbase, sm = R.sm;
+        const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
+        const int sk = uni(Rr.sk), sbase = uni(Rr.sbase), sm = uni(Rr.sm);
         const u64 flo = a.flags[2 * g], fhi = a.flags[2 * g + 1];
         const u64 lflags = range_bits(flo, fhi, lbase, lm);
         const u64 sflags = range_bits(flo, fhi, sbase, sm);
@@ -200,22 +216,10 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
             }
             __syncthreads();   // stream chunk visible
 
-            // ---- hot loop: every (lane element, stream element) pair
-            u32 nw[FD];
-            int noff = 0;
-#pragma unroll
-            for (int f = 0; f < FD; ++f) nw[f] = stream[0].w[f];
-            noff = stream[0].off;
-            for (int s = 0; s < len; ++s) {
-                u32 ew[FD];
-#pragma unroll
-                for (int f = 0; f < FD; ++f) ew[f] = nw[f];
-                const int eoff = noff;
-                if (s + 1 < len) {   // next element's LDS read overlaps this step's VALU work
-#pragma unroll
-                    for (int f = 0; f < FD; ++f) nw[f] = stream[s + 1].w[f];
-                    noff = stream[s + 1].off;
-                }
+            // ---- hot loop: every (lane element, stream element) pair. Two steps per trip
+            // with two element registers: the LDS read of the next element is in flight
+            // while the current one is tested, without register copies.
+            auto step = [&](const u32 (&ew)[FD], const int eoff, const int s) {
                 int acc = kBigOffset;
 #pragma unroll
                 for (int k = 0; k < RPL; ++k) {
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                         if (tk < thr) cand |= 1u << k;
                     }
                     ++medium_entries;
-                    if (__ballot(cand != 0) == 0ull) continue;
+                    if (__ballot(cand != 0) == 0ull) return;
                     // ---- slow path: exact weight and logical test of the light candidates
                     ++slow_entries;
                     const int owner = s / per;
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                                     }
                                 }
                             }
-                            continue;
+                            continue;   // next candidate of this lane
                         }
                         if (w < bound && w < mine) {
                             bool logical = (SD == 0);
@@ -312,6 +316,26 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                     }
                     __syncthreads();
                 }
+            };
+            u32 aw[FD], bw[FD];
+            int aoff = 0, boff = 0;
+#pragma unroll
+            for (int f = 0; f < FD; ++f) aw[f] = stream[0].w[f];
+            aoff = stream[0].off;
+            for (int s = 0; s < len; s += 2) {
+                if (s + 1 < len) {
+#pragma unroll
+                    for (int f = 0; f < FD; ++f) bw[f] = stream[s + 1].w[f];
+                    boff = stream[s + 1].off;
+                }
+                step(aw, aoff, s);
+                if (s + 1 >= len) break;
+                if (s + 2 < len) {
+#pragma unroll
+                    for (int f = 0; f < FD; ++f) aw[f] = stream[s + 2].w[f];
+                    aoff = stream[s + 2].off;
+                }
+                step(bw, boff, s + 1);
             }
             if (bound <= a.floor) break;
         }
@@ -653,7 +677,7 @@ struct Engine {
         }
         if (const char* env = std::getenv("QSA_TILED_FD")) {   // tuning/testing override
             int f = std::atoi(env);
-            if (f == 1) return s.RD == 1 ? table[0] : table[7];
+            if (f == 1) return s.RD == 1 ? table[0] : table[6];
             if (f == 2) return s.RD == 2 ? table[2] : table[3];
             if (f == 3) return table[4];
             if (f == 4) return table[5];
@@ -664,7 +688,7 @@ struct Engine {
         const long thr = (long)p.current_best - min_off;
         const int res = std::max(1, s.maxres);
         auto rate = [&](int fd, int rpl) { return 64.0 * rpl * binom_cdf_below(std::min(32 * fd, res), thr); };
-        if (rate(1, 32) < 0.05) return table[7];   // {1, 2, 32}
+        if (rate(1, 24) < 0.05) return table[6];   // {1, 2, 24}
         if (s.RD == 2) return table[2];
         if (rate(2, 12) < 0.05) return table[3];
         if (s.RD >= 3 && rate(3, 12) < 0.05) return table[4];
