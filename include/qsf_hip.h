@@ -3,7 +3,8 @@
  * These entry points replace the reference's GPU plugin, `qsf::Backend` as implemented by
  * `CudaBackend` (cpp/qsf/backend.hpp:53-61; cpp/qsf/cuda/backend_cuda.cu:506-645). A
  * host that already builds EnumPlans (the reference's bz.cpp / codeaut/bz.cpp drivers)
- * calls them through a ~40-line `HipBackend : qsf::Backend` shim.
+ * calls them through a ~40-line `HipBackend : qsf::Backend` shim that forwards each virtual
+ * to the entry point of the same name.
  *
  * Plain pointers and sizes only. Every call is synchronous with respect to the caller:
  * it returns after the device work has finished. Calls from several host threads are
@@ -73,6 +74,20 @@ int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out)
  * or an error: device-buffer overflow is retried with a larger buffer, never truncated. */
 int qsf_hip_collect(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits, int64_t* rows);
 void qsf_hip_free(void* p);
+
+/* Random information-set upper bound on the device (replaces the CPU heuristic
+ * random_is_seed, cpp/distfind/bz.cpp:24-70, which bz_distance and bz_min_interleaved run
+ * before level 1, bz.cpp:128 and bz.cpp:252). `gen` holds K rows of stride = (n+63)/64
+ * u64 words spanning the code searched; `check` kcheck detector rows (a row is a logical when it has odd overlap with
+ * one of them), or every_nonzero = 1 when every nonzero codeword is a logical. Each of
+ * `trials` random column orders puts gen into systematic form; *best_out = min(init,
+ * lightest nonzero logical row seen). Trial t draws its order by Fisher–Yates from the
+ * splitmix64 stream seeded with seed ^ (0xd1b54a32d192ed03 * (t + 1)), so the result is a
+ * pure function of the arguments. n <= 4096. Return codes as qsf_hip_enumerate. */
+int qsf_hip_is_bound(const uint64_t* gen, int K, int n, const uint64_t* check, int kcheck, int every_nonzero,
+                     int trials, uint64_t seed, int init, int* best_out);
+/* Device time in milliseconds of the last qsf_hip_is_bound call. */
+float qsf_hip_last_is_ms(void);
 
 /* Statistics of the last enumerate/collect call on this thread's library instance:
  * codewords examined (C(K,d) x num_gamma, the reference's `work`, bz.cpp:151-154), device

@@ -895,6 +895,46 @@ int oracle_upper_seed(int kind, const uint8_t* A, int ar, int ac, const uint8_t*
     return orc::is_seed(p, inner, 256, 0x9e3779b9u, hw_threads(is_threads));
 }
 
+// The engine's random information-set trials. The per-trial work restates bz.cpp:41-51
+// (systematic form over a random column order, lightest nonzero logical row); the column
+// order comes from the engine's own stream in place of the reference's mt19937 +
+// std::shuffle (bz.cpp:34-40): trial t shuffles 0..n-1 by Fisher–Yates, swapping i with
+// splitmix64() % (i + 1) for i = n-1 .. 1, the splitmix64 state seeded with
+// seed ^ (0xd1b54a32d192ed03 * (t + 1)). When rows_out / perm_out are given (one trial,
+// trials == 1 and first == t), they receive the systematic form (K x stride) and the order.
+static uint64_t splitmix_step(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+int oracle_is_bound_trials(const uint64_t* gen, int K, int n, const uint64_t* check, int kcheck, int every_nonzero,
+                           int first, int trials, uint64_t seed, int init, uint64_t* rows_out, int* perm_out) {
+    const int st = orc::wfor(n);
+    if (K == 0 || (!every_nonzero && kcheck == 0)) return init;
+    int best = init;
+    std::vector<int> cols(n);
+    for (int t = first; t < first + trials; ++t) {
+        uint64_t s = seed ^ (0xd1b54a32d192ed03ull * (uint64_t)(t + 1));
+        std::iota(cols.begin(), cols.end(), 0);
+        for (int i = n - 1; i > 0; --i) std::swap(cols[i], cols[(size_t)(splitmix_step(s) % (uint64_t)(i + 1))]);
+        orc::Mat M(K, n);
+        std::memcpy(M.w.data(), gen, (size_t)K * st * 8);
+        orc::rref_cols(M, cols);
+        if (rows_out) std::memcpy(rows_out, M.w.data(), (size_t)K * st * 8);
+        if (perm_out) std::memcpy(perm_out, cols.data(), (size_t)n * 4);
+        for (int r = 0; r < K; ++r) {
+            const int w = M.weight(r);
+            if (w == 0 || w >= best) continue;
+            bool lg = every_nonzero != 0;
+            for (int c = 0; c < kcheck && !lg; ++c) lg = orc::dot(check + (size_t)c * st, M.row(r), st);
+            if (lg) best = w;
+        }
+    }
+    return best;
+}
+
 // Problem dimensions and information-set ranks, for the host-logic tests.
 int oracle_problem_info(int kind, const uint8_t* A, int ar, int ac, const uint8_t* B2, int br, int bc,
                         int* K, int* kcheck, int* even, int* seed_upper, int* ranks, int max_ranks) {

@@ -143,6 +143,9 @@ def load() -> ctypes.CDLL:
     sig("qsf_hip_last_stats", None, [ctypes.POINTER(HipStats)])
     sig("qsf_hip_last_error", c_char_p, [])
     sig("qsf_hip_set_kernel_policy", c_int, [c_int])
+    sig("qsf_hip_is_bound", c_int, [u64p, c_int, c_int, u64p, c_int, c_int, c_int, ctypes.c_uint64, c_int,
+                                    ctypes.POINTER(c_int)])
+    sig("qsf_hip_last_is_ms", ctypes.c_float, [])
     _lib = L
     return L
 

@@ -70,7 +70,9 @@ struct Side {
         if (kcheck) std::memcpy(check.data(), p.detector.data.data(), (size_t)kcheck * stride * 8);
         key = fresh_key();
         inner = p.seed_upper > 0 ? p.seed_upper : kNoWeight;
-        inner = random_is_bound(p, inner, o.is_trials, o.seed);
+        int rc = qsf_hip_is_bound(p.gen.data.data(), K, n, check.data(), kcheck, p.every_nonzero ? 1 : 0,
+                                  o.is_trials, o.seed, inner, &inner);
+        if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
     }
 
     int active(int d) const {

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/qsf_hip.h"
+#include "elim.hpp"
 #include "kernels.hpp"
 
 namespace qsa {
@@ -621,6 +622,41 @@ struct Engine {
         s.hflags = std::move(fl);
     }
 
+    // ---- random information-set upper bound ------------------------------------------
+    DevBuf<u64> is_gen, is_check, is_scratch;
+    int is_bound(const u64* gen, int K, int n, const u64* check, int kcheck, int every_nonzero,
+                 int trials, u64 seed, int init, int* out) {
+        *out = init;
+        if (K == 0 || trials <= 0 || (!every_nonzero && kcheck == 0)) return 0;
+        const int st = (n + 63) / 64;
+        is_gen.reserve((size_t)K * st);
+        is_check.reserve((size_t)std::max(1, kcheck) * st);
+        HIPCHK(hipMemcpyAsync(is_gen.p, gen, (size_t)K * st * 8, hipMemcpyHostToDevice, stream));
+        if (kcheck) HIPCHK(hipMemcpyAsync(is_check.p, check, (size_t)kcheck * st * 8, hipMemcpyHostToDevice, stream));
+        best.reserve(1);
+        HIPCHK(hipMemcpyAsync(best.p, &init, sizeof(int), hipMemcpyHostToDevice, stream));
+        // batches bound the global scratch of matrices too large for LDS
+        const int batch = std::min(trials, 1024);
+        const size_t sw = elim_scratch_words(K, st, batch);
+        if (sw) is_scratch.reserve(sw);
+        ElimArgs a{is_gen.p, is_check.p, K, n, st, kcheck, every_nonzero, seed, 0, sw ? is_scratch.p : nullptr, best.p};
+        HIPCHK(hipEventRecord(ev0, stream));
+        for (int t0 = 0; t0 < trials; t0 += batch) {
+            a.trial_base = t0;
+            HIPCHK(launch_is_trials(a, std::min(batch, trials - t0), stream));
+        }
+        HIPCHK(hipEventRecord(ev1, stream));
+        int result = init;
+        HIPCHK(hipMemcpyAsync(&result, best.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        is_ms = ms;
+        *out = std::min(result, init);
+        return 0;
+    }
+    float is_ms = 0;
+
     // ---- min mode -------------------------------------------------------------------
     int enumerate(const qsf_enum_plan& p, int floor, int* best_out) {
         stats = qsf_hip_stats{};
@@ -750,7 +786,7 @@ struct Engine {
                 emit(f_lo, f_hi, xb, xm, xk, yb, ym, yk, nx, ny);
                 return;
             }
-            const int sb = x_small ? xb : yb, sm = x_small ? xm : ym, sk = x_small ? xk : yk;
+            const int sb = x_small ? xb : yb, sk = x_small ? xk : yk;   // small side, enumerated
             const int mid = bm / 2;
             u64 m = sk == 0 ? 0 : (sk >= 64 ? ~0ull : ((1ull << sk) - 1));
             for (u64 e = 0; e < ns; ++e) {
@@ -966,6 +1002,20 @@ int qsf_hip_collect(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits,
     }
     return run_guarded("qsf_hip_collect", [&](qsa::Engine& e) { return e.collect(*plan, keep_weight, hits, rows); });
 }
+
+int qsf_hip_is_bound(const uint64_t* gen, int K, int n, const uint64_t* check, int kcheck, int every_nonzero,
+                     int trials, uint64_t seed, int init, int* best_out) {
+    if (!best_out || K < 0 || n <= 0 || n > qsa::kElimMaxN || (K > 0 && !gen) || kcheck < 0 ||
+        (kcheck > 0 && !check) || trials < 0) {
+        engine().err = "qsf_hip_is_bound: bad arguments (n must be in [1, 4096])";
+        return 1;
+    }
+    return run_guarded("qsf_hip_is_bound", [&](qsa::Engine& e) {
+        return e.is_bound(gen, K, n, check, kcheck, every_nonzero, trials, seed, init, best_out);
+    });
+}
+
+float qsf_hip_last_is_ms(void) { return engine().is_ms; }
 
 void qsf_hip_free(void* p) { std::free(p); }
 

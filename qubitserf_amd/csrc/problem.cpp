@@ -3,7 +3,6 @@
 #include <algorithm>
 #include <functional>
 #include <numeric>
-#include <thread>
 
 namespace qsa {
 
@@ -393,48 +392,6 @@ std::vector<InfoSet> information_sets(const BitMat& gen) {
     if (seq.empty() || count_full(seq, K) <= gfull) { sort_by_rank(greedy); return greedy; }
     sort_by_rank(seq);
     return seq;
-}
-
-// ------------------------------------------------------------------ random IS bound
-
-namespace {
-inline u64 mix64(u64& s) {   // splitmix64 step
-    u64 z = (s += 0x9e3779b97f4a7c15ull);
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
-}  // namespace
-
-int random_is_bound(const Problem& p, int init, int trials, u64 seed) {
-    const int K = p.gen.rows, n = p.n;
-    if (K == 0 || trials <= 0) return init;
-    int workers = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
-    workers = std::min(workers, trials);
-    std::vector<int> best(workers, init);
-    auto job = [&](int w) {
-        std::vector<int> order(n);
-        for (int t = w; t < trials; t += workers) {
-            u64 s = seed ^ (0xd1b54a32d192ed03ull * (u64)(t + 1));
-            std::iota(order.begin(), order.end(), 0);
-            for (int i = n - 1; i > 0; --i) std::swap(order[i], order[(int)(mix64(s) % (u64)(i + 1))]);
-            BitMat m = p.gen;
-            eliminate(m, order.data(), n);
-            for (int r = 0; r < m.rows; ++r) {
-                int wt = m.weight(r);
-                if (wt == 0 || wt >= best[w]) continue;
-                bool logical = p.every_nonzero;
-                for (int c = 0; c < p.detector.rows && !logical; ++c)
-                    logical = parity_and(p.detector.row(c), m.row(r), m.ld);
-                if (logical) best[w] = wt;
-            }
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < workers; ++w) pool.emplace_back(job, w);
-    job(0);
-    for (auto& t : pool) t.join();
-    return *std::min_element(best.begin(), best.end());
 }
 
 }  // namespace qsa

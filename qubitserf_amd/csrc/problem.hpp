@@ -49,9 +49,4 @@ std::vector<InfoSet> information_sets(const BitMat& gen);
 // Same, from an explicit column partition (each part eliminated over its own columns).
 std::vector<InfoSet> information_sets_from_parts(const BitMat& gen, const std::vector<std::vector<int>>& parts);
 
-// Upper bound on the distance from `trials` random column orders: each order puts gen
-// in systematic form, every row is a codeword, the lightest logical row bounds d
-// (the heuristic of cpp/distfind/bz.cpp:24-70; its random stream is this engine's own).
-int random_is_bound(const Problem& p, int init, int trials, u64 seed);
-
 }  // namespace qsa

@@ -60,6 +60,26 @@ def enumerate_level(gamma, n, d, *, check=None, num_gamma=None, current_best=WEI
     return out.value
 
 
+def is_bound(gen, n, *, check=None, trials=256, seed=0x9E3779B9, init=WEIGHT_NONE) -> int:
+    """Random information-set upper bound on the device: min(init, lightest nonzero logical
+    row over `trials` systematic forms of the packed generator `gen` (K x stride uint64)).
+    check=None: every nonzero codeword is a logical (cpp/distfind/bz.cpp:24-70)."""
+    L = _native.load()
+    g = np.ascontiguousarray(gen, np.uint64)
+    stride = (n + 63) // 64
+    if g.ndim != 2 or (g.shape[0] and g.shape[1] != stride):
+        raise ValueError(f"gen must be K x {stride} uint64")
+    c = np.ascontiguousarray(check if check is not None else np.zeros((1, stride), np.uint64), np.uint64)
+    kcheck = 0 if check is None else c.shape[0]
+    out = ctypes.c_int(0)
+    rc = L.qsf_hip_is_bound(g.ctypes.data_as(_native.u64p) if g.size else None, g.shape[0], int(n),
+                            c.ctypes.data_as(_native.u64p), kcheck, 1 if check is None else 0, int(trials),
+                            int(seed), int(init), ctypes.byref(out))
+    if rc:
+        _err(rc, "qsf_hip_is_bound")
+    return out.value
+
+
 def collect_level(gamma, n, d, keep_weight, *, num_gamma=None, key=None):
     """(rows, stride) uint64 array of every XOR of d rows with weight in [1, keep_weight] (dups allowed)."""
     L = _native.load()

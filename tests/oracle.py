@@ -50,6 +50,7 @@ def lib():
         L.oracle_mitm_distance.argtypes = [i, u8p, i, i, u8p, i, i, i]
         ip = ctypes.POINTER(ctypes.c_int)
         L.oracle_problem_info.argtypes = [i, u8p, i, i, u8p, i, i, ip, ip, ip, ip, ip, i]
+        L.oracle_is_bound_trials.argtypes = [u64p, i, i, u64p, i, i, i, i, ctypes.c_uint64, i, u64p, ip]
         _o = L
     return _o
 
@@ -169,6 +170,30 @@ def collect_plan(gamma, n, d, keep, num_gamma=None, threads=0) -> np.ndarray:
         if rows <= cap:
             return out[:rows]
         cap = rows
+
+
+def is_bound_trials(gen, n, check=None, trials=256, seed=0x9E3779B9, init=WEIGHT_NONE, first=0) -> int:
+    """The engine's random information-set bound restated on the CPU (oracle_is_bound_trials)."""
+    g = np.ascontiguousarray(gen, np.uint64)
+    K = g.shape[0]
+    stride = (n + 63) // 64
+    c = np.ascontiguousarray(check if check is not None else np.zeros((1, stride), np.uint64), np.uint64)
+    kcheck = 0 if check is None else c.shape[0]
+    return lib().oracle_is_bound_trials(g.ctypes.data_as(u64p) if K else None, K, n, c.ctypes.data_as(u64p), kcheck,
+                                        1 if check is None else 0, first, trials, seed, init, None, None)
+
+
+def is_trial(gen, n, t, seed=0x9E3779B9):
+    """(systematic form K x stride, column order) of trial t of the engine's stream."""
+    g = np.ascontiguousarray(gen, np.uint64)
+    K = g.shape[0]
+    stride = (n + 63) // 64
+    rows = np.zeros((K, stride), np.uint64)
+    perm = np.zeros(n, np.int32)
+    c = np.zeros((1, stride), np.uint64)
+    lib().oracle_is_bound_trials(g.ctypes.data_as(u64p), K, n, c.ctypes.data_as(u64p), 0, 1, t, 1, seed, WEIGHT_NONE,
+                                 rows.ctypes.data_as(u64p), perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return rows, perm
 
 
 # ---------------------------------------------------------------- reference CPU library

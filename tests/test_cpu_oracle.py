@@ -266,3 +266,123 @@ def test_reference_enumerator_agrees_with_oracle():
         best = NONE if i % 3 == 0 else int(rng.integers(10, 60))
         assert O.ref_enumerate_plan(gamma, stride * 64, d, check=check, current_best=best) == \
             O.enumerate_plan(gamma, stride * 64, d, check=check, current_best=best)
+
+
+# ---------------------------------------------------------------- random information sets
+
+def _gf2_inv(A):
+    """Inverse over GF(2) of a square 0/1 matrix, by row reduction of [A | I]."""
+    k = A.shape[0]
+    M = np.concatenate([A.astype(np.uint8) & 1, np.eye(k, dtype=np.uint8)], axis=1)
+    for c in range(k):
+        r = c + int(np.flatnonzero(M[c:, c])[0])
+        M[[c, r]] = M[[r, c]]
+        for rr in np.flatnonzero(M[:, c]):
+            if rr != c:
+                M[rr] ^= M[c]
+    return M[:, k:]
+
+
+def _nullspace(H):
+    """Basis of {x : H x = 0} over GF(2), by row reduction of H."""
+    H = (np.asarray(H, np.uint8) & 1).copy()
+    r, n = H.shape
+    piv, row = [], 0
+    for c in range(n):
+        nz = [i for i in range(row, r) if H[i, c]]
+        if not nz:
+            continue
+        H[[row, nz[0]]] = H[[nz[0], row]]
+        for i in range(r):
+            if i != row and H[i, c]:
+                H[i] ^= H[row]
+        piv.append(c)
+        row += 1
+    free = [c for c in range(n) if c not in piv]
+    B = np.zeros((len(free), n), np.uint8)
+    for j, f in enumerate(free):
+        B[j, f] = 1
+        for i, p in enumerate(piv):
+            B[j, p] = H[i, f]
+    return B
+
+
+def _golay23_generator():
+    """Cyclic [23,12,7] Golay code from g(x) = 1 + x^2 + x^4 + x^5 + x^6 + x^10 + x^11."""
+    g = np.zeros(23, np.uint8)
+    g[[0, 2, 4, 5, 6, 10, 11]] = 1
+    return np.stack([np.roll(g, i) for i in range(12)])
+
+
+def _random_basis(rng, K, n):
+    while True:
+        G = rng.integers(0, 2, size=(K, n), dtype=np.uint8)
+        if codes.gf2_rank(G) == K:
+            return G
+
+
+@pytest.mark.parametrize("K,n", [(1, 5), (6, 13), (20, 70), (33, 130)])
+def test_is_trial_is_the_unique_rref(K, n):
+    """Each trial's systematic form is THE reduced row echelon form of the row space over
+    the trial's column order, computed here another way: the pivots are the columns that
+    raise the column rank in that order, and R = G[:, pivots]^-1 G."""
+    from qubitserf_amd.engine import pack_rows, unpack_rows
+    rng = np.random.default_rng(K * 1000 + n)
+    G = _random_basis(rng, K, n)
+    for t in (0, 1, 17):
+        rows, perm = O.is_trial(pack_rows(G), n, t)
+        assert sorted(perm.tolist()) == list(range(n))
+        piv, rank = [], 0
+        for c in perm:
+            cand = piv + [int(c)]
+            if codes.gf2_rank(G[:, cand]) > rank:
+                piv, rank = cand, rank + 1
+            if rank == K:
+                break
+        R = (_gf2_inv(G[:, piv]).astype(np.int64) @ G.astype(np.int64)) % 2
+        assert np.array_equal(unpack_rows(rows, n), R.astype(np.uint8))
+
+
+def test_is_trial_orders_differ_and_depend_on_seed():
+    from qubitserf_amd.engine import pack_rows
+    G = pack_rows(_random_basis(np.random.default_rng(5), 8, 40))
+    p = [O.is_trial(G, 40, t)[1].tolist() for t in range(4)]
+    assert len({tuple(x) for x in p}) == 4
+    assert O.is_trial(G, 40, 0, seed=1)[1].tolist() != p[0]
+
+
+def test_is_bound_is_min_over_trials_and_bounds_distance():
+    from qubitserf_amd.engine import pack_rows, unpack_rows
+    rng = np.random.default_rng(11)
+    for trial in range(6):
+        K, n = int(rng.integers(3, 12)), int(rng.integers(14, 40))
+        G = _random_basis(rng, K, n)
+        H = _nullspace(G)
+        d = O.bruteforce_distance(0, H)
+        Gp = pack_rows(G)
+        want = min(int(unpack_rows(O.is_trial(Gp, n, t)[0], n).sum(axis=1).min()) for t in range(9))
+        assert O.is_bound_trials(Gp, n, trials=9) == want
+        assert O.is_bound_trials(Gp, n, trials=256) >= d
+        assert O.is_bound_trials(Gp, n, trials=9, init=1) == 1
+    # trials split across calls fold to the same bound
+    assert min(O.is_bound_trials(Gp, n, trials=5), O.is_bound_trials(Gp, n, trials=4, first=5)) == \
+        O.is_bound_trials(Gp, n, trials=9)
+
+
+def test_is_bound_exact_on_small_codes():
+    """Hamming [7,4,3] and the [23,12,7] Golay code: 256 systematic forms reach d."""
+    from qubitserf_amd.engine import pack_rows
+    for G, d in ((_nullspace(codes.hamming_parity(3)), 3), (_golay23_generator(), 7)):
+        assert codes.gf2_rank(G) == G.shape[0]
+        assert O.is_bound_trials(pack_rows(G), G.shape[1], trials=256) == d
+
+
+def test_is_bound_logical_test():
+    """With detector rows only odd-overlap rows count: the 6x6 toric code's searched space
+    holds its weight-4 plaquette stabilizers, yet the bound with detectors is >= d = 6."""
+    gamma, ranks, check, even = O.side_plan(1, *codes.toric(6))
+    G = gamma[0]
+    n = 2 * 36
+    assert check.shape[0] > 0
+    assert O.is_bound_trials(G, n, check=check, trials=256) >= 6
+    assert O.is_bound_trials(G, n, trials=256) < 6
