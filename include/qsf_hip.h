@@ -75,19 +75,63 @@ int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out)
 int qsf_hip_collect(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits, int64_t* rows);
 void qsf_hip_free(void* p);
 
-/* Random information-set upper bound on the device (replaces the CPU heuristic
- * random_is_seed, cpp/distfind/bz.cpp:24-70, which bz_distance and bz_min_interleaved run
- * before level 1, bz.cpp:128 and bz.cpp:252). `gen` holds K rows of stride = (n+63)/64
- * u64 words spanning the code searched; `check` kcheck detector rows (a row is a logical when it has odd overlap with
- * one of them), or every_nonzero = 1 when every nonzero codeword is a logical. Each of
- * `trials` random column orders puts gen into systematic form; *best_out = min(init,
- * lightest nonzero logical row seen). Trial t draws its order by Fisher–Yates from the
- * splitmix64 stream seeded with seed ^ (0xd1b54a32d192ed03 * (t + 1)), so the result is a
- * pure function of the arguments. n <= 4096. Return codes as qsf_hip_enumerate. */
+/* Elimination phase on the device, many problems per call (one wave per problem, or per
+ * problem and trial). Each job describes one BZ problem: `gen` K rows of stride =
+ * (n+63)/64 u64 words spanning the code searched, `check` kcheck detector rows (a row is a
+ * logical when it has odd overlap with one of them) or every_nonzero = 1.
+ *
+ * QSF_ELIM_SETS: the information sets in systematic form (replaces gamma_sequence_greedy,
+ *   cpp/distfind/css.cpp:177-203, over restricted_rref, gf2.cpp:42-58). Without a
+ *   partition: the greedy peel — eliminate over the live columns in order (pivot = first
+ *   row at/below the rank with a 1), the pivots form a set and the matrix its generator,
+ *   continue from that matrix over the live columns that were neither pivots nor zero
+ *   columns, stop when a round removes nothing. With a partition (nparts > 0; part p is
+ *   part_cols[part_off[p] .. part_off[p+1])): set p is gen made systematic over part p
+ *   (css.cpp:358-361). Outputs: nsets, and the first min(nsets, set_cap) generators
+ *   (set_cap x K x stride u64, in `sets`) and ranks (in `ranks`), caller-owned.
+ * QSF_ELIM_BOUND: the random information-set upper bound (replaces random_is_seed,
+ *   cpp/distfind/bz.cpp:24-70, run before level 1 at bz.cpp:128 and bz.cpp:252): best =
+ *   min(init, lightest nonzero logical row over `trials` systematic forms). Trial t orders
+ *   the columns by Fisher–Yates over the splitmix64 stream seeded with
+ *   seed ^ (0xd1b54a32d192ed03 * (t + 1)), swapping position i with floor(hi32(x) (i+1) / 2^32),
+ *   so the result is a pure function of the arguments.
+ * n <= 4096. Return codes as qsf_hip_enumerate. */
+#define QSF_ELIM_SETS 1
+#define QSF_ELIM_BOUND 2
+typedef struct {
+    const uint64_t* gen;
+    int K;
+    int n;
+    const uint64_t* check;
+    int kcheck;
+    int every_nonzero;
+    const int* part_cols;
+    const int* part_off;
+    int nparts;
+    int init;
+    int best;            /* out */
+    int nsets;           /* out */
+    uint64_t* sets;      /* out, set_cap x K x stride */
+    int set_cap;
+    int* ranks;          /* out, set_cap */
+} qsf_elim_job;
+int qsf_hip_eliminate(qsf_elim_job* jobs, int njobs, int trials, uint64_t seed, int what);
+
+/* One job, QSF_ELIM_BOUND only. */
 int qsf_hip_is_bound(const uint64_t* gen, int K, int n, const uint64_t* check, int kcheck, int every_nonzero,
                      int trials, uint64_t seed, int init, int* best_out);
-/* Device time in milliseconds of the last qsf_hip_is_bound call. */
-float qsf_hip_last_is_ms(void);
+
+/* Last qsf_hip_eliminate call: device milliseconds of the information-set kernel and of
+ * the random-IS kernels, and the information-set kernel's algorithmic HBM bytes
+ * (generators read once + systematic generators written). */
+typedef struct {
+    double sets_ms;
+    double bound_ms;
+    uint64_t sets_bytes;
+    int jobs;
+    int trials;
+} qsf_hip_elim_stats;
+void qsf_hip_last_elim_stats(qsf_hip_elim_stats* out);
 
 /* Statistics of the last enumerate/collect call on this thread's library instance:
  * codewords examined (C(K,d) x num_gamma, the reference's `work`, bz.cpp:151-154), device

@@ -899,7 +899,7 @@ int oracle_upper_seed(int kind, const uint8_t* A, int ar, int ac, const uint8_t*
 // (systematic form over a random column order, lightest nonzero logical row); the column
 // order comes from the engine's own stream in place of the reference's mt19937 +
 // std::shuffle (bz.cpp:34-40): trial t shuffles 0..n-1 by Fisher–Yates, swapping i with
-// splitmix64() % (i + 1) for i = n-1 .. 1, the splitmix64 state seeded with
+// floor(hi32(splitmix64()) (i + 1) / 2^32) for i = n-1 .. 1, the splitmix64 state seeded with
 // seed ^ (0xd1b54a32d192ed03 * (t + 1)). When rows_out / perm_out are given (one trial,
 // trials == 1 and first == t), they receive the systematic form (K x stride) and the order.
 static uint64_t splitmix_step(uint64_t& s) {
@@ -918,7 +918,7 @@ int oracle_is_bound_trials(const uint64_t* gen, int K, int n, const uint64_t* ch
     for (int t = first; t < first + trials; ++t) {
         uint64_t s = seed ^ (0xd1b54a32d192ed03ull * (uint64_t)(t + 1));
         std::iota(cols.begin(), cols.end(), 0);
-        for (int i = n - 1; i > 0; --i) std::swap(cols[i], cols[(size_t)(splitmix_step(s) % (uint64_t)(i + 1))]);
+        for (int i = n - 1; i > 0; --i) std::swap(cols[i], cols[(size_t)(((splitmix_step(s) >> 32) * (uint64_t)(i + 1)) >> 32)]);
         orc::Mat M(K, n);
         std::memcpy(M.w.data(), gen, (size_t)K * st * 8);
         orc::rref_cols(M, cols);
@@ -933,6 +933,44 @@ int oracle_is_bound_trials(const uint64_t* gen, int K, int n, const uint64_t* ch
         }
     }
     return best;
+}
+
+// The problem's generator (K x stride) and detector rows (kcheck x stride), css.cpp:78-120.
+int oracle_problem_gen(int kind, const uint8_t* A, int ar, int ac, const uint8_t* B2, int br, int bc,
+                       uint64_t* gen, int64_t gen_cap, uint64_t* check, int64_t check_cap, int* K, int* kcheck) {
+    orc::Problem p = make_problem(kind, A, ar, ac, B2, br, bc);
+    *K = p.gen.rows;
+    *kcheck = p.count_all ? 0 : p.check.rows;
+    if ((int64_t)p.gen.w.size() > gen_cap || (int64_t)((size_t)*kcheck * p.gen.stride) > check_cap) return -1;
+    std::memcpy(gen, p.gen.w.data(), p.gen.w.size() * 8);
+    if (*kcheck) std::memcpy(check, p.check.w.data(), (size_t)*kcheck * p.gen.stride * 8);
+    return 0;
+}
+
+// The greedy peel alone, in peel order (css.cpp:177-203): returns the number of sets;
+// stores the first min(nsets, cap) generators (K x stride each) and ranks.
+int oracle_greedy_sets(const uint64_t* gen, int K, int n, uint64_t* out, int cap, int* ranks) {
+    orc::Mat G(K, n);
+    std::memcpy(G.w.data(), gen, G.w.size() * 8);
+    std::vector<orc::Gamma> seq = orc::gammas_greedy(G);
+    for (int i = 0; i < (int)seq.size() && i < cap; ++i) {
+        std::memcpy(out + (size_t)i * G.w.size(), seq[i].g.w.data(), G.w.size() * 8);
+        ranks[i] = seq[i].rank;
+    }
+    return (int)seq.size();
+}
+
+// gen made systematic over each part's columns in order (css.cpp:358-361).
+void oracle_part_sets(const uint64_t* gen, int K, int n, const int* cols, const int* off, int nparts,
+                      uint64_t* out, int* ranks) {
+    orc::Mat G(K, n);
+    std::memcpy(G.w.data(), gen, G.w.size() * 8);
+    for (int p = 0; p < nparts; ++p) {
+        orc::Mat W = G;
+        std::vector<int> c(cols + off[p], cols + off[p + 1]);
+        ranks[p] = (int)orc::rref_cols(W, c).size();
+        std::memcpy(out + (size_t)p * G.w.size(), W.w.data(), G.w.size() * 8);
+    }
 }
 
 // Problem dimensions and information-set ranks, for the host-logic tests.

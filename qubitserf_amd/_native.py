@@ -75,6 +75,32 @@ class HipStats(ctypes.Structure):
     ]
 
 
+class ElimJob(ctypes.Structure):
+    """Layout of qsf_elim_job (include/qsf_hip.h)."""
+    _fields_ = [
+        ("gen", ctypes.POINTER(ctypes.c_uint64)),
+        ("K", ctypes.c_int),
+        ("n", ctypes.c_int),
+        ("check", ctypes.POINTER(ctypes.c_uint64)),
+        ("kcheck", ctypes.c_int),
+        ("every_nonzero", ctypes.c_int),
+        ("part_cols", ctypes.POINTER(ctypes.c_int)),
+        ("part_off", ctypes.POINTER(ctypes.c_int)),
+        ("nparts", ctypes.c_int),
+        ("init", ctypes.c_int),
+        ("best", ctypes.c_int),
+        ("nsets", ctypes.c_int),
+        ("sets", ctypes.POINTER(ctypes.c_uint64)),
+        ("set_cap", ctypes.c_int),
+        ("ranks", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+class ElimStats(ctypes.Structure):
+    _fields_ = [("sets_ms", ctypes.c_double), ("bound_ms", ctypes.c_double), ("sets_bytes", ctypes.c_uint64),
+                ("jobs", ctypes.c_int), ("trials", ctypes.c_int)]
+
+
 class DistfindLevel(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("side", "d", "active", "ngamma", "K", "inner", "outer", "kernel",
                                               "fd", "xd", "rpl", "pad")] + [
@@ -145,7 +171,8 @@ def load() -> ctypes.CDLL:
     sig("qsf_hip_set_kernel_policy", c_int, [c_int])
     sig("qsf_hip_is_bound", c_int, [u64p, c_int, c_int, u64p, c_int, c_int, c_int, ctypes.c_uint64, c_int,
                                     ctypes.POINTER(c_int)])
-    sig("qsf_hip_last_is_ms", ctypes.c_float, [])
+    sig("qsf_hip_eliminate", c_int, [ctypes.POINTER(ElimJob), c_int, c_int, ctypes.c_uint64, c_int])
+    sig("qsf_hip_last_elim_stats", None, [ctypes.POINTER(ElimStats)])
     _lib = L
     return L
 

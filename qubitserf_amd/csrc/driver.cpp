@@ -34,6 +34,74 @@ void say_final(int d, double secs, const char* tag) {
     print_elapsed(secs);
 }
 
+// Information sets of gen as the level loop uses them: systematic generators computed on
+// the device, ordered by rank, descending, stable (bz.cpp:96-97).
+struct Family {
+    int ng = 0;
+    std::vector<u64> gamma;   // ng x K x stride
+    std::vector<int> ranks;
+    int full(int K) const { return (int)std::count(ranks.begin(), ranks.end(), K); }
+};
+
+Family device_sets(const BitMat& gen, const std::vector<std::vector<int>>* parts) {
+    const int K = gen.rows, n = gen.cols, st = words(n);
+    std::vector<int> cols, off;
+    qsf_elim_job q{};
+    q.gen = gen.data.data();
+    q.K = K;
+    q.n = n;
+    q.init = kNoWeight;
+    if (parts) {
+        off.push_back(0);
+        for (auto& p : *parts) {
+            cols.insert(cols.end(), p.begin(), p.end());
+            off.push_back((int)cols.size());
+        }
+        q.part_cols = cols.data();
+        q.part_off = off.data();
+        q.nparts = (int)parts->size();
+    }
+    int cap = parts ? std::max(1, (int)parts->size()) : std::min(n + 1, 16);
+    std::vector<u64> sets;
+    std::vector<int> ranks;
+    for (;;) {   // a greedy peel longer than the first guess reruns with room for all sets
+        sets.assign((size_t)cap * K * st + 1, 0);
+        ranks.assign(cap, 0);
+        q.sets = sets.data();
+        q.set_cap = cap;
+        q.ranks = ranks.data();
+        const int rc = qsf_hip_eliminate(&q, 1, 0, 0, QSF_ELIM_SETS);
+        if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+        if (q.nsets <= cap) break;
+        cap = q.nsets;
+    }
+    std::vector<int> idx(q.nsets);
+    for (int i = 0; i < q.nsets; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ranks[a] > ranks[b]; });
+    Family f;
+    f.ng = q.nsets;
+    f.gamma.resize((size_t)f.ng * K * st);
+    for (int i = 0; i < f.ng; ++i) {
+        std::memcpy(f.gamma.data() + (size_t)i * K * st, sets.data() + (size_t)idx[i] * K * st, (size_t)K * st * 8);
+        f.ranks.push_back(ranks[idx[i]]);
+    }
+    return f;
+}
+
+// Greedy peel, replaced by the matroid partition's sets when n <= 200 and the partition
+// yields more full-rank sets (cpp/distfind/css.cpp:336-366).
+Family information_sets(const BitMat& gen) {
+    Family greedy = device_sets(gen, nullptr);
+    const int K = gen.rows, n = gen.cols;
+    if (K == 0) return greedy;
+    const int gfull = greedy.full(K);
+    if (gfull >= nonzero_columns(gen) / K || n > 200) return greedy;
+    std::vector<std::vector<int>> parts;
+    if (!matroid_parts(gen, parts) || parts.empty()) return greedy;
+    Family seq = device_sets(gen, &parts);
+    return seq.full(K) <= gfull ? greedy : seq;
+}
+
 // One subproblem's level-stepping state. Owns the flat generator / detector arrays the
 // device caches under `key` for the whole solve.
 struct Side {
@@ -56,15 +124,18 @@ struct Side {
         even = p.even;
         has = !(K == 0 || (!p.every_nonzero && p.detector.rows == 0));
         if (!has) { resolved = true; return; }
-        std::vector<InfoSet> sets = o.parts && !o.parts->empty() ? information_sets_from_parts(p.gen, *o.parts)
-                                                                 : information_sets(p.gen);
-        ng = (int)sets.size();
-        gamma.assign((size_t)ng * K * stride, 0);
-        ranks.resize(ng);
-        for (int g = 0; g < ng; ++g) {
-            ranks[g] = sets[g].rank;
-            std::memcpy(gamma.data() + (size_t)g * K * stride, sets[g].gen.data.data(), (size_t)K * stride * 8);
+        Family fam;
+        if (o.parts && !o.parts->empty()) {
+            std::vector<std::vector<int>> parts;
+            for (auto& c : *o.parts)
+                if (!c.empty()) parts.push_back(c);
+            fam = device_sets(p.gen, &parts);
+        } else {
+            fam = information_sets(p.gen);
         }
+        ng = fam.ng;
+        gamma = std::move(fam.gamma);
+        ranks = std::move(fam.ranks);
         kcheck = p.every_nonzero ? 0 : p.detector.rows;
         check.assign((size_t)std::max(1, kcheck) * stride, 0);
         if (kcheck) std::memcpy(check.data(), p.detector.data.data(), (size_t)kcheck * stride * 8);

@@ -622,40 +622,127 @@ struct Engine {
         s.hflags = std::move(fl);
     }
 
-    // ---- random information-set upper bound ------------------------------------------
-    DevBuf<u64> is_gen, is_check, is_scratch;
-    int is_bound(const u64* gen, int K, int n, const u64* check, int kcheck, int every_nonzero,
-                 int trials, u64 seed, int init, int* out) {
-        *out = init;
-        if (K == 0 || trials <= 0 || (!every_nonzero && kcheck == 0)) return 0;
-        const int st = (n + 63) / 64;
-        is_gen.reserve((size_t)K * st);
-        is_check.reserve((size_t)std::max(1, kcheck) * st);
-        HIPCHK(hipMemcpyAsync(is_gen.p, gen, (size_t)K * st * 8, hipMemcpyHostToDevice, stream));
-        if (kcheck) HIPCHK(hipMemcpyAsync(is_check.p, check, (size_t)kcheck * st * 8, hipMemcpyHostToDevice, stream));
-        best.reserve(1);
-        HIPCHK(hipMemcpyAsync(best.p, &init, sizeof(int), hipMemcpyHostToDevice, stream));
-        // batches bound the global scratch of matrices too large for LDS
-        const int batch = std::min(trials, 1024);
-        const size_t sw = elim_scratch_words(K, st, batch);
-        if (sw) is_scratch.reserve(sw);
-        ElimArgs a{is_gen.p, is_check.p, K, n, st, kcheck, every_nonzero, seed, 0, sw ? is_scratch.p : nullptr, best.p};
-        HIPCHK(hipEventRecord(ev0, stream));
-        for (int t0 = 0; t0 < trials; t0 += batch) {
-            a.trial_base = t0;
-            HIPCHK(launch_is_trials(a, std::min(batch, trials - t0), stream));
+    // ---- elimination: information sets and random-IS bounds, many problems per launch --
+    DevBuf<u64> el_u64;        // gens, checks, set outputs
+    DevBuf<int> el_int;        // bounds, set counts, ranks, partitions
+    DevBuf<u64> el_scratch;
+    DevBuf<ElimJob> el_jobs;
+    std::vector<ElimJob> el_hjobs;
+    qsf_hip_elim_stats el_stats{};
+
+    int eliminate(qsf_elim_job* jobs, int njobs, int trials, u64 seed, int what) {
+        el_stats = qsf_hip_elim_stats{};
+        el_stats.jobs = njobs;
+        const bool want_sets = (what & QSF_ELIM_SETS) != 0, want_bound = (what & QSF_ELIM_BOUND) != 0 && trials > 0;
+        // layout: per job [gen | check | sets]  and  [best | nsets | ranks | part_off | part_cols]
+        std::vector<size_t> u_off(njobs), i_off(njobs);
+        size_t nu = 0, ni = 0, scratch_sets = 0, scratch_trial = 0;
+        int max_n = 0;
+        for (int j = 0; j < njobs; ++j) {
+            const qsf_elim_job& q = jobs[j];
+            const int st = (q.n + 63) / 64;
+            u_off[j] = nu;
+            nu += (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st + (want_sets ? (size_t)q.set_cap * q.K * st : 0);
+            i_off[j] = ni;
+            ni += 2 + (want_sets ? (size_t)q.set_cap : 0) + (q.nparts > 0 ? (size_t)q.nparts + 1 + q.part_off[q.nparts] : 0);
+            scratch_sets += elim_scratch_words(q.K, st, 1);
+            scratch_trial += elim_scratch_words(q.K, st, 1);
+            max_n = std::max(max_n, q.n);
         }
-        HIPCHK(hipEventRecord(ev1, stream));
-        int result = init;
-        HIPCHK(hipMemcpyAsync(&result, best.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        // random-IS trials per launch: bounded so that the scratch stays under 1 GiB
+        const int batch = !want_bound ? 0
+                          : (int)std::max<size_t>(1, std::min<size_t>((size_t)trials, scratch_trial ? (1ull << 27) / scratch_trial : (size_t)trials));
+        const size_t nscratch = std::max(scratch_sets, scratch_trial * (size_t)batch);
+        std::vector<u64> hu(nu, 0);
+        std::vector<int> hi(ni, 0);
+        for (int j = 0; j < njobs; ++j) {
+            const qsf_elim_job& q = jobs[j];
+            const int st = (q.n + 63) / 64;
+            std::memcpy(hu.data() + u_off[j], q.gen, (size_t)q.K * st * 8);
+            if (q.kcheck > 0) std::memcpy(hu.data() + u_off[j] + (size_t)q.K * st, q.check, (size_t)q.kcheck * st * 8);
+            int* h = hi.data() + i_off[j];
+            h[0] = q.init;
+            if (q.nparts > 0) {
+                int* po = h + 2 + (want_sets ? q.set_cap : 0);
+                std::memcpy(po, q.part_off, (size_t)(q.nparts + 1) * 4);
+                std::memcpy(po + q.nparts + 1, q.part_cols, (size_t)q.part_off[q.nparts] * 4);
+            }
+        }
+        el_u64.reserve(std::max<size_t>(nu, 1));
+        el_int.reserve(std::max<size_t>(ni, 1));
+        if (nscratch) el_scratch.reserve(nscratch);
+        el_jobs.reserve(njobs);
+        el_hjobs.assign(njobs, ElimJob{});
+        size_t sc = 0;
+        size_t lds_sets = 0, lds_trials = 0;
+        for (int j = 0; j < njobs; ++j) {
+            const qsf_elim_job& q = jobs[j];
+            const int st = (q.n + 63) / 64;
+            ElimJob& e = el_hjobs[j];
+            u64* base = el_u64.p + u_off[j];
+            int* ib = el_int.p + i_off[j];
+            e.gen = base;
+            e.check = base + (size_t)q.K * st;
+            e.K = q.K; e.n = q.n; e.stride = st; e.kcheck = std::max(0, q.kcheck);
+            e.every_nonzero = q.every_nonzero ? 1 : 0;
+            e.best = ib;
+            e.nsets = ib + 1;
+            e.set_cap = want_sets ? q.set_cap : 0;
+            e.sets = want_sets ? base + (size_t)q.K * st + (size_t)e.kcheck * st : nullptr;
+            e.ranks = want_sets ? ib + 2 : nullptr;
+            e.nparts = q.nparts > 0 ? q.nparts : 0;
+            e.part_off = e.nparts ? ib + 2 + e.set_cap : nullptr;
+            e.part_cols = e.nparts ? ib + 2 + e.set_cap + e.nparts + 1 : nullptr;
+            e.scratch = elim_scratch_words(q.K, st, 1) ? el_scratch.p + sc : nullptr;
+            sc += elim_scratch_words(q.K, st, 1) * (size_t)std::max(1, batch);
+            lds_sets = std::max(lds_sets, info_sets_lds_bytes(e));
+            lds_trials = std::max(lds_trials, is_trials_lds_bytes(e));
+            if (want_sets) el_stats.sets_bytes += (u64)q.K * st * 8;   // generator read once
+        }
+        HIPCHK(hipMemcpyAsync(el_u64.p, hu.data(), nu * 8, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(el_int.p, hi.data(), ni * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(el_jobs.p, el_hjobs.data(), njobs * sizeof(ElimJob), hipMemcpyHostToDevice, stream));
+        if (want_sets) {
+            HIPCHK(hipEventRecord(ev0, stream));
+            HIPCHK(launch_info_sets(el_jobs.p, njobs, lds_sets, stream));
+            HIPCHK(hipEventRecord(ev1, stream));
+        }
+        hipEvent_t b0 = nullptr, b1 = nullptr;
+        if (want_bound) {
+            HIPCHK(hipEventCreate(&b0));
+            HIPCHK(hipEventCreate(&b1));
+            HIPCHK(hipEventRecord(b0, stream));
+            for (int t0 = 0; t0 < trials; t0 += batch)
+                HIPCHK(launch_is_trials(el_jobs.p, njobs, lds_trials, seed, t0, std::min(batch, trials - t0), stream));
+            HIPCHK(hipEventRecord(b1, stream));
+        }
+        HIPCHK(hipMemcpyAsync(hu.data(), el_u64.p, nu * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(hi.data(), el_int.p, ni * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-        is_ms = ms;
-        *out = std::min(result, init);
+        if (want_sets) { HIPCHK(hipEventElapsedTime(&ms, ev0, ev1)); el_stats.sets_ms = ms; }
+        if (want_bound) {
+            HIPCHK(hipEventElapsedTime(&ms, b0, b1));
+            el_stats.bound_ms = ms;
+            el_stats.trials = trials;
+            (void)hipEventDestroy(b0);
+            (void)hipEventDestroy(b1);
+        }
+        for (int j = 0; j < njobs; ++j) {
+            qsf_elim_job& q = jobs[j];
+            const int st = (q.n + 63) / 64;
+            const int* h = hi.data() + i_off[j];
+            q.best = want_bound ? std::min(q.init, h[0]) : q.init;
+            if (!want_sets) continue;
+            q.nsets = h[1];
+            const int keep = std::min(q.nsets, q.set_cap);
+            std::memcpy(q.ranks, h + 2, (size_t)keep * 4);
+            std::memcpy(q.sets, hu.data() + u_off[j] + (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st,
+                        (size_t)keep * q.K * st * 8);
+            el_stats.sets_bytes += (u64)keep * q.K * st * 8;   // systematic generators written
+        }
         return 0;
     }
-    float is_ms = 0;
 
     // ---- min mode -------------------------------------------------------------------
     int enumerate(const qsf_enum_plan& p, int floor, int* best_out) {
@@ -1003,19 +1090,36 @@ int qsf_hip_collect(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits,
     return run_guarded("qsf_hip_collect", [&](qsa::Engine& e) { return e.collect(*plan, keep_weight, hits, rows); });
 }
 
-int qsf_hip_is_bound(const uint64_t* gen, int K, int n, const uint64_t* check, int kcheck, int every_nonzero,
-                     int trials, uint64_t seed, int init, int* best_out) {
-    if (!best_out || K < 0 || n <= 0 || n > qsa::kElimMaxN || (K > 0 && !gen) || kcheck < 0 ||
-        (kcheck > 0 && !check) || trials < 0) {
-        engine().err = "qsf_hip_is_bound: bad arguments (n must be in [1, 4096])";
-        return 1;
+int qsf_hip_eliminate(qsf_elim_job* jobs, int njobs, int trials, uint64_t seed, int what) {
+    if (!jobs || njobs < 0 || trials < 0) { engine().err = "qsf_hip_eliminate: bad arguments"; return 1; }
+    for (int j = 0; j < njobs; ++j) {
+        const qsf_elim_job& q = jobs[j];
+        const bool sets = (what & QSF_ELIM_SETS) != 0;
+        if (q.K < 0 || q.n <= 0 || q.n > qsa::kElimMaxN || (q.K > 0 && !q.gen) || q.kcheck < 0 ||
+            (q.kcheck > 0 && !q.check) || (sets && (q.set_cap < 1 || !q.sets || !q.ranks)) ||
+            (q.nparts > 0 && (!q.part_off || !q.part_cols))) {
+            engine().err = "qsf_hip_eliminate: bad job " + std::to_string(j) + " (n must be in [1, 4096])";
+            return 1;
+        }
     }
-    return run_guarded("qsf_hip_is_bound", [&](qsa::Engine& e) {
-        return e.is_bound(gen, K, n, check, kcheck, every_nonzero, trials, seed, init, best_out);
-    });
+    if (njobs == 0) return 0;
+    return run_guarded("qsf_hip_eliminate", [&](qsa::Engine& e) { return e.eliminate(jobs, njobs, trials, seed, what); });
 }
 
-float qsf_hip_last_is_ms(void) { return engine().is_ms; }
+int qsf_hip_is_bound(const uint64_t* gen, int K, int n, const uint64_t* check, int kcheck, int every_nonzero,
+                     int trials, uint64_t seed, int init, int* best_out) {
+    if (!best_out) { engine().err = "qsf_hip_is_bound: null best_out"; return 1; }
+    qsf_elim_job q{};
+    q.gen = gen; q.K = K; q.n = n; q.check = check; q.kcheck = kcheck; q.every_nonzero = every_nonzero;
+    q.init = init;
+    const int rc = qsf_hip_eliminate(&q, 1, trials, seed, QSF_ELIM_BOUND);
+    if (rc == 0) *best_out = q.best;
+    return rc;
+}
+
+void qsf_hip_last_elim_stats(qsf_hip_elim_stats* out) {
+    if (out) *out = engine().el_stats;
+}
 
 void qsf_hip_free(void* p) { std::free(p); }
 

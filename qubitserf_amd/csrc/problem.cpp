@@ -214,39 +214,6 @@ Problem symplectic_isometric(const BitMat& S) {
 
 // ------------------------------------------------------------------ information sets
 
-static InfoSet eliminate_set(const BitMat& gen, const std::vector<int>& cols) {
-    InfoSet s;
-    s.gen = gen;
-    s.pivots = eliminate(s.gen, cols.data(), (int)cols.size());
-    s.rank = (int)s.pivots.size();
-    return s;
-}
-
-static std::vector<InfoSet> greedy_sets(const BitMat& gen) {
-    std::vector<InfoSet> out;
-    const int n = gen.cols;
-    std::vector<int> live(n);
-    std::iota(live.begin(), live.end(), 0);
-    BitMat work = gen;
-    while (!live.empty()) {
-        std::vector<int> piv = eliminate(work, live.data(), (int)live.size());
-        InfoSet s;
-        s.gen = work;
-        s.rank = (int)piv.size();
-        s.pivots = piv;
-        out.push_back(s);
-        std::vector<char> used(n, 0), nonzero(n, 0);
-        for (int c : piv) used[c] = 1;
-        for (int r = 0; r < work.rows; ++r)
-            for (int c : live) if (work.bit(r, c)) nonzero[c] = 1;
-        std::vector<int> next;
-        for (int c : live) if (!used[c] && nonzero[c]) next.push_back(c);
-        if (next.size() == live.size()) break;
-        live.swap(next);
-    }
-    return out;
-}
-
 namespace {
 
 // Edmonds matroid partition of the column matroid of gen (columns as K-bit vectors),
@@ -350,48 +317,28 @@ class Partitioner {
     std::vector<char> seen_;
 };
 
-int count_full(const std::vector<InfoSet>& s, int K) {
-    int f = 0;
-    for (auto& x : s) f += (x.rank == K);
-    return f;
-}
-
-void sort_by_rank(std::vector<InfoSet>& s) {
-    std::stable_sort(s.begin(), s.end(), [](const InfoSet& a, const InfoSet& b) { return a.rank > b.rank; });
-}
-
 }  // namespace
 
-std::vector<InfoSet> information_sets_from_parts(const BitMat& gen, const std::vector<std::vector<int>>& parts) {
-    std::vector<InfoSet> out;
-    for (auto& cols : parts)
-        if (!cols.empty()) out.push_back(eliminate_set(gen, cols));
-    sort_by_rank(out);
-    return out;
+int nonzero_columns(const BitMat& gen) {
+    std::vector<u64> any(gen.ld, 0);
+    for (int i = 0; i < gen.rows; ++i)
+        for (int k = 0; k < gen.ld; ++k) any[k] |= gen.row(i)[k];
+    int c = 0;
+    for (u64 x : any) c += __builtin_popcountll(x);
+    return c;
 }
 
-std::vector<InfoSet> information_sets(const BitMat& gen) {
-    const int K = gen.rows, n = gen.cols;
-    std::vector<InfoSet> greedy = greedy_sets(gen);
-    if (K == 0) { sort_by_rank(greedy); return greedy; }
-    int nonzero_cols = 0;
-    for (int j = 0; j < n; ++j)
-        for (int i = 0; i < K; ++i)
-            if (gen.bit(i, j)) { ++nonzero_cols; break; }
-    const int gfull = count_full(greedy, K);
-    if (gfull >= nonzero_cols / K || n > 200) { sort_by_rank(greedy); return greedy; }
-    std::vector<std::vector<int>> parts;
+bool matroid_parts(const BitMat& gen, std::vector<std::vector<int>>& parts) {
+    std::vector<std::vector<int>> all;
     Partitioner P(gen);
-    if (!P.run(parts)) { sort_by_rank(greedy); return greedy; }
+    if (!P.run(all)) return false;
     // plain (unstable) sort by size, as the reference orders its classes (css.cpp:354-355)
-    std::sort(parts.begin(), parts.end(),
+    std::sort(all.begin(), all.end(),
               [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
-    std::vector<InfoSet> seq;
-    for (auto& cols : parts)
-        if (!cols.empty()) seq.push_back(eliminate_set(gen, cols));
-    if (seq.empty() || count_full(seq, K) <= gfull) { sort_by_rank(greedy); return greedy; }
-    sort_by_rank(seq);
-    return seq;
+    parts.clear();
+    for (auto& c : all)
+        if (!c.empty()) parts.push_back(c);
+    return true;
 }
 
 }  // namespace qsa

@@ -32,21 +32,13 @@ Problem symplectic_isometric(const BitMat& S);
 bool is_css_symplectic(const BitMat& S);
 void split_css(const BitMat& S, BitMat& Hx, BitMat& Hz);
 
-// One information set: a generator of the same code that is the identity on the set's
-// pivot columns (rows [0, rank)) and zero there on rows [rank, K).
-struct InfoSet {
-    BitMat gen;
-    int rank = 0;
-    std::vector<int> pivots;   // pivot column of row i, i < rank
-};
+// Nonzero columns of gen.
+int nonzero_columns(const BitMat& gen);
 
-// Disjoint information sets covering the nonzero columns: greedy peeling (each round
-// eliminates over the columns left over by the previous rounds), replaced by an
-// Edmonds matroid partition when n <= 200 and the partition yields more full-rank sets
-// (cpp/distfind/css.cpp:177-203, :274-328, :336-366). Returned sorted by rank,
-// descending, stable.
-std::vector<InfoSet> information_sets(const BitMat& gen);
-// Same, from an explicit column partition (each part eliminated over its own columns).
-std::vector<InfoSet> information_sets_from_parts(const BitMat& gen, const std::vector<std::vector<int>>& parts);
+// The Edmonds matroid partition of gen's columns (cpp/distfind/css.cpp:274-328): classes
+// in the reference's order (unstable sort by size, css.cpp:354-355), empty classes
+// dropped. false when the search fails its consistency check (css.cpp:322-327). The
+// information sets themselves are eliminated on the device (qsf_hip_eliminate).
+bool matroid_parts(const BitMat& gen, std::vector<std::vector<int>>& parts);
 
 }  // namespace qsa

@@ -80,6 +80,71 @@ def is_bound(gen, n, *, check=None, trials=256, seed=0x9E3779B9, init=WEIGHT_NON
     return out.value
 
 
+ELIM_SETS, ELIM_BOUND = 1, 2
+
+
+def eliminate(problems, *, what=ELIM_SETS | ELIM_BOUND, trials=256, seed=0x9E3779B9):
+    """Batched device elimination (qsf_hip_eliminate). Each problem is a dict with `gen`
+    (K x stride uint64), `n`, optional `check` (None: every nonzero codeword counts),
+    `parts` (list of column lists), `init`. Returns, per problem, a dict with `best` and,
+    with ELIM_SETS, `sets` (ns x K x stride, peel/part order) and `ranks`."""
+    L = _native.load()
+    jobs = (_native.ElimJob * max(1, len(problems)))()
+    keep = []
+    for j, p in enumerate(problems):
+        n = int(p["n"])
+        stride = (n + 63) // 64
+        g = np.ascontiguousarray(p["gen"], np.uint64).reshape(-1, stride)
+        K = g.shape[0]
+        q = jobs[j]
+        q.gen = g.ctypes.data_as(_native.u64p) if K else None
+        q.K, q.n = K, n
+        chk = p.get("check")
+        if chk is not None:
+            c = np.ascontiguousarray(chk, np.uint64).reshape(-1, stride)
+            q.check, q.kcheck, q.every_nonzero = c.ctypes.data_as(_native.u64p), c.shape[0], 0
+            keep.append(c)
+        else:
+            q.check, q.kcheck, q.every_nonzero = None, 0, 1
+        parts = p.get("parts")
+        if parts:
+            cols = np.array([c for part in parts for c in part] or [0], np.int32)
+            off = np.cumsum([0] + [len(part) for part in parts]).astype(np.int32)
+            q.part_cols = cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            q.part_off = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            q.nparts = len(parts)
+            keep += [cols, off]
+        cap = len(parts) if parts else int(p.get("set_cap", n + 2))
+        sets = np.zeros((max(cap, 1), max(K, 1), stride), np.uint64)
+        ranks = np.zeros(max(cap, 1), np.int32)
+        q.sets = sets.ctypes.data_as(_native.u64p)
+        q.set_cap = max(cap, 1)
+        q.ranks = ranks.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        q.init = int(p.get("init", WEIGHT_NONE))
+        keep += [g, sets, ranks]
+        p["_out"] = (sets, ranks)
+    rc = L.qsf_hip_eliminate(jobs, len(problems), int(trials), int(seed), int(what))
+    if rc:
+        _err(rc, "qsf_hip_eliminate")
+    out = []
+    for j, p in enumerate(problems):
+        sets, ranks = p.pop("_out")
+        q = jobs[j]
+        r = {"best": q.best}
+        if what & ELIM_SETS:
+            ns = min(q.nsets, q.set_cap)
+            r.update(nsets=q.nsets, sets=sets[:ns, : q.K].copy(), ranks=ranks[:ns].tolist())
+        out.append(r)
+    del keep
+    return out
+
+
+def last_elim_stats() -> dict:
+    s = _native.ElimStats()
+    _native.load().qsf_hip_last_elim_stats(ctypes.byref(s))
+    return {f: getattr(s, f) for f, _ in s._fields_}
+
+
 def collect_level(gamma, n, d, keep_weight, *, num_gamma=None, key=None):
     """(rows, stride) uint64 array of every XOR of d rows with weight in [1, keep_weight] (dups allowed)."""
     L = _native.load()

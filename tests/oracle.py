@@ -50,6 +50,10 @@ def lib():
         L.oracle_mitm_distance.argtypes = [i, u8p, i, i, u8p, i, i, i]
         ip = ctypes.POINTER(ctypes.c_int)
         L.oracle_problem_info.argtypes = [i, u8p, i, i, u8p, i, i, ip, ip, ip, ip, ip, i]
+        L.oracle_problem_gen.argtypes = [i, u8p, i, i, u8p, i, i, u64p, ctypes.c_int64, u64p, ctypes.c_int64, ip, ip]
+        L.oracle_greedy_sets.argtypes = [u64p, i, i, u64p, i, ip]
+        L.oracle_part_sets.argtypes = [u64p, i, i, ip, ip, i, u64p, ip]
+        L.oracle_part_sets.restype = None
         L.oracle_is_bound_trials.argtypes = [u64p, i, i, u64p, i, i, i, i, ctypes.c_uint64, i, u64p, ip]
         _o = L
     return _o
@@ -170,6 +174,48 @@ def collect_plan(gamma, n, d, keep, num_gamma=None, threads=0) -> np.ndarray:
         if rows <= cap:
             return out[:rows]
         cap = rows
+
+
+def problem_gen(kind, A, B=None):
+    """(gen[K, stride], check[kcheck, stride] or None when every nonzero codeword counts)."""
+    a, b = _kind_args(kind, A, B)
+    n = a.shape[1]
+    stride = (n + 63) // 64
+    gen = np.zeros((n, stride), np.uint64)
+    chk = np.zeros((n + 1, stride), np.uint64)
+    K, kc = ctypes.c_int(), ctypes.c_int()
+    rc = lib().oracle_problem_gen(kind, _p(a), a.shape[0], a.shape[1], _p(b), b.shape[0], b.shape[1],
+                                  gen.ctypes.data_as(u64p), gen.size, chk.ctypes.data_as(u64p), chk.size,
+                                  ctypes.byref(K), ctypes.byref(kc))
+    assert rc == 0
+    return gen[: K.value].copy(), (chk[: kc.value].copy() if kc.value else None)
+
+
+def greedy_sets(gen, n):
+    """(sets[ns, K, stride], ranks) of the greedy peel, in peel order."""
+    g = np.ascontiguousarray(gen, np.uint64)
+    K = g.shape[0]
+    stride = (n + 63) // 64
+    cap = n + 2
+    out = np.zeros((cap, max(K, 1), stride), np.uint64)
+    ranks = np.zeros(cap, np.int32)
+    ns = lib().oracle_greedy_sets(g.ctypes.data_as(u64p), K, n, out.ctypes.data_as(u64p), cap,
+                                  ranks.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out[:ns, :K], ranks[:ns].tolist()
+
+
+def part_sets(gen, n, parts):
+    g = np.ascontiguousarray(gen, np.uint64)
+    K = g.shape[0]
+    stride = (n + 63) // 64
+    cols = np.array([c for p in parts for c in p] or [0], np.int32)
+    off = np.cumsum([0] + [len(p) for p in parts]).astype(np.int32)
+    out = np.zeros((len(parts), max(K, 1), stride), np.uint64)
+    ranks = np.zeros(len(parts), np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib().oracle_part_sets(g.ctypes.data_as(u64p), K, n, cols.ctypes.data_as(ip), off.ctypes.data_as(ip), len(parts),
+                           out.ctypes.data_as(u64p), ranks.ctypes.data_as(ip))
+    return out[:, :K], ranks.tolist()
 
 
 def is_bound_trials(gen, n, check=None, trials=256, seed=0x9E3779B9, init=WEIGHT_NONE, first=0) -> int:

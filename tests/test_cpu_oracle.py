@@ -386,3 +386,33 @@ def test_is_bound_logical_test():
     assert check.shape[0] > 0
     assert O.is_bound_trials(G, n, check=check, trials=256) >= 6
     assert O.is_bound_trials(G, n, trials=256) < 6
+
+
+@pytest.mark.parametrize("name,kind,mats", [("toric5", 1, codes.toric(5)), ("bb72", 2, codes.bb_family(6)),
+                                            ("hamming4", 0, (codes.hamming_parity(4), None))])
+def test_greedy_sets_are_information_sets(name, kind, mats):
+    """Every set of the peel is an information set: its generator spans the code, is the
+    identity on the set's pivot columns (the columns left over that raise the rank, in
+    order), and the sets are disjoint and cover the nonzero columns."""
+    from qubitserf_amd.engine import unpack_rows
+    gen, _ = O.problem_gen(kind, *mats)
+    n = mats[0].shape[1]
+    G = unpack_rows(gen, n)
+    K = G.shape[0]
+    sets, ranks = O.greedy_sets(gen, n)
+    seen = np.zeros(n, bool)
+    nonzero = G.any(axis=0)
+    for S, r in zip(sets, ranks):
+        M = unpack_rows(S, n)
+        assert codes.gf2_rank(np.vstack([G, M])) == K == codes.gf2_rank(M)
+        # pivots: the columns left over, in increasing order, that raise the column rank
+        piv, rank = [], 0
+        for c in np.flatnonzero(~seen & nonzero):
+            if codes.gf2_rank(G[:, piv + [int(c)]]) > rank:
+                piv, rank = piv + [int(c)], rank + 1
+        assert rank == r
+        for i, c in enumerate(piv):
+            col = M[:, c]
+            assert col[i] == 1 and col.sum() == 1
+            seen[c] = True
+    assert np.array_equal(seen, nonzero)
