@@ -103,6 +103,39 @@ typedef struct {
 /* Copies up to `cap` levels into `out` (may be NULL); returns the number recorded. */
 int distfind_last_trace(DistfindLevel* out, int cap);
 
+/* ---- batched solves (new; no reference counterpart) ---------------------------------
+ * Many codes in one call: every problem is stepped level by level exactly as the single
+ * entry points step it (same distance, lower_bound, proven, levels), and each round
+ * enumerates one level of every unfinished problem in one batched device call
+ * (qsf_hip_enumerate_batch); the information sets and upper bounds of all problems come
+ * from three batched device eliminations. */
+typedef struct {
+    int kind;              /* 0 classical: A = H; 1 CSS: A = Hx, B = Hz */
+    const uint8_t* A;
+    int ar, ac;
+    const uint8_t* B;
+    int br, bc;
+    char which;            /* CSS: 'M' (min of both sides, interleaved), 'Z' or 'X' */
+    const int* order;      /* optional, single-side problems: column order (permutation of
+                              [0, n)) for the greedy information-set peel; NULL = 0..n-1 */
+} DistfindBatchItem;
+
+typedef struct {
+    int rounds;            /* batched enumeration calls */
+    int capped;            /* problems stopped by work_cap (proven = 0) */
+    uint64_t codewords;    /* enumerated in total */
+    double device_ms;      /* enumeration kernels */
+    double setup_s;        /* problem build, information sets and upper bounds */
+    double seconds;        /* whole call */
+} DistfindBatchStats;
+
+/* work_cap > 0: a problem stops before a level would take its enumerated codewords past
+ * work_cap; it then reports its bounds so far with proven = 0. Return codes as
+ * distfind_css_distance; out has `count` entries. */
+int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int verbose,
+                            DistfindResult* out);
+void distfind_last_batch_stats(DistfindBatchStats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,13 @@ int qsf_hip_enumerate(const qsf_enum_plan* plan, int* best_out);
  * number is <= floor or no codeword <= floor exists. */
 int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out);
 
+/* Batched min mode (no reference counterpart: the reference runs one level of one problem
+ * per call). Plan j is one level of its own problem (distinct buffers_key per problem);
+ * best_out[j] is what qsf_hip_enumerate_floor(&plans[j], floors[j], ...) would return
+ * (floors may be NULL: no early stop). Every tileable level's tiles share one persistent
+ * launch per kernel variant; qsf_hip_last_stats then sums codewords and device time. */
+int qsf_hip_enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out);
+
 /* Collect mode (replaces CudaBackend::collect, backend_cuda.cu:561-645): every codeword of
  * Hamming weight in [1, keep_weight] that is the XOR of d rows of one of the first
  * num_gamma generators, `stride` u64 words each, duplicates allowed. On success *hits is a
@@ -85,7 +92,8 @@ void qsf_hip_free(void* p);
  *   partition: the greedy peel — eliminate over the live columns in order (pivot = first
  *   row at/below the rank with a 1), the pivots form a set and the matrix its generator,
  *   continue from that matrix over the live columns that were neither pivots nor zero
- *   columns, stop when a round removes nothing. With a partition (nparts > 0; part p is
+ *   columns, stop when a round removes nothing; the live columns start as 0..n-1, or as
+ *   `order` when given (a proposal's column order). With a partition (nparts > 0; part p is
  *   part_cols[part_off[p] .. part_off[p+1])): set p is gen made systematic over part p
  *   (css.cpp:358-361). Outputs: nsets, and the first min(nsets, set_cap) generators
  *   (set_cap x K x stride u64, in `sets`) and ranks (in `ranks`), caller-owned.
@@ -114,6 +122,7 @@ typedef struct {
     uint64_t* sets;      /* out, set_cap x K x stride */
     int set_cap;
     int* ranks;          /* out, set_cap */
+    const int* order;    /* optional: the peel's initial column order, a permutation of [0, n) */
 } qsf_elim_job;
 int qsf_hip_eliminate(qsf_elim_job* jobs, int njobs, int trials, uint64_t seed, int what);
 

@@ -93,12 +93,25 @@ class ElimJob(ctypes.Structure):
         ("sets", ctypes.POINTER(ctypes.c_uint64)),
         ("set_cap", ctypes.c_int),
         ("ranks", ctypes.POINTER(ctypes.c_int)),
+        ("order", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
 class ElimStats(ctypes.Structure):
     _fields_ = [("sets_ms", ctypes.c_double), ("bound_ms", ctypes.c_double), ("sets_bytes", ctypes.c_uint64),
                 ("jobs", ctypes.c_int), ("trials", ctypes.c_int)]
+
+
+class DistfindBatchItem(ctypes.Structure):
+    """Layout of DistfindBatchItem (include/distfind_capi.h)."""
+    _fields_ = [("kind", ctypes.c_int), ("A", ctypes.POINTER(ctypes.c_uint8)), ("ar", ctypes.c_int),
+                ("ac", ctypes.c_int), ("B", ctypes.POINTER(ctypes.c_uint8)), ("br", ctypes.c_int),
+                ("bc", ctypes.c_int), ("which", ctypes.c_char), ("order", ctypes.POINTER(ctypes.c_int))]
+
+
+class DistfindBatchStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_int), ("capped", ctypes.c_int), ("codewords", ctypes.c_uint64),
+                ("device_ms", ctypes.c_double), ("setup_s", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
 class DistfindLevel(ctypes.Structure):
@@ -147,6 +160,9 @@ def load() -> ctypes.CDLL:
     sig("distfind_version", c_char_p, [])
     sig("distfind_last_error", c_char_p, [])
     sig("distfind_last_trace", c_int, [ctypes.POINTER(DistfindLevel), c_int])
+    sig("distfind_batch_distance", c_int,
+        [ctypes.POINTER(DistfindBatchItem), c_int, ctypes.c_int64, c_int, rp])
+    sig("distfind_last_batch_stats", None, [ctypes.POINTER(DistfindBatchStats)])
 
     vp = ctypes.c_void_p
     sig("codeaut_bz_collect", vp,

@@ -182,6 +182,67 @@ int distfind_last_trace(DistfindLevel* out, int cap) {
     return n;
 }
 
+// ---------------------------------------------------------------- batched solves
+
+namespace {
+thread_local BatchStats g_batch;
+}
+
+int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int verbose,
+                            DistfindResult* out) {
+    if (count < 0 || (count > 0 && (!items || !out))) { g_err = "null batch"; return RC_NULL; }
+    if (int rc = check_args("bz", "gpu")) return rc;
+    std::vector<BatchProblem> probs(count);
+    for (int i = 0; i < count; ++i) {
+        const DistfindBatchItem& it = items[i];
+        if (!it.A || (it.kind == 1 && !it.B)) { g_err = "null matrix in batch item " + std::to_string(i); return RC_NULL; }
+        BatchProblem& bp = probs[i];
+        if (it.kind == 0) {
+            bp.a = classical(BitMat::from_bytes(it.A, it.ar, it.ac));
+        } else if (it.kind == 1) {
+            if (it.ac != it.bc) { g_err = "Hx and Hz column counts differ in batch item " + std::to_string(i); return RC_NULL; }
+            BitMat Hx = BitMat::from_bytes(it.A, it.ar, it.ac), Hz = BitMat::from_bytes(it.B, it.br, it.bc);
+            const char w = it.which == 'z' ? 'Z' : it.which == 'x' ? 'X' : it.which;
+            if (w == 'Z' || w == 'X') {
+                bp.a = css_side(Hx, Hz, w);
+            } else {
+                bp.a = css_side(Hx, Hz, 'Z');
+                bp.b = css_side(Hx, Hz, 'X');
+                bp.pair = true;
+            }
+        } else {
+            g_err = "batch item kind must be 0 (classical) or 1 (CSS)";
+            return RC_METHOD;
+        }
+        if (it.order && !bp.pair) {
+            const int n = it.ac;
+            bp.order.assign(it.order, it.order + n);
+            std::vector<char> seen(n, 0);
+            for (int c : bp.order) {
+                if (c < 0 || c >= n || seen[c]) { g_err = "batch item order is not a permutation"; return RC_NULL; }
+                seen[c] = 1;
+            }
+        }
+    }
+    BzOptions o = options(verbose);
+    o.trace = nullptr;
+    return guarded([&] {
+        std::vector<BzResult> r = bz_batch(probs, o, work_cap > 0 ? (u64)work_cap : 0, &g_batch);
+        for (int i = 0; i < count; ++i) fill(out + i, r[i]);
+        return RC_OK;
+    });
+}
+
+void distfind_last_batch_stats(DistfindBatchStats* out) {
+    if (!out) return;
+    out->rounds = g_batch.rounds;
+    out->capped = g_batch.capped;
+    out->codewords = g_batch.codewords;
+    out->device_ms = g_batch.device_ms;
+    out->setup_s = g_batch.setup_s;
+    out->seconds = g_batch.seconds;
+}
+
 // ---------------------------------------------------------------- codeaut collect
 
 namespace {

@@ -5,6 +5,8 @@
 #include <chrono>
 #include <cstring>
 #include <cstdio>
+#include <memory>
+#include <unordered_map>
 
 #include "../../include/qsf_hip.h"
 
@@ -34,8 +36,8 @@ void say_final(int d, double secs, const char* tag) {
     print_elapsed(secs);
 }
 
-// Information sets of gen as the level loop uses them: systematic generators computed on
-// the device, ordered by rank, descending, stable (bz.cpp:96-97).
+// Information sets of a generator as the level loop uses them: systematic generators
+// computed on the device, ordered by rank, descending, stable (bz.cpp:96-97).
 struct Family {
     int ng = 0;
     std::vector<u64> gamma;   // ng x K x stride
@@ -43,69 +45,86 @@ struct Family {
     int full(int K) const { return (int)std::count(ranks.begin(), ranks.end(), K); }
 };
 
-Family device_sets(const BitMat& gen, const std::vector<std::vector<int>>* parts) {
-    const int K = gen.rows, n = gen.cols, st = words(n);
-    std::vector<int> cols, off;
-    qsf_elim_job q{};
-    q.gen = gen.data.data();
-    q.K = K;
-    q.n = n;
-    q.init = kNoWeight;
-    if (parts) {
-        off.push_back(0);
-        for (auto& p : *parts) {
-            cols.insert(cols.end(), p.begin(), p.end());
-            off.push_back((int)cols.size());
-        }
-        q.part_cols = cols.data();
-        q.part_off = off.data();
-        q.nparts = (int)parts->size();
-    }
-    int cap = parts ? std::max(1, (int)parts->size()) : std::min(n + 1, 16);
-    std::vector<u64> sets;
-    std::vector<int> ranks;
-    for (;;) {   // a greedy peel longer than the first guess reruns with room for all sets
-        sets.assign((size_t)cap * K * st + 1, 0);
-        ranks.assign(cap, 0);
-        q.sets = sets.data();
-        q.set_cap = cap;
-        q.ranks = ranks.data();
-        const int rc = qsf_hip_eliminate(&q, 1, 0, 0, QSF_ELIM_SETS);
-        if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
-        if (q.nsets <= cap) break;
-        cap = q.nsets;
-    }
-    std::vector<int> idx(q.nsets);
-    for (int i = 0; i < q.nsets; ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ranks[a] > ranks[b]; });
-    Family f;
-    f.ng = q.nsets;
-    f.gamma.resize((size_t)f.ng * K * st);
-    for (int i = 0; i < f.ng; ++i) {
-        std::memcpy(f.gamma.data() + (size_t)i * K * st, sets.data() + (size_t)idx[i] * K * st, (size_t)K * st * 8);
-        f.ranks.push_back(ranks[idx[i]]);
-    }
-    return f;
-}
+// One information-set request: greedy peel of gen (over `order` when given), or gen made
+// systematic over each part of `parts`.
+struct SetRequest {
+    const BitMat* gen;
+    const std::vector<int>* order = nullptr;
+    const std::vector<std::vector<int>>* parts = nullptr;
+    Family out;
+};
 
-// Greedy peel, replaced by the matroid partition's sets when n <= 200 and the partition
-// yields more full-rank sets (cpp/distfind/css.cpp:336-366).
-Family information_sets(const BitMat& gen) {
-    Family greedy = device_sets(gen, nullptr);
-    const int K = gen.rows, n = gen.cols;
-    if (K == 0) return greedy;
-    const int gfull = greedy.full(K);
-    if (gfull >= nonzero_columns(gen) / K || n > 200) return greedy;
-    std::vector<std::vector<int>> parts;
-    if (!matroid_parts(gen, parts) || parts.empty()) return greedy;
-    Family seq = device_sets(gen, &parts);
-    return seq.full(K) <= gfull ? greedy : seq;
+// All requests in one device call (a request whose peel outgrows the first capacity guess
+// is redone with room for every set).
+void device_sets(std::vector<SetRequest*>& reqs) {
+    std::vector<SetRequest*> todo = reqs;
+    std::vector<int> caps(reqs.size());
+    std::unordered_map<SetRequest*, int> cap;
+    for (SetRequest* r : reqs)
+        cap[r] = r->parts ? std::max(1, (int)r->parts->size()) : std::min(r->gen->cols + 1, 16);
+    while (!todo.empty()) {
+        const size_t m = todo.size();
+        std::vector<qsf_elim_job> jobs(m);
+        std::vector<std::vector<u64>> sets(m);
+        std::vector<std::vector<int>> ranks(m), cols(m), off(m);
+        for (size_t k = 0; k < m; ++k) {
+            SetRequest& r = *todo[k];
+            const BitMat& g = *r.gen;
+            qsf_elim_job& q = jobs[k];
+            q.gen = g.data.data();
+            q.K = g.rows;
+            q.n = g.cols;
+            q.init = kNoWeight;
+            q.order = r.order ? r.order->data() : nullptr;
+            if (r.parts) {
+                off[k].push_back(0);
+                for (auto& pc : *r.parts) {
+                    cols[k].insert(cols[k].end(), pc.begin(), pc.end());
+                    off[k].push_back((int)cols[k].size());
+                }
+                q.part_cols = cols[k].data();
+                q.part_off = off[k].data();
+                q.nparts = (int)r.parts->size();
+            }
+            const int c = cap[todo[k]];
+            sets[k].assign((size_t)c * g.rows * g.ld + 1, 0);
+            ranks[k].assign(c, 0);
+            q.sets = sets[k].data();
+            q.set_cap = c;
+            q.ranks = ranks[k].data();
+        }
+        const int rc = qsf_hip_eliminate(jobs.data(), (int)m, 0, 0, QSF_ELIM_SETS);
+        if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+        std::vector<SetRequest*> again;
+        for (size_t k = 0; k < m; ++k) {
+            SetRequest& r = *todo[k];
+            const qsf_elim_job& q = jobs[k];
+            if (q.nsets > q.set_cap) {
+                cap[todo[k]] = q.nsets;
+                again.push_back(todo[k]);
+                continue;
+            }
+            const size_t w = (size_t)q.K * r.gen->ld;
+            std::vector<int> idx(q.nsets);
+            for (int t = 0; t < q.nsets; ++t) idx[t] = t;
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ranks[k][a] > ranks[k][b]; });
+            r.out = Family{};
+            r.out.ng = q.nsets;
+            r.out.gamma.resize((size_t)q.nsets * w);
+            for (int t = 0; t < q.nsets; ++t) {
+                std::memcpy(r.out.gamma.data() + (size_t)t * w, sets[k].data() + (size_t)idx[t] * w, w * 8);
+                r.out.ranks.push_back(ranks[k][idx[t]]);
+            }
+        }
+        todo.swap(again);
+    }
 }
 
 // One subproblem's level-stepping state. Owns the flat generator / detector arrays the
 // device caches under `key` for the whole solve.
 struct Side {
     const char* tag = "";
+    const Problem* prob = nullptr;
     bool has = false, even = false;
     int n = 0, K = 0, stride = 0, ng = 0, kcheck = 0;
     std::vector<u64> gamma, check;
@@ -114,8 +133,12 @@ struct Side {
     int inner = kNoWeight, outer = 0, level = 0;
     bool proven = false, resolved = false;
     const BzOptions* opt = nullptr;
+    u64 work = 0;          // codewords enumerated so far
 
-    void setup(const Problem& p, const BzOptions& o, const char* t) {
+    // Host part of the set-up; the information sets and the upper bound come from
+    // setup_sides, on the device, for many sides at once.
+    void prepare(const Problem& p, const BzOptions& o, const char* t) {
+        prob = &p;
         opt = &o;
         tag = t;
         n = p.n;
@@ -124,26 +147,11 @@ struct Side {
         even = p.even;
         has = !(K == 0 || (!p.every_nonzero && p.detector.rows == 0));
         if (!has) { resolved = true; return; }
-        Family fam;
-        if (o.parts && !o.parts->empty()) {
-            std::vector<std::vector<int>> parts;
-            for (auto& c : *o.parts)
-                if (!c.empty()) parts.push_back(c);
-            fam = device_sets(p.gen, &parts);
-        } else {
-            fam = information_sets(p.gen);
-        }
-        ng = fam.ng;
-        gamma = std::move(fam.gamma);
-        ranks = std::move(fam.ranks);
         kcheck = p.every_nonzero ? 0 : p.detector.rows;
         check.assign((size_t)std::max(1, kcheck) * stride, 0);
         if (kcheck) std::memcpy(check.data(), p.detector.data.data(), (size_t)kcheck * stride * 8);
         key = fresh_key();
         inner = p.seed_upper > 0 ? p.seed_upper : kNoWeight;
-        int rc = qsf_hip_is_bound(p.gen.data.data(), K, n, check.data(), kcheck, p.every_nonzero ? 1 : 0,
-                                  o.is_trials, o.seed, inner, &inner);
-        if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
     }
 
     int active(int d) const {
@@ -152,26 +160,32 @@ struct Side {
         return a;
     }
 
-    void step(int d) {
+    // Level d's plan; false when no generator is active at d (nothing to enumerate).
+    bool plan(int d, qsf_enum_plan& pl) const {
         const int act = active(d);
-        int found = kNoWeight;
-        qsf_hip_stats st{};
-        auto t0 = clk::now();
-        if (act > 0) {
-            qsf_enum_plan pl{};
-            pl.n = n; pl.stride = stride; pl.K = K; pl.d = d;
-            pl.num_gamma = act; pl.num_gamma_total = ng;
-            pl.gamma = gamma.data(); pl.kcheck = kcheck; pl.check = check.data();
-            pl.current_best = inner; pl.buffers_key = key;
-            // every codeword not enumerated before this level weighs >= outer (the BZ bound
-            // after level d-1), so a logical that light ends the level early
-            int rc = qsf_hip_enumerate_floor(&pl, outer, &found);
-            if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
-            qsf_hip_last_stats(&st);
-        }
-        double ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        if (act == 0) return false;
+        pl = qsf_enum_plan{};
+        pl.n = n; pl.stride = stride; pl.K = K; pl.d = d;
+        pl.num_gamma = act; pl.num_gamma_total = ng;
+        pl.gamma = gamma.data(); pl.kcheck = kcheck; pl.check = check.data();
+        pl.current_best = inner; pl.buffers_key = key;
+        return true;
+    }
+
+    u64 level_work(int d) const {
+        const int act = active(d);
+        long double c = 1;
+        for (int i = 0; i < d; ++i) c = c * (K - i) / (i + 1);
+        const long double w = c * act;
+        return w > 1.8e19L ? ~0ull : (u64)(w + 0.5L);
+    }
+
+    // Bounds after level d, given what its enumeration found.
+    void apply(int d, int found, double ms, const qsf_hip_stats& st) {
+        const int act = active(d);
         inner = std::min(inner, found);
         ++level;
+        work += st.codewords;
         outer = 0;
         for (int g = 0; g < ng; ++g) outer += std::max(0, (d + 1) - (K - ranks[g]));
         if (even) outer += outer & 1;
@@ -181,7 +195,78 @@ struct Side {
                          (unsigned long long)st.medium_hits, (unsigned long long)st.slow_hits, ms, st.kernel_ms);
         if (opt->trace) opt->trace->push_back(LevelTrace{*tag ? tag[0] : 'S', d, act, ng, K, inner, outer, ms, st});
     }
+
+    void step(int d) {
+        int found = kNoWeight;
+        qsf_hip_stats st{};
+        auto t0 = clk::now();
+        qsf_enum_plan pl;
+        if (plan(d, pl)) {
+            // every codeword not enumerated before this level weighs >= outer (the BZ bound
+            // after level d-1), so a logical that light ends the level early
+            int rc = qsf_hip_enumerate_floor(&pl, outer, &found);
+            if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+            qsf_hip_last_stats(&st);
+        }
+        apply(d, found, std::chrono::duration<double, std::milli>(clk::now() - t0).count(), st);
+    }
 };
+
+// Information sets (greedy peel over the proposal's order when given, then the matroid
+// partition's sets when n <= 200 and they include more full-rank sets, css.cpp:336-366)
+// and the random information-set upper bound of every side, in three device calls.
+void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<int>*>& orders) {
+    std::vector<Side*> live;
+    std::vector<SetRequest> greedy(sides.size());
+    std::vector<SetRequest*> reqs;
+    for (size_t i = 0; i < sides.size(); ++i) {
+        if (!sides[i]->has) continue;
+        greedy[i].gen = &sides[i]->prob->gen;
+        greedy[i].order = orders.empty() ? nullptr : orders[i];
+        reqs.push_back(&greedy[i]);
+        live.push_back(sides[i]);
+    }
+    if (live.empty()) return;
+    device_sets(reqs);
+    std::vector<std::vector<std::vector<int>>> parts(sides.size());
+    std::vector<SetRequest> part_req(sides.size());
+    std::vector<SetRequest*> preqs;
+    for (size_t i = 0; i < sides.size(); ++i) {
+        Side& S = *sides[i];
+        if (!S.has) continue;
+        const BitMat& gen = S.prob->gen;
+        const int gfull = greedy[i].out.full(S.K);
+        if (gfull >= nonzero_columns(gen) / S.K || gen.cols > 200) continue;
+        if (!matroid_parts(gen, parts[i]) || parts[i].empty()) continue;
+        part_req[i].gen = &gen;
+        part_req[i].parts = &parts[i];
+        preqs.push_back(&part_req[i]);
+    }
+    if (!preqs.empty()) device_sets(preqs);
+    std::vector<qsf_elim_job> jobs;
+    for (size_t i = 0; i < sides.size(); ++i) {
+        Side& S = *sides[i];
+        if (!S.has) continue;
+        Family& f = (part_req[i].gen && part_req[i].out.full(S.K) > greedy[i].out.full(S.K)) ? part_req[i].out
+                                                                                              : greedy[i].out;
+        S.ng = f.ng;
+        S.gamma = std::move(f.gamma);
+        S.ranks = std::move(f.ranks);
+        qsf_elim_job q{};
+        q.gen = S.prob->gen.data.data();
+        q.K = S.K;
+        q.n = S.n;
+        q.check = S.check.data();
+        q.kcheck = S.kcheck;
+        q.every_nonzero = S.prob->every_nonzero ? 1 : 0;
+        q.init = S.inner;
+        jobs.push_back(q);
+    }
+    const BzOptions& o = *live[0]->opt;
+    const int rc = qsf_hip_eliminate(jobs.data(), (int)jobs.size(), o.is_trials, o.seed, QSF_ELIM_BOUND);
+    if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+    for (size_t k = 0; k < live.size(); ++k) live[k]->inner = jobs[k].best;
+}
 
 }  // namespace
 
@@ -189,7 +274,9 @@ BzResult bz_distance(const Problem& p, const BzOptions& opt) {
     auto t0 = clk::now();
     BzResult r;
     Side S;
-    S.setup(p, opt, "");
+    S.prepare(p, opt, "");
+    std::vector<Side*> one{&S};
+    setup_sides(one, {opt.order});
     if (S.has) {
         for (int d = 1; d <= S.K; ++d) {
             auto tl = clk::now();
@@ -213,8 +300,10 @@ BzResult bz_distance(const Problem& p, const BzOptions& opt) {
 BzResult bz_min_interleaved(const Problem& pz, const Problem& px, const BzOptions& opt) {
     auto t0 = clk::now();
     Side Z, X;
-    Z.setup(pz, opt, "Z");
-    X.setup(px, opt, "X");
+    Z.prepare(pz, opt, "Z");
+    X.prepare(px, opt, "X");
+    std::vector<Side*> two{&Z, &X};
+    setup_sides(two, {});
     Side* sides[2] = {&Z, &X};
     int best = kNoWeight;
     if (Z.has) best = std::min(best, Z.inner);
@@ -247,6 +336,182 @@ BzResult bz_min_interleaved(const Problem& pz, const Problem& px, const BzOption
     r.proven = Z.resolved && X.resolved && dist < kNoWeight;
     r.seconds = std::chrono::duration<double>(clk::now() - t0).count();
     return r;
+}
+
+namespace {
+
+// One problem of a batch, stepped exactly as bz_distance (single side) or
+// bz_min_interleaved (pair) would step it, one level per round.
+struct Stepper {
+    Side A, B;
+    bool pair = false;
+    int d = 1, turn = 0, best = kNoWeight, maxw = 0;
+    bool done = false, capped = false;
+    Side* cur = nullptr;
+
+    void init() {
+        if (!pair) {
+            done = !A.has;
+            maxw = A.K;
+            return;
+        }
+        if (A.has) best = std::min(best, A.inner);
+        if (B.has) best = std::min(best, B.inner);
+        maxw = std::max(A.has ? A.K : 0, B.has ? B.K : 0);
+        done = maxw < 1;
+    }
+    void advance() {   // pair: the other side at this level, or the next level
+        if (++turn == 2) {
+            turn = 0;
+            if (A.resolved && B.resolved) { done = true; return; }
+            ++d;
+        }
+    }
+    // The side to step next (cur), at level d; done when the solve is over.
+    void next() {
+        cur = nullptr;
+        if (done) return;
+        if (!pair) {
+            if (d > A.K) done = true;
+            else cur = &A;
+            return;
+        }
+        for (;;) {
+            if (d > maxw) { done = true; return; }
+            Side* S = turn == 0 ? &A : &B;
+            if (!(S->resolved || d > S->K)) {
+                if (S->outer >= best) S->resolved = true;
+                else { cur = S; return; }
+            }
+            advance();
+            if (done) return;
+        }
+    }
+    void after(int found, double ms, const qsf_hip_stats& st) {
+        Side* S = cur;
+        S->apply(d, found, ms, st);
+        if (!pair) {
+            if (S->inner <= S->outer) { S->proven = true; done = true; }
+            else ++d;
+            return;
+        }
+        best = std::min(best, S->inner);
+        if (S->inner <= S->outer) S->proven = S->resolved = true;
+        else if (S->outer >= best) S->resolved = true;
+        advance();
+    }
+    BzResult result() const {
+        BzResult r;
+        if (!pair) {
+            if (A.has) {
+                r.distance = A.inner >= kNoWeight ? -1 : A.inner;
+                r.levels = A.level;
+                r.lower_bound = A.outer;
+                r.proven = A.proven && !capped;
+            }
+            return r;
+        }
+        auto v = [](const Side& S) { return S.has ? S.inner : kNoWeight; };
+        const int dist = std::min(v(A), v(B));
+        r.distance = dist >= kNoWeight ? -1 : dist;
+        const int lb = std::min(A.has ? A.outer : kNoWeight, B.has ? B.outer : kNoWeight);
+        r.lower_bound = lb >= kNoWeight ? 0 : lb;
+        r.levels = std::max(A.level, B.level);
+        r.proven = A.resolved && B.resolved && dist < kNoWeight && !capped;
+        return r;
+    }
+};
+
+}  // namespace
+
+std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions& opt, unsigned long long work_cap,
+                               BatchStats* stats) {
+    auto t0 = clk::now();
+    BatchStats bs;
+    const size_t np = probs.size();
+    std::vector<std::unique_ptr<Stepper>> st(np);
+    std::vector<Side*> sides;
+    std::vector<const std::vector<int>*> orders;
+    for (size_t i = 0; i < np; ++i) {
+        st[i] = std::make_unique<Stepper>();
+        Stepper& S = *st[i];
+        BatchProblem& bp = probs[i];
+        S.pair = bp.pair;
+        S.A.prepare(bp.a, opt, bp.pair ? "Z" : "");
+        sides.push_back(&S.A);
+        orders.push_back(!bp.pair && !bp.order.empty() ? &bp.order : nullptr);
+        if (bp.pair) {
+            S.B.prepare(bp.b, opt, "X");
+            sides.push_back(&S.B);
+            orders.push_back(nullptr);
+        }
+    }
+    setup_sides(sides, orders);
+    bs.setup_s = std::chrono::duration<double>(clk::now() - t0).count();
+    for (auto& S : st) {
+        S->init();
+        S->next();
+    }
+    std::vector<qsf_enum_plan> plans;
+    std::vector<int> floors, found;
+    std::vector<Stepper*> who;
+    std::vector<u64> cw;
+    for (;;) {
+        plans.clear();
+        floors.clear();
+        who.clear();
+        cw.clear();
+        bool any = false;
+        for (auto& up : st) {
+            Stepper& S = *up;
+            while (!S.done && S.cur) {
+                Side* side = S.cur;
+                const u64 w = side->level_work(S.d);
+                const u64 spent = S.A.work + (S.pair ? S.B.work : 0);
+                if (work_cap && (w > work_cap || spent > work_cap - w)) { S.capped = S.done = true; break; }
+                qsf_enum_plan pl;
+                if (side->plan(S.d, pl)) {
+                    plans.push_back(pl);
+                    floors.push_back(side->outer);
+                    who.push_back(&S);
+                    cw.push_back(w);
+                    any = true;
+                    break;
+                }
+                S.after(kNoWeight, 0.0, qsf_hip_stats{});   // no generator active at this level
+                S.next();
+            }
+        }
+        if (!any) break;
+        found.assign(plans.size(), kNoWeight);
+        auto tr = clk::now();
+        const int rc = qsf_hip_enumerate_batch(plans.data(), floors.data(), (int)plans.size(), found.data());
+        if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+        qsf_hip_stats round{};
+        qsf_hip_last_stats(&round);
+        const double ms = std::chrono::duration<double, std::milli>(clk::now() - tr).count();
+        ++bs.rounds;
+        bs.codewords += round.codewords;
+        bs.device_ms += round.kernel_ms;
+        for (size_t k = 0; k < plans.size(); ++k) {
+            qsf_hip_stats one{};
+            one.codewords = cw[k];
+            one.kernel = round.kernel;
+            one.kernel_ms = round.kernel_ms;
+            who[k]->after(found[k], ms, one);
+            who[k]->next();
+        }
+    }
+    std::vector<BzResult> out(np);
+    const double secs = std::chrono::duration<double>(clk::now() - t0).count();
+    for (size_t i = 0; i < np; ++i) {
+        out[i] = st[i]->result();
+        out[i].seconds = secs;
+        bs.capped += st[i]->capped ? 1 : 0;
+    }
+    bs.seconds = secs;
+    if (stats) *stats = bs;
+    return out;
 }
 
 }  // namespace qsa

@@ -30,9 +30,9 @@ struct BzOptions {
     int is_trials = 256;            // random information-set trials for the upper bound
     unsigned long long seed = 0x9e3779b9ull;
     std::vector<LevelTrace>* trace = nullptr;
-    // Optional column partition to build the information sets from (the learned proposal);
-    // empty = the greedy / matroid-partition sets.
-    const std::vector<std::vector<int>>* parts = nullptr;
+    // Optional column order for the greedy information-set peel of a single-side solve (a
+    // proposal's order); null = 0..n-1, the reference's order.
+    const std::vector<int>* order = nullptr;
 };
 
 // Raised when the device engine reports an error; `code` is its qsf_hip_* return code.
@@ -47,5 +47,29 @@ BzResult bz_distance(const Problem& p, const BzOptions& opt);
 // Exact min over two subproblems stepped level by level in turn, a side retiring once its
 // lower bound reaches the best weight found so far (cpp/distfind/bz.cpp:296-357).
 BzResult bz_min_interleaved(const Problem& pz, const Problem& px, const BzOptions& opt);
+
+// One problem of a batch: a single side (classical, or one CSS side), or the two CSS
+// sides of a min-distance solve (pair), with an optional proposal order for a single side.
+struct BatchProblem {
+    Problem a, b;
+    bool pair = false;
+    std::vector<int> order;
+};
+
+struct BatchStats {
+    int rounds = 0;                 // level-synchronous rounds (device launches of the enumerator)
+    int capped = 0;                 // problems stopped by the work cap
+    unsigned long long codewords = 0;
+    double device_ms = 0.0;         // enumeration kernels
+    double setup_s = 0.0;           // information sets and upper bounds (device) incl. host set-up
+    double seconds = 0.0;
+};
+
+// Many problems at once: each is stepped level by level exactly as bz_distance /
+// bz_min_interleaved step it, and every round enumerates one level of every unfinished
+// problem in one batched device call. work_cap > 0 stops a problem before a level would
+// take its enumerated codewords past the cap (it then reports proven = false).
+std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions& opt, unsigned long long work_cap,
+                               BatchStats* stats);
 
 }  // namespace qsa

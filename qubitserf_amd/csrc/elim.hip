@@ -5,7 +5,8 @@
 //    generator over the live columns in order; the pivots form a set and the matrix, now
 //    systematic on them, is that set's generator; the next round continues from that
 //    matrix over the live columns that were neither pivots nor zero columns; stop when a
-//    round removes nothing. One wave per problem. Given an explicit column partition
+//    round removes nothing (the live columns start as 0..n-1, or as a given column
+//    order). One wave per problem. Given an explicit column partition
 //    instead (the matroid partition, css.cpp:274-328), each part is eliminated from the
 //    original generator.
 //  * is_trials: the random information-set upper bound (the heuristic of
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(64) void info_sets(const ElimJob* __restrict__ jobs
         nzc[w] = o;
         used[w] = 0;
     }
-    for (int i = lane; i < n; i += 64) live[i] = i;
+    for (int i = lane; i < n; i += 64) live[i] = J.order ? J.order[i] : i;
     __syncthreads();
 
     int nlive = n, nsets = 0;

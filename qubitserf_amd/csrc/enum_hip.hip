@@ -97,10 +97,9 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
     extern __shared__ u32 rows_full[];       // whole residual+syndrome table when it fits (dynamic size)
 
     const int lane = threadIdx.x;
-    const int K = a.K, RW = a.RW, RD = a.RD, SD = a.SD;
-    int bound = a.collect ? a.keep + 1
-                          : __builtin_amdgcn_readfirstlane(__hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    int cur_g = -1;
+    int cur_job = -1, cur_g = -1;
+    int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
+    const TJob* J = nullptr;
     unsigned slow_entries = 0, medium_entries = 0;
 
     for (;;) {
@@ -110,11 +109,6 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         // it, provably wave-uniform: scalar registers and scalar loops, no exec masking
         const u64 t = uni64(t0);
         if (t >= a.total_tiles) break;
-        if (!a.collect) {
-            int gb = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bound = min(bound, __builtin_amdgcn_readfirstlane(gb));
-            if (bound <= a.floor) break;
-        }
 
         // locate the rectangle holding tile t (rects sorted by tile_base)
         int lo = 0, hi = a.nrects - 1;
@@ -123,6 +117,22 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
             if (a.rects[mid].tile_base <= t) lo = mid; else hi = mid - 1;
         }
         const Rect& Rr = a.rects[uni(lo)];
+        const int job = uni(Rr.job);
+        if (job != cur_job) {   // the problem's parameters, and its running bound
+            J = a.jobs + job;
+            K = uni(J->K); RW = uni(J->RW); RD = uni(J->RD); SD = uni(J->SD); floor = uni(J->floor);
+            bound = a.collect ? a.keep + 1
+                              : uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            cur_job = job;
+            cur_g = -1;
+        }
+        if (!a.collect) {
+            bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+            if (bound <= floor) {
+                if (a.njobs == 1) break;
+                continue;   // this problem is settled; its remaining tiles are skipped
+            }
+        }
         struct {
             u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream, f_lo, f_hi;
             int fixed;
@@ -135,17 +145,17 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
 
         const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
         const int sk = uni(Rr.sk), sbase = uni(Rr.sbase), sm = uni(Rr.sm);
-        const u64 flo = a.flags[2 * g], fhi = a.flags[2 * g + 1];
+        const u64 flo = J->flags[2 * g], fhi = J->flags[2 * g + 1];
         const u64 lflags = range_bits(flo, fhi, lbase, lm);
         const u64 sflags = range_bits(flo, fhi, sbase, sm);
-        const u32* fx = R.fixed >= 0 ? a.fixed + (size_t)R.fixed * (RW + 1) : nullptr;
+        const u32* fx = R.fixed >= 0 ? J->fixed + (size_t)R.fixed * (RW + 1) : nullptr;
         u32 fw[XD];
 #pragma unroll
         for (int f = 0; f < XD; ++f) fw[f] = (fx && f < RD) ? fx[f] : 0u;
         const int foff = fx ? (int)fx[RW] : 0;
-        const u32* gtbl = a.tbl + (size_t)g * K * RW;
+        const u32* gtbl = J->tbl + (size_t)g * K * RW;
 
-        const bool full_in_lds = K * RW <= kLdsFullRows;
+        const bool full_in_lds = K * RW <= a.lds_full;
         const u32* slow_tbl = full_in_lds ? rows_full : gtbl;
         __syncthreads();   // previous tile's readers of LDS are done
         if (g != cur_g) {
@@ -282,14 +292,15 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                             if (w >= 1 && w < bound) {   // bound = keep + 1
                                 const unsigned long long slot = atomicAdd(a.out_rows, 1ull);
                                 if (slot < a.out_cap) {
-                                    const u64* G = a.gamma + (size_t)g * K * a.stride;
-                                    for (int wd = 0; wd < a.stride; ++wd) {
+                                    const int stride = J->stride;
+                                    const u64* G = J->gamma + (size_t)g * K * stride;
+                                    for (int wd = 0; wd < stride; ++wd) {
                                         u64 v = 0;
-                                        for (u64 mm = R.f_lo; mm; mm &= mm - 1) v ^= G[(size_t)__builtin_ctzll(mm) * a.stride + wd];
-                                        for (u64 mm = R.f_hi; mm; mm &= mm - 1) v ^= G[(size_t)(64 + __builtin_ctzll(mm)) * a.stride + wd];
-                                        for (u64 mm = smask; mm; mm &= mm - 1) v ^= G[(size_t)(sbase + __builtin_ctzll(mm)) * a.stride + wd];
-                                        for (u64 mm = lmask; mm; mm &= mm - 1) v ^= G[(size_t)(lbase + __builtin_ctzll(mm)) * a.stride + wd];
-                                        a.out[slot * a.stride + wd] = v;
+                                        for (u64 mm = R.f_lo; mm; mm &= mm - 1) v ^= G[(size_t)__builtin_ctzll(mm) * stride + wd];
+                                        for (u64 mm = R.f_hi; mm; mm &= mm - 1) v ^= G[(size_t)(64 + __builtin_ctzll(mm)) * stride + wd];
+                                        for (u64 mm = smask; mm; mm &= mm - 1) v ^= G[(size_t)(sbase + __builtin_ctzll(mm)) * stride + wd];
+                                        for (u64 mm = lmask; mm; mm &= mm - 1) v ^= G[(size_t)(lbase + __builtin_ctzll(mm)) * stride + wd];
+                                        a.out[slot * stride + wd] = v;
                                     }
                                 }
                             }
@@ -313,7 +324,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                     const int wm = __builtin_amdgcn_readfirstlane(wave_min(mine));
                     if (!a.collect && wm < bound) {
                         bound = wm;
-                        if (lane == 0) atomicMin(a.best, wm);
+                        if (lane == 0) atomicMin(J->best, wm);
                     }
                     __syncthreads();
                 }
@@ -338,9 +349,8 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 }
                 step(bw, boff, s + 1);
             }
-            if (bound <= a.floor) break;
+            if (bound <= floor) break;
         }
-        if (bound <= a.floor) break;
     }
     if (lane == 0 && slow_entries) atomicAdd(a.slow_hits, (unsigned long long)slow_entries);
     if (lane == 0 && medium_entries) atomicAdd(a.slow_hits + 1, (unsigned long long)medium_entries);
@@ -494,8 +504,12 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
-    DevBuf<Rect> rects;
-    DevBuf<u32> fixed;
+    hipEvent_t gev[2 * 8] = {};          // start/stop per kernel-variant launch of a tiled run
+    DevBuf<u64> gbinom;                  // (kMaxTiledK + 1) x (kMaxD + 1) saturating C(n, k) for the tiled kernel
+    DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
+    unsigned char* pin = nullptr;        // its pinned host staging
+    size_t pin_cap = 0;
+    size_t cache_cap = 6;
     DevBuf<int> best;
     DevBuf<unsigned long long> counter;   // [0] tile counter, [1] slow-path entries
     DevBuf<u64> out;
@@ -521,6 +535,22 @@ struct Engine {
             probe_msg = "stream/event creation failed";
             return ok = false;
         }
+        for (hipEvent_t& e : gev)
+            if (hipEventCreate(&e) != hipSuccess) { probe_msg = "event creation failed"; return ok = false; }
+        try {
+            const int bk = kMaxD + 1;
+            std::vector<u64> B((size_t)(kMaxTiledK + 1) * bk, 0);
+            for (int i = 0; i <= kMaxTiledK; ++i) {
+                B[(size_t)i * bk] = 1;
+                for (int j = 1; j < bk && j <= i; ++j)
+                    B[(size_t)i * bk + j] = sat_add(B[(size_t)(i - 1) * bk + j - 1], j <= i - 1 ? B[(size_t)(i - 1) * bk + j] : 0);
+            }
+            gbinom.reserve(B.size());
+            HIPCHK(hipMemcpy(gbinom.p, B.data(), B.size() * 8, hipMemcpyHostToDevice));
+        } catch (const HipError& e) {
+            probe_msg = e.what;
+            return ok = false;
+        }
         return ok = true;
     }
 
@@ -536,7 +566,7 @@ struct Engine {
         auto s = std::make_unique<Solve>();
         build(*s, p);
         cache.push_front(std::move(s));
-        while (cache.size() > 6) cache.pop_back();
+        while (cache.size() > cache_cap) cache.pop_back();
         return *cache.front();
     }
 
@@ -644,7 +674,8 @@ struct Engine {
             u_off[j] = nu;
             nu += (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st + (want_sets ? (size_t)q.set_cap * q.K * st : 0);
             i_off[j] = ni;
-            ni += 2 + (want_sets ? (size_t)q.set_cap : 0) + (q.nparts > 0 ? (size_t)q.nparts + 1 + q.part_off[q.nparts] : 0);
+            ni += 2 + (want_sets ? (size_t)q.set_cap : 0) + (q.nparts > 0 ? (size_t)q.nparts + 1 + q.part_off[q.nparts] : 0) +
+                  (q.order ? (size_t)q.n : 0);
             scratch_sets += elim_scratch_words(q.K, st, 1);
             scratch_trial += elim_scratch_words(q.K, st, 1);
             max_n = std::max(max_n, q.n);
@@ -662,11 +693,13 @@ struct Engine {
             if (q.kcheck > 0) std::memcpy(hu.data() + u_off[j] + (size_t)q.K * st, q.check, (size_t)q.kcheck * st * 8);
             int* h = hi.data() + i_off[j];
             h[0] = q.init;
+            int* po = h + 2 + (want_sets ? q.set_cap : 0);
             if (q.nparts > 0) {
-                int* po = h + 2 + (want_sets ? q.set_cap : 0);
                 std::memcpy(po, q.part_off, (size_t)(q.nparts + 1) * 4);
                 std::memcpy(po + q.nparts + 1, q.part_cols, (size_t)q.part_off[q.nparts] * 4);
+                po += q.nparts + 1 + q.part_off[q.nparts];
             }
+            if (q.order) std::memcpy(po, q.order, (size_t)q.n * 4);
         }
         el_u64.reserve(std::max<size_t>(nu, 1));
         el_int.reserve(std::max<size_t>(ni, 1));
@@ -693,6 +726,7 @@ struct Engine {
             e.nparts = q.nparts > 0 ? q.nparts : 0;
             e.part_off = e.nparts ? ib + 2 + e.set_cap : nullptr;
             e.part_cols = e.nparts ? ib + 2 + e.set_cap + e.nparts + 1 : nullptr;
+            e.order = q.order ? ib + 2 + e.set_cap + (e.nparts ? e.nparts + 1 + q.part_off[q.nparts] : 0) : nullptr;
             e.scratch = elim_scratch_words(q.K, st, 1) ? el_scratch.p + sc : nullptr;
             sc += elim_scratch_words(q.K, st, 1) * (size_t)std::max(1, batch);
             lds_sets = std::max(lds_sets, info_sets_lds_bytes(e));
@@ -745,6 +779,10 @@ struct Engine {
     }
 
     // ---- min mode -------------------------------------------------------------------
+    static bool tileable(const Solve& s, const qsf_enum_plan& p) {
+        return p.K <= kMaxTiledK && s.RW <= kMaxRW && p.d <= 32;
+    }
+
     int enumerate(const qsf_enum_plan& p, int floor, int* best_out) {
         stats = qsf_hip_stats{};
         Solve& s = solve_for(p);
@@ -752,26 +790,78 @@ struct Engine {
         stats.codewords = sat_mul(per_g, (u64)p.num_gamma);
         if (per_g == 0 || p.num_gamma == 0) { *best_out = p.current_best; return 0; }
         if (stats.codewords > (1ull << 62)) throw HipError{4, "level has more than 2^62 codewords"};
+        const bool tiled = tileable(s, p) && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
+        if (tiled) {
+            std::vector<TItem> items{TItem{&s, p, floor, choose_variant(s, p)}};
+            std::vector<int> res;
+            run_tiled(items, false, 0, 0, res);
+            *best_out = std::min(res[0], p.current_best);
+            return 0;
+        }
         best.reserve(1);
         counter.reserve(4);
         HIPCHK(hipMemcpyAsync(best.p, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned long long), stream));
-
-        const bool can_tile = p.K <= kMaxTiledK && s.RW <= kMaxRW && p.d <= 32;
-        const bool tiled = can_tile && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
-        if (tiled) launch_tiled(s, p, floor);
-        else launch_sweep(s, p, false, 0, 0);
-
+        launch_sweep(s, p, false, 0, 0);
         int result = 0;
-        unsigned long long ctr[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(&result, best.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(ctr, counter.p, sizeof(ctr), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        if (tiled) { stats.slow_hits = ctr[1]; stats.medium_hits = ctr[2]; }
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         stats.kernel_ms = ms;
         *best_out = std::min(result, p.current_best);
+        return 0;
+    }
+
+    // Many problems' levels at once: every tileable level's tiles share one persistent
+    // launch per kernel variant; levels the tiled kernel cannot take run on the row-sweep
+    // kernel one after another.
+    int enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out) {
+        stats = qsf_hip_stats{};
+        cache_cap = std::max(cache_cap, (size_t)n + 6);
+        std::vector<Solve*> solves(n, nullptr);
+        for (int j = 0; j < n; ++j) solves[j] = &solve_for(plans[j]);   // cache_cap >= n: none evicted
+        std::vector<TItem> items;
+        std::vector<int> item_of(n, -1), sweep;
+        for (int j = 0; j < n; ++j) {
+            const qsf_enum_plan& p = plans[j];
+            Solve& s = *solves[j];
+            best_out[j] = p.current_best;
+            const u64 per_g = s.C(p.K, p.d);
+            const u64 cw = sat_mul(per_g, (u64)p.num_gamma);
+            if (cw == 0) continue;
+            if (cw > (1ull << 62)) throw HipError{4, "level has more than 2^62 codewords"};
+            stats.codewords = sat_add(stats.codewords, cw);
+            if (tileable(s, p) && policy != 1) {
+                item_of[j] = (int)items.size();
+                items.push_back(TItem{&s, p, floors ? floors[j] : -1, choose_variant(s, p)});
+            } else {
+                sweep.push_back(j);
+            }
+        }
+        double ms_total = 0;
+        if (!items.empty()) {
+            std::vector<int> res;
+            run_tiled(items, false, 0, 0, res);
+            ms_total += stats.kernel_ms;
+            for (int j = 0; j < n; ++j)
+                if (item_of[j] >= 0) best_out[j] = std::min(res[item_of[j]], plans[j].current_best);
+        }
+        best.reserve(1);
+        counter.reserve(4);
+        for (int j : sweep) {
+            const qsf_enum_plan& p = plans[j];
+            HIPCHK(hipMemcpyAsync(best.p, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
+            launch_sweep(*solves[j], p, false, 0, 0);
+            int result = 0;
+            HIPCHK(hipMemcpyAsync(&result, best.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            ms_total += ms;
+            best_out[j] = std::min(result, p.current_best);
+        }
+        stats.kernel_ms = ms_total;
+        stats.kernel = items.empty() ? (sweep.empty() ? 0 : 1) : 2;
         return 0;
     }
 
@@ -892,75 +982,182 @@ struct Engine {
         }
     };
 
-    std::vector<Rect> h_rects;   // host staging, alive until the copies complete
-    std::vector<u32> h_fixed;
+    struct TItem {
+        Solve* s;
+        qsf_enum_plan p;
+        int floor;
+        Variant V;
+    };
 
-    void launch_tiled(Solve& s, const qsf_enum_plan& p, int floor, bool collect = false, int keep = 0, u64 cap = 0) {
-        const int K = p.K, d = p.d, h = s.h;
-        qsf_enum_plan pv = p;
-        if (collect) pv.current_best = keep + 1;
-        const Variant V = choose_variant(s, pv);
-        const int RPL = V.rpl;
-        const int S = kStreamMax;
-        RectBuilder B{s, 0, s.RW, S, 64ull * RPL, {}, {}};
-        for (int g = 0; g < p.num_gamma; ++g) {
-            B.g = g;
-            for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
-                B.split(0, 0, 0, h, ka, h, K - h, d - ka, 0);
+    static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+    void pin_reserve(size_t bytes) {
+        if (bytes <= pin_cap) return;
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_cap = 0;
+        HIPCHK(hipHostMalloc((void**)&pin, bytes));
+        pin_cap = bytes;
+    }
+
+    // One tiled run: the levels of `items` (job i = items[i]) decomposed into rectangles,
+    // grouped by kernel variant. One upload of a block laid out as
+    //   [groups x 4 u64 counters | items x int bounds | items x TJob | rects | fixed rows],
+    // one persistent launch per variant (tiles pulled from the group's counter), one
+    // read-back of counters and bounds. res[i] = the bound job i ended with.
+    void run_tiled(std::vector<TItem>& items, bool collect, int keep, u64 cap, std::vector<int>& res) {
+        static const Variant vtab[8] = {{1, 1, 32}, {1, 2, 16}, {1, 2, 24}, {1, 2, 32},
+                                        {2, 2, 16}, {2, 4, 12}, {3, 4, 12}, {4, 4, 12}};
+        auto vidx = [&](const Variant& v) {
+            for (int k = 0; k < 8; ++k)
+                if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl) return k;
+            return 7;
+        };
+        const int ni = (int)items.size();
+        std::vector<std::vector<Rect>> grect(8);
+        std::vector<u32> hfixed;
+        std::vector<size_t> fixed_off(ni, 0);
+        std::vector<int> glds(8, 0);
+        for (int i = 0; i < ni; ++i) {
+            TItem& it = items[i];
+            const Solve& s = *it.s;
+            const int K = it.p.K, d = it.p.d, h = s.h;
+            const int vi = vidx(it.V);
+            RectBuilder B{s, 0, s.RW, kStreamMax, 64ull * it.V.rpl, {}, {}};
+            for (int g = 0; g < it.p.num_gamma; ++g) {
+                B.g = g;
+                for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
+                    B.split(0, 0, 0, h, ka, h, K - h, d - ka, 0);
+            }
+            for (Rect& r : B.rv) r.job = i;
+            grect[vi].insert(grect[vi].end(), B.rv.begin(), B.rv.end());
+            fixed_off[i] = hfixed.size();
+            hfixed.insert(hfixed.end(), B.fixed.begin(), B.fixed.end());
+            if (K * s.RW <= kLdsFullRows) glds[vi] = std::max(glds[vi], K * s.RW);
         }
-        h_rects = std::move(B.rv);
-        h_fixed = std::move(B.fixed);
-        std::vector<Rect>& rv = h_rects;
-        u64 chunks = 0;
-        for (const Rect& r : rv) chunks += r.lane_blocks * r.stream_chunks;
-        // chunks per tile: large enough to amortise the register tile, small enough to
-        // leave >= 8 tiles per resident wave for load balance
+        // tiles: a lane block x G stream chunks; G large enough to amortise the register
+        // tile, small enough to leave >= 8 tiles per resident wave for load balance
         const u64 waves = (u64)std::max(1, cus) * 16;
-        const int G = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * waves)));
-        u64 tiles = 0;
-        for (Rect& r : rv) {
-            r.stream_groups = (r.stream_chunks + G - 1) / G;
-            r.tile_base = tiles;
-            tiles += r.lane_blocks * r.stream_groups;
+        std::vector<int> gG(8, 1);
+        std::vector<u64> gtiles(8, 0);
+        size_t nrect = 0;
+        for (int v = 0; v < 8; ++v) {
+            u64 chunks = 0;
+            for (const Rect& r : grect[v]) chunks += r.lane_blocks * r.stream_chunks;
+            gG[v] = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * waves)));
+            u64 tiles = 0;
+            for (Rect& r : grect[v]) {
+                r.stream_groups = (r.stream_chunks + gG[v] - 1) / gG[v];
+                r.tile_base = tiles;
+                tiles += r.lane_blocks * r.stream_groups;
+            }
+            gtiles[v] = tiles;
+            nrect += grect[v].size();
         }
-        rects.reserve(rv.size());
-        HIPCHK(hipMemcpyAsync(rects.p, rv.data(), rv.size() * sizeof(Rect), hipMemcpyHostToDevice, stream));
-        fixed.reserve(std::max<size_t>(1, h_fixed.size()));
-        if (!h_fixed.empty())
-            HIPCHK(hipMemcpyAsync(fixed.p, h_fixed.data(), h_fixed.size() * 4, hipMemcpyHostToDevice, stream));
-        TiledArgs a{};
-        a.tbl = s.tbl.p; a.flags = s.flags.p; a.fixed = fixed.p; a.rects = rects.p; a.nrects = (int)rv.size();
-        a.total_tiles = tiles; a.binom = s.dbinom.p; a.bk = s.bk;
-        a.K = K; a.h = h; a.RD = s.RD; a.SD = s.SD; a.RW = s.RW; a.S = S; a.G = G;
-        a.best = best.p; a.next_tile = counter.p; a.slow_hits = counter.p + 1; a.floor = floor;
-        a.collect = collect ? 1 : 0; a.keep = keep; a.stride = p.stride; a.gamma = s.gamma.p;
-        a.out_rows = counter.p + 3; a.out = out.p; a.out_cap = cap;
-        const u64 grid = std::min<u64>(tiles, waves);
+        const size_t o_cnt = 0, o_best = 8 * 4 * sizeof(u64);
+        const size_t o_jobs = align16(o_best + (size_t)ni * sizeof(int));
+        const size_t o_rects = align16(o_jobs + (size_t)ni * sizeof(TJob));
+        const size_t o_fixed = align16(o_rects + nrect * sizeof(Rect));
+        const size_t total = align16(o_fixed + hfixed.size() * sizeof(u32) + 4);
+        pin_reserve(total);
+        blk.reserve(total);
+        std::memset(pin, 0, o_jobs);
+        unsigned char* dev = blk.p;
+        int* hb = reinterpret_cast<int*>(pin + o_best);
+        TJob* hj = reinterpret_cast<TJob*>(pin + o_jobs);
+        for (int i = 0; i < ni; ++i) {
+            const TItem& it = items[i];
+            const Solve& s = *it.s;
+            hb[i] = collect ? keep + 1 : it.p.current_best;
+            TJob J{};
+            J.tbl = s.tbl.p;
+            J.flags = s.flags.p;
+            J.fixed = reinterpret_cast<const u32*>(dev + o_fixed) + fixed_off[i];
+            J.gamma = s.gamma.p;
+            J.best = reinterpret_cast<int*>(dev + o_best) + i;
+            J.K = it.p.K; J.RD = s.RD; J.SD = s.SD; J.RW = s.RW; J.stride = it.p.stride;
+            J.floor = it.floor;
+            hj[i] = J;
+        }
+        std::vector<size_t> grect_off(8, 0);
+        {
+            size_t at = o_rects;
+            for (int v = 0; v < 8; ++v) {
+                grect_off[v] = at;
+                if (!grect[v].empty()) std::memcpy(pin + at, grect[v].data(), grect[v].size() * sizeof(Rect));
+                at += grect[v].size() * sizeof(Rect);
+            }
+        }
+        if (!hfixed.empty()) std::memcpy(pin + o_fixed, hfixed.data(), hfixed.size() * sizeof(u32));
+        HIPCHK(hipMemcpyAsync(dev, pin, total, hipMemcpyHostToDevice, stream));
+
+        auto* cnt = reinterpret_cast<unsigned long long*>(dev + o_cnt);
         stats.kernel = 2;
-        stats.tiles = (int)std::min<u64>(tiles, 0x7fffffff);
-        stats.fd = V.fd;
-        stats.xd = V.xd;
-        stats.rpl = V.rpl;
-        stats.ops_per_codeword = (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
-        HIPCHK(hipEventRecord(ev0, stream));
-        if (grid > 0) {
+        stats.tiles = 0;
+        int used[8] = {0};
+        for (int v = 0; v < 8; ++v) {
+            if (gtiles[v] == 0) continue;
+            used[v] = 1;
+            const Variant& V = vtab[v];
+            TiledArgs a{};
+            a.jobs = reinterpret_cast<const TJob*>(dev + o_jobs);
+            a.njobs = ni;
+            a.rects = reinterpret_cast<const Rect*>(dev + grect_off[v]);
+            a.nrects = (int)grect[v].size();
+            a.total_tiles = gtiles[v];
+            a.binom = gbinom.p;
+            a.bk = kMaxD + 1;
+            a.S = kStreamMax;
+            a.G = gG[v];
+            a.lds_full = glds[v];
+            a.next_tile = cnt + 4 * v;
+            a.slow_hits = cnt + 4 * v + 1;
+            a.collect = collect ? 1 : 0;
+            a.keep = keep;
+            a.out_rows = cnt + 4 * v + 3;
+            a.out = out.p;
+            a.out_cap = cap;
+            const u64 grid = std::min<u64>(gtiles[v], waves);
             const dim3 gd((unsigned)grid), bd(64);
-            const size_t lds = (size_t)(p.K * s.RW <= kLdsFullRows ? p.K * s.RW : 0) * sizeof(u32);
-            const int code = V.fd * 1000 + V.xd * 100 + V.rpl;
-            switch (code) {
-                case 1132: hipLaunchKernelGGL((bz_tiled<1, 1, 32>), gd, bd, lds, stream, a); break;
-                case 1216: hipLaunchKernelGGL((bz_tiled<1, 2, 16>), gd, bd, lds, stream, a); break;
-                case 1224: hipLaunchKernelGGL((bz_tiled<1, 2, 24>), gd, bd, lds, stream, a); break;
-                case 1232: hipLaunchKernelGGL((bz_tiled<1, 2, 32>), gd, bd, lds, stream, a); break;
-                case 2216: hipLaunchKernelGGL((bz_tiled<2, 2, 16>), gd, bd, lds, stream, a); break;
-                case 2412: hipLaunchKernelGGL((bz_tiled<2, 4, 12>), gd, bd, lds, stream, a); break;
-                case 3412: hipLaunchKernelGGL((bz_tiled<3, 4, 12>), gd, bd, lds, stream, a); break;
+            const size_t lds = (size_t)glds[v] * sizeof(u32);
+            HIPCHK(hipEventRecord(gev[2 * v], stream));
+            switch (v) {
+                case 0: hipLaunchKernelGGL((bz_tiled<1, 1, 32>), gd, bd, lds, stream, a); break;
+                case 1: hipLaunchKernelGGL((bz_tiled<1, 2, 16>), gd, bd, lds, stream, a); break;
+                case 2: hipLaunchKernelGGL((bz_tiled<1, 2, 24>), gd, bd, lds, stream, a); break;
+                case 3: hipLaunchKernelGGL((bz_tiled<1, 2, 32>), gd, bd, lds, stream, a); break;
+                case 4: hipLaunchKernelGGL((bz_tiled<2, 2, 16>), gd, bd, lds, stream, a); break;
+                case 5: hipLaunchKernelGGL((bz_tiled<2, 4, 12>), gd, bd, lds, stream, a); break;
+                case 6: hipLaunchKernelGGL((bz_tiled<3, 4, 12>), gd, bd, lds, stream, a); break;
                 default: hipLaunchKernelGGL((bz_tiled<4, 4, 12>), gd, bd, lds, stream, a); break;
             }
             HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(gev[2 * v + 1], stream));
+            stats.tiles = (int)std::min<u64>((u64)stats.tiles + gtiles[v], 0x7fffffff);
+            stats.fd = V.fd;
+            stats.xd = V.xd;
+            stats.rpl = V.rpl;
+            stats.ops_per_codeword = (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
         }
-        HIPCHK(hipEventRecord(ev1, stream));
+        HIPCHK(hipMemcpyAsync(pin, dev, o_jobs, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        res.assign(hb, hb + ni);
+        const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(pin + o_cnt);
+        double ms_total = 0;
+        stats.slow_hits = stats.medium_hits = 0;
+        out_rows_last = 0;
+        for (int v = 0; v < 8; ++v) {
+            if (!used[v]) continue;
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, gev[2 * v], gev[2 * v + 1]));
+            ms_total += ms;
+            stats.slow_hits += hc[4 * v + 1];
+            stats.medium_hits += hc[4 * v + 2];
+            out_rows_last += hc[4 * v + 3];
+        }
+        stats.kernel_ms = ms_total;
     }
+    unsigned long long out_rows_last = 0;   // collect mode: rows the last tiled run produced
 
     void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap) {
         if (p.stride > 16) throw HipError{4, "row-sweep kernel supports n <= 1024"};
@@ -997,24 +1194,31 @@ struct Engine {
         if (stats.codewords > (1ull << 62)) throw HipError{4, "level has more than 2^62 codewords"};
         best.reserve(1);
         counter.reserve(4);
-        const bool can_tile = p.K <= kMaxTiledK && s.RW <= kMaxRW && p.d <= 32;
-        const bool tiled = can_tile && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
+        const bool tiled = tileable(s, p) && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
         u64 cap = 1 << 16;
         for (;;) {
             out.reserve(cap * (u64)p.stride);
-            HIPCHK(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned long long), stream));
-            if (tiled) launch_tiled(s, p, -1, true, keep, cap);
-            else launch_sweep(s, p, true, keep, cap);
             unsigned long long got = 0;
-            HIPCHK(hipMemcpyAsync(&got, counter.p + 3, sizeof(got), hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
+            if (tiled) {
+                qsf_enum_plan pv = p;
+                pv.current_best = keep + 1;
+                std::vector<TItem> items{TItem{&s, p, -1, choose_variant(s, pv)}};
+                std::vector<int> res;
+                run_tiled(items, true, keep, cap, res);
+                got = out_rows_last;
+            } else {
+                HIPCHK(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned long long), stream));
+                launch_sweep(s, p, true, keep, cap);
+                HIPCHK(hipMemcpyAsync(&got, counter.p + 3, sizeof(got), hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+                stats.kernel_ms = ms;
+            }
             if (got > cap) { cap = got; continue; }   // overflow: rerun with room for all
             u64* h = (u64*)std::malloc(std::max<u64>(1, got) * p.stride * 8);
             if (!h) throw HipError{3, "host allocation failed"};
             if (got) HIPCHK(hipMemcpy(h, out.p, got * p.stride * 8, hipMemcpyDeviceToHost));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-            stats.kernel_ms = ms;
             *hits = h;
             *rows = (int64_t)got;
             return 0;
@@ -1025,6 +1229,16 @@ struct Engine {
 Engine& engine() {
     static Engine* e = new Engine();   // never destroyed: device teardown order at exit is unsafe
     return *e;
+}
+
+bool is_permutation(const int* order, int n) {   // null counts as the identity
+    if (!order) return true;
+    std::vector<char> seen(n, 0);
+    for (int i = 0; i < n; ++i) {
+        if (order[i] < 0 || order[i] >= n || seen[order[i]]) return false;
+        seen[order[i]] = 1;
+    }
+    return true;
 }
 
 bool plan_ok(const qsf_enum_plan* p, std::string& why) {
@@ -1077,6 +1291,19 @@ int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out)
     return run_guarded("qsf_hip_enumerate", [&](qsa::Engine& e) { return e.enumerate(*plan, floor, best_out); });
 }
 
+int qsf_hip_enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out) {
+    if (n < 0 || (n > 0 && (!plans || !best_out))) { engine().err = "qsf_hip_enumerate_batch: bad arguments"; return 1; }
+    for (int j = 0; j < n; ++j) {
+        std::string why;
+        if (!qsa::plan_ok(plans + j, why)) {
+            engine().err = "qsf_hip_enumerate_batch: plan " + std::to_string(j) + ": " + why;
+            return 1;
+        }
+    }
+    if (n == 0) return 0;
+    return run_guarded("qsf_hip_enumerate_batch", [&](qsa::Engine& e) { return e.enumerate_batch(plans, floors, n, best_out); });
+}
+
 int qsf_hip_enumerate(const qsf_enum_plan* plan, int* best_out) {
     return qsf_hip_enumerate_floor(plan, -1, best_out);
 }
@@ -1097,7 +1324,7 @@ int qsf_hip_eliminate(qsf_elim_job* jobs, int njobs, int trials, uint64_t seed, 
         const bool sets = (what & QSF_ELIM_SETS) != 0;
         if (q.K < 0 || q.n <= 0 || q.n > qsa::kElimMaxN || (q.K > 0 && !q.gen) || q.kcheck < 0 ||
             (q.kcheck > 0 && !q.check) || (sets && (q.set_cap < 1 || !q.sets || !q.ranks)) ||
-            (q.nparts > 0 && (!q.part_off || !q.part_cols))) {
+            (q.nparts > 0 && (!q.part_off || !q.part_cols)) || !qsa::is_permutation(q.order, q.n)) {
             engine().err = "qsf_hip_eliminate: bad job " + std::to_string(j) + " (n must be in [1, 4096])";
             return 1;
         }

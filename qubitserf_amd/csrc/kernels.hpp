@@ -44,6 +44,7 @@ constexpr int kBigOffset = (1 << 30) + (1 << 29);  // masks lanes past the end o
 // codeword is XORed into every stream element). The register tile holds subsets of L,
 // the stream subsets of Sr.
 struct Rect {
+    int job;               // index into TiledArgs::jobs
     int g;
     int lbase, lm, lk;     // lane list: lk-subsets of rows [lbase, lbase + lm)
     int sbase, sm, sk;     // stream list: sk-subsets of rows [sbase, sbase + sm)
@@ -56,25 +57,32 @@ struct Rect {
     u64 tile_base;         // index of this rectangle's first tile
 };
 
-struct TiledArgs {
+// One problem's level inside a tiled launch (a launch may hold many problems' levels).
+struct TJob {
     const u32* tbl;          // [g][row][RW]: RD residual dwords, then SD syndrome dwords
     const u64* flags;        // [g][2]: flagged rows 0..63 and 64..127 as bit masks
     const u32* fixed;        // [f][RW + 1]: residual+syndrome of a fixed row set, then its flagged count
-    const Rect* rects;
+    const u64* gamma;        // [g][row][stride] full rows (collect mode)
+    int* best;               // running bound; starts at current_best
+    int K, RD, SD, RW, stride;
+    int floor;               // stop once best <= floor (floor < 0: never)
+};
+
+struct TiledArgs {
+    const TJob* jobs;
+    int njobs;
+    const Rect* rects;       // sorted by tile_base
     int nrects;
     u64 total_tiles;
-    const u64* binom;        // C(n, k) at binom[n * bk + k], n <= K, k < bk, saturating
+    const u64* binom;        // C(n, k) at binom[n * bk + k], n <= kMaxTiledK, k < bk, saturating
     int bk;
-    int K, h, RD, SD, RW;
     int S;                   // stream chunk length (<= kStreamMax)
     int G;                   // stream chunks per tile (the register tile is built once per tile)
-    int* best;               // running bound; starts at current_best
+    int lds_full;            // dwords of dynamic LDS for a job's whole table (0: none)
     unsigned long long* next_tile;
-    unsigned long long* slow_hits;   // slow-path entries (diagnostics)
-    int floor;               // stop once best <= floor (floor < 0: never)
-    // collect mode: every codeword of weight in [1, keep] is written out in full
-    int collect, keep, stride;
-    const u64* gamma;        // [g][row][stride] full rows
+    unsigned long long* slow_hits;   // [0] slow-path entries, [1] medium-test entries (diagnostics)
+    // collect mode (one job): every codeword of weight in [1, keep] is written out in full
+    int collect, keep;
     unsigned long long* out_rows;
     u64* out;
     u64 out_cap;             // rows

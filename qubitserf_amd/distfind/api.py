@@ -151,3 +151,46 @@ def last_trace():
         rec["side"] = chr(rec["side"])
         out.append(rec)
     return out
+
+
+def batch_distance(problems, *, work_cap=0, verbose=False):
+    """Distances of many codes in one batched solve (``distfind_batch_distance``).
+
+    Each problem is ``("classical", H)``, ``("classical", H, order)`` or
+    ``("css", Hx, Hz, which)``; ``order`` is a column order for the information-set peel.
+    Returns a list of ``DistfindResult`` (distance, lower_bound, proven, levels), identical
+    to the single-code entry points' results; with ``work_cap`` > 0 a problem whose next
+    level would exceed that many codewords stops with ``proven == 0``."""
+    L = _native.load()
+    _normalize_backend("gpu")
+    items = (_native.DistfindBatchItem * max(1, len(problems)))()
+    keep = []
+    for i, p in enumerate(problems):
+        it = items[i]
+        if p[0] == "classical":
+            H = _as_u8(p[1])
+            it.kind, it.A, it.ar, it.ac = 0, _ptr(H), H.shape[0], H.shape[1]
+            keep.append(H)
+            if len(p) > 2 and p[2] is not None:
+                o = np.ascontiguousarray(p[2], np.int32)
+                it.order = o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+                keep.append(o)
+        elif p[0] == "css":
+            Hx, Hz = _as_css(p[1], p[2])
+            it.kind, it.A, it.ar, it.ac = 1, _ptr(Hx), Hx.shape[0], Hx.shape[1]
+            it.B, it.br, it.bc = _ptr(Hz), Hz.shape[0], Hz.shape[1]
+            it.which = _which(p[3] if len(p) > 3 else "min").encode()
+            keep += [Hx, Hz]
+        else:
+            raise ValueError("problem kind must be 'classical' or 'css'")
+    out = (_native.DistfindResult * max(1, len(problems)))()
+    rc = L.distfind_batch_distance(items, len(problems), int(work_cap), 1 if verbose else 0, out)
+    _check(rc, "batch_distance")
+    del keep
+    return [out[i] for i in range(len(problems))]
+
+
+def last_batch_stats() -> dict:
+    s = _native.DistfindBatchStats()
+    _native.load().distfind_last_batch_stats(ctypes.byref(s))
+    return {f: getattr(s, f) for f, _ in s._fields_}

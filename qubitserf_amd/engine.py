@@ -86,7 +86,7 @@ ELIM_SETS, ELIM_BOUND = 1, 2
 def eliminate(problems, *, what=ELIM_SETS | ELIM_BOUND, trials=256, seed=0x9E3779B9):
     """Batched device elimination (qsf_hip_eliminate). Each problem is a dict with `gen`
     (K x stride uint64), `n`, optional `check` (None: every nonzero codeword counts),
-    `parts` (list of column lists), `init`. Returns, per problem, a dict with `best` and,
+    `parts` (list of column lists), `order` (the peel's column order), `init`. Returns, per problem, a dict with `best` and,
     with ELIM_SETS, `sets` (ns x K x stride, peel/part order) and `ranks`."""
     L = _native.load()
     jobs = (_native.ElimJob * max(1, len(problems)))()
@@ -114,6 +114,11 @@ def eliminate(problems, *, what=ELIM_SETS | ELIM_BOUND, trials=256, seed=0x9E377
             q.part_off = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
             q.nparts = len(parts)
             keep += [cols, off]
+        order = p.get("order")
+        if order is not None:
+            o = np.ascontiguousarray(order, np.int32)
+            q.order = o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            keep.append(o)
         cap = len(parts) if parts else int(p.get("set_cap", n + 2))
         sets = np.zeros((max(cap, 1), max(K, 1), stride), np.uint64)
         ranks = np.zeros(max(cap, 1), np.int32)
