@@ -118,6 +118,11 @@ typedef struct {
     char which;            /* CSS: 'M' (min of both sides, interleaved), 'Z' or 'X' */
     const int* order;      /* optional, single-side problems: column order (permutation of
                               [0, n)) for the greedy information-set peel; NULL = 0..n-1 */
+    const float* scores;   /* optional, single-side problems: a proposal's score per column */
+    int candidates;        /* with scores: column orders sampled from the scores (descending
+                              score + Gumbel noise); the peel with the least predicted BZ work
+                              replaces the natural one when strictly better */
+    uint64_t proposal_seed;
 } DistfindBatchItem;
 
 typedef struct {

@@ -123,6 +123,8 @@ typedef struct {
     int set_cap;
     int* ranks;          /* out, set_cap */
     const int* order;    /* optional: the peel's initial column order, a permutation of [0, n) */
+    int* set_of_col;     /* optional out (QSF_ELIM_SETS, n ints): index of the set (in peel or
+                            part order) whose pivot column c became, -1 for none */
 } qsf_elim_job;
 int qsf_hip_eliminate(qsf_elim_job* jobs, int njobs, int trials, uint64_t seed, int what);
 

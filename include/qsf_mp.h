@@ -1,0 +1,57 @@
+/* Device kernels of the learned information-set proposal (no reference counterpart: the
+ * reference builds its information sets greedily, cpp/distfind/css.cpp:177-203). A
+ * three-layer message-passing scorer on the Tanner graph, BF16 operands with FP32
+ * accumulation. All tensor pointers are device pointers; `stream` is a hipStream_t (NULL =
+ * the null stream). Kernel calls are asynchronous on that stream. Return 0 on success, 1 bad
+ * arguments, 3 launch failure. */
+#ifndef QSF_MP_H
+#define QSF_MP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device memory for the scorer's tensors (hipMalloc / hipMemcpy / hipMemset on the null
+ * stream). kind: 1 host to device, 2 device to host, 3 device to device. */
+void* qsf_dev_alloc(size_t bytes);
+void qsf_dev_free(void* p);
+int qsf_dev_copy(void* dst, const void* src, size_t bytes, int kind);
+int qsf_dev_zero(void* dst, size_t bytes);
+int qsf_dev_sync(void);
+
+/* C[b] (M x N) = op(A[b]) (M x K) * op(B[b]) (K x N), b < batch, FP32 accumulation.
+ * A, B: BF16 (uint16 bit patterns). op(A)[i][k] = transA ? A[k lda + i] : A[i lda + k];
+ * op(B)[k][j] = transB ? B[j ldb + k] : B[k ldb + j]. C: FP32, or BF16 (out_bf16 = 1,
+ * round to nearest even). accumulate = 1 adds the old C; bias (FP32, N values, may be
+ * NULL) is added per column; relu = 1 clamps at 0. Batch strides sA, sB, sC in elements. */
+typedef struct {
+    const void* A;
+    const void* B;
+    void* C;
+    const float* bias;
+    int batch, M, N, K, lda, ldb, ldc;
+    long long sA, sB, sC;
+    int transA, transB, out_bf16, accumulate, relu;
+} qsf_mp_gemm_desc;
+int qsf_mp_gemm(const qsf_mp_gemm_desc* d, void* stream);
+
+int qsf_mp_to_bf16(const float* x, uint16_t* y, long long n, void* stream);
+/* dy[i] = 0 where the BF16 activation y[i] <= 0 */
+int qsf_mp_relu_bwd(const uint16_t* y, float* dy, long long n, void* stream);
+/* out[j] (+)= sum_r x[r cols + j] */
+int qsf_mp_colsum(const float* x, float* out, long long rows, int cols, int accumulate, void* stream);
+/* *loss += sum_i mask_i BCE(z_i, t_i) inv_count; dz_i = mask_i (sigmoid(z_i) - t_i) inv_count */
+int qsf_mp_bce(const float* z, const float* t, const float* mask, float* dz, float* loss, long long n, float inv_count,
+               void* stream);
+/* Adam step `step` (1-based) on FP32 weights w with moments m, v; wb (may be NULL) gets BF16 copies */
+int qsf_mp_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, long long n, float lr, float b1, float b2,
+                float eps, int step, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -94,6 +94,7 @@ class ElimJob(ctypes.Structure):
         ("set_cap", ctypes.c_int),
         ("ranks", ctypes.POINTER(ctypes.c_int)),
         ("order", ctypes.POINTER(ctypes.c_int)),
+        ("set_of_col", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
@@ -106,7 +107,9 @@ class DistfindBatchItem(ctypes.Structure):
     """Layout of DistfindBatchItem (include/distfind_capi.h)."""
     _fields_ = [("kind", ctypes.c_int), ("A", ctypes.POINTER(ctypes.c_uint8)), ("ar", ctypes.c_int),
                 ("ac", ctypes.c_int), ("B", ctypes.POINTER(ctypes.c_uint8)), ("br", ctypes.c_int),
-                ("bc", ctypes.c_int), ("which", ctypes.c_char), ("order", ctypes.POINTER(ctypes.c_int))]
+                ("bc", ctypes.c_int), ("which", ctypes.c_char), ("order", ctypes.POINTER(ctypes.c_int)),
+                ("scores", ctypes.POINTER(ctypes.c_float)), ("candidates", ctypes.c_int),
+                ("proposal_seed", ctypes.c_uint64)]
 
 
 class DistfindBatchStats(ctypes.Structure):

@@ -34,6 +34,34 @@ def gf2_rank(M: np.ndarray) -> int:
     return rank
 
 
+def gf2_kernel(H: np.ndarray) -> np.ndarray:
+    """A basis (rows) of {x : H x = 0} over GF(2): reduced row echelon form of H, then one
+    basis vector per free column (the free column set to 1, pivots solved)."""
+    A = (np.asarray(H, np.uint8) & 1).copy()
+    m, n = A.shape
+    piv = []
+    r = 0
+    for c in range(n):
+        if r == m:
+            break
+        nz = np.flatnonzero(A[r:, c])
+        if nz.size == 0:
+            continue
+        k = r + int(nz[0])
+        A[[r, k]] = A[[k, r]]
+        hit = np.flatnonzero(A[:, c])
+        hit = hit[hit != r]
+        A[hit] ^= A[r]
+        piv.append(c)
+        r += 1
+    free = np.setdiff1d(np.arange(n), piv)
+    B = np.zeros((free.size, n), np.uint8)
+    B[np.arange(free.size), free] = 1
+    if piv:
+        B[:, piv] = A[: len(piv)][:, free].T
+    return B
+
+
 # ---------------------------------------------------------------- classical families
 
 

@@ -214,6 +214,11 @@ int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t w
             g_err = "batch item kind must be 0 (classical) or 1 (CSS)";
             return RC_METHOD;
         }
+        if (it.scores && it.candidates > 0 && !bp.pair) {
+            bp.proposal.scores.assign(it.scores, it.scores + it.ac);
+            bp.proposal.candidates = std::min(it.candidates, 1024);
+            bp.proposal.seed = it.proposal_seed;
+        }
         if (it.order && !bp.pair) {
             const int n = it.ac;
             bp.order.assign(it.order, it.order + n);

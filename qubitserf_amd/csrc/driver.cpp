@@ -4,6 +4,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <cmath>
 #include <cstdio>
 #include <memory>
 #include <unordered_map>
@@ -212,26 +213,115 @@ struct Side {
     }
 };
 
-// Information sets (greedy peel over the proposal's order when given, then the matroid
+// Codewords BZ enumerates before its lower bound reaches `upper` (or level K), for a
+// family of information-set ranks: the sum over levels d of C(K, d) x (sets active at d),
+// the bound after level d being sum_g max(0, d + 1 - (K - r_g)), rounded up to even for an
+// even code (bz.cpp:142-162). A proposal's candidates are compared by it.
+long double predicted_work(int K, const std::vector<int>& ranks, bool even, int upper) {
+    long double work = 0, c = 1;
+    for (int d = 1; d <= K; ++d) {
+        c = c * (K - d + 1) / d;
+        int act = 0, outer = 0;
+        for (int r : ranks) {
+            act += (d + 1) - (K - r) > 0;
+            outer += std::max(0, (d + 1) - (K - r));
+        }
+        if (even) outer += outer & 1;
+        work += c * act;
+        if (upper <= outer) break;
+    }
+    return work;
+}
+
+// Candidate column orders from a proposal's column scores: descending score + Gumbel(0, 1)
+// noise, i.e. samples of the Plackett–Luce distribution the scores define. Candidate t of
+// a problem draws its noise from splitmix64 seeded with seed ^ (0x9e3779b97f4a7c15 (t+1)).
+std::vector<std::vector<int>> proposal_orders(const std::vector<float>& scores, int count, u64 seed) {
+    const int n = (int)scores.size();
+    std::vector<std::vector<int>> out;
+    for (int t = 0; t < count; ++t) {
+        u64 s = seed ^ (0x9e3779b97f4a7c15ull * (u64)(t + 1));
+        std::vector<double> key(n);
+        for (int j = 0; j < n; ++j) {
+            s += 0x9e3779b97f4a7c15ull;
+            u64 z = s;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            z ^= z >> 31;
+            const double u = ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);   // (0, 1)
+            key[j] = scores[j] - std::log(-std::log(u));
+        }
+        std::vector<int> o(n);
+        for (int j = 0; j < n; ++j) o[j] = j;
+        std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return key[a] > key[b]; });
+        out.push_back(std::move(o));
+    }
+    return out;
+}
+
+// Information sets (greedy peel over the given or proposed order, then the matroid
 // partition's sets when n <= 200 and they include more full-rank sets, css.cpp:336-366)
-// and the random information-set upper bound of every side, in three device calls.
-void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<int>*>& orders) {
+// and the random information-set upper bound of every side, in batched device calls.
+// A side with a proposal gets its candidates' peels too; the one with the least predicted
+// work is kept, the natural order winning ties.
+void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<int>*>& orders,
+                 const std::vector<const Proposal*>& proposals = {}) {
     std::vector<Side*> live;
-    std::vector<SetRequest> greedy(sides.size());
+    std::vector<qsf_elim_job> jobs;
+    for (Side* S : sides) {
+        if (!S->has) continue;
+        qsf_elim_job q{};
+        q.gen = S->prob->gen.data.data();
+        q.K = S->K;
+        q.n = S->n;
+        q.check = S->check.data();
+        q.kcheck = S->kcheck;
+        q.every_nonzero = S->prob->every_nonzero ? 1 : 0;
+        q.init = S->inner;
+        jobs.push_back(q);
+        live.push_back(S);
+    }
+    if (live.empty()) return;
+    const BzOptions& o = *live[0]->opt;
+    const int rc = qsf_hip_eliminate(jobs.data(), (int)jobs.size(), o.is_trials, o.seed, QSF_ELIM_BOUND);
+    if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+    for (size_t k = 0; k < live.size(); ++k) live[k]->inner = jobs[k].best;
+
+    // peels: the given order (or 0..n-1), plus every proposal candidate
+    const size_t ns = sides.size();
+    std::vector<SetRequest> greedy(ns);
+    std::vector<std::vector<std::vector<int>>> cand(ns);
+    std::vector<std::vector<SetRequest>> cand_req(ns);
     std::vector<SetRequest*> reqs;
-    for (size_t i = 0; i < sides.size(); ++i) {
+    for (size_t i = 0; i < ns; ++i) {
         if (!sides[i]->has) continue;
         greedy[i].gen = &sides[i]->prob->gen;
         greedy[i].order = orders.empty() ? nullptr : orders[i];
         reqs.push_back(&greedy[i]);
-        live.push_back(sides[i]);
+        const Proposal* pr = proposals.empty() ? nullptr : proposals[i];
+        if (!pr || pr->candidates <= 0 || (int)pr->scores.size() != sides[i]->n) continue;
+        cand[i] = proposal_orders(pr->scores, pr->candidates, pr->seed);
+        cand_req[i].resize(cand[i].size());
+        for (size_t t = 0; t < cand[i].size(); ++t) {
+            cand_req[i][t].gen = &sides[i]->prob->gen;
+            cand_req[i][t].order = &cand[i][t];
+            reqs.push_back(&cand_req[i][t]);
+        }
     }
-    if (live.empty()) return;
     device_sets(reqs);
-    std::vector<std::vector<std::vector<int>>> parts(sides.size());
-    std::vector<SetRequest> part_req(sides.size());
+    for (size_t i = 0; i < ns; ++i) {
+        Side& S = *sides[i];
+        if (!S.has || cand_req[i].empty()) continue;
+        long double best = predicted_work(S.K, greedy[i].out.ranks, S.even, S.inner);
+        for (SetRequest& r : cand_req[i]) {
+            const long double w = predicted_work(S.K, r.out.ranks, S.even, S.inner);
+            if (w < best) { best = w; greedy[i].out = std::move(r.out); }
+        }
+    }
+    std::vector<std::vector<std::vector<int>>> parts(ns);
+    std::vector<SetRequest> part_req(ns);
     std::vector<SetRequest*> preqs;
-    for (size_t i = 0; i < sides.size(); ++i) {
+    for (size_t i = 0; i < ns; ++i) {
         Side& S = *sides[i];
         if (!S.has) continue;
         const BitMat& gen = S.prob->gen;
@@ -243,8 +333,7 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
         preqs.push_back(&part_req[i]);
     }
     if (!preqs.empty()) device_sets(preqs);
-    std::vector<qsf_elim_job> jobs;
-    for (size_t i = 0; i < sides.size(); ++i) {
+    for (size_t i = 0; i < ns; ++i) {
         Side& S = *sides[i];
         if (!S.has) continue;
         Family& f = (part_req[i].gen && part_req[i].out.full(S.K) > greedy[i].out.full(S.K)) ? part_req[i].out
@@ -252,20 +341,7 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
         S.ng = f.ng;
         S.gamma = std::move(f.gamma);
         S.ranks = std::move(f.ranks);
-        qsf_elim_job q{};
-        q.gen = S.prob->gen.data.data();
-        q.K = S.K;
-        q.n = S.n;
-        q.check = S.check.data();
-        q.kcheck = S.kcheck;
-        q.every_nonzero = S.prob->every_nonzero ? 1 : 0;
-        q.init = S.inner;
-        jobs.push_back(q);
     }
-    const BzOptions& o = *live[0]->opt;
-    const int rc = qsf_hip_eliminate(jobs.data(), (int)jobs.size(), o.is_trials, o.seed, QSF_ELIM_BOUND);
-    if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
-    for (size_t k = 0; k < live.size(); ++k) live[k]->inner = jobs[k].best;
 }
 
 }  // namespace
@@ -432,6 +508,7 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
     std::vector<std::unique_ptr<Stepper>> st(np);
     std::vector<Side*> sides;
     std::vector<const std::vector<int>*> orders;
+    std::vector<const Proposal*> props;
     for (size_t i = 0; i < np; ++i) {
         st[i] = std::make_unique<Stepper>();
         Stepper& S = *st[i];
@@ -440,13 +517,15 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
         S.A.prepare(bp.a, opt, bp.pair ? "Z" : "");
         sides.push_back(&S.A);
         orders.push_back(!bp.pair && !bp.order.empty() ? &bp.order : nullptr);
+        props.push_back(!bp.pair && bp.proposal.candidates > 0 ? &bp.proposal : nullptr);
         if (bp.pair) {
             S.B.prepare(bp.b, opt, "X");
             sides.push_back(&S.B);
             orders.push_back(nullptr);
+            props.push_back(nullptr);
         }
     }
-    setup_sides(sides, orders);
+    setup_sides(sides, orders, props);
     bs.setup_s = std::chrono::duration<double>(clk::now() - t0).count();
     for (auto& S : st) {
         S->init();

@@ -50,10 +50,21 @@ BzResult bz_min_interleaved(const Problem& pz, const Problem& px, const BzOption
 
 // One problem of a batch: a single side (classical, or one CSS side), or the two CSS
 // sides of a min-distance solve (pair), with an optional proposal order for a single side.
+// A learned proposal for one single-side problem: a score per column; `candidates` column
+// orders are sampled from the Plackett–Luce distribution of the scores (seeded), and the
+// peel of the order with the least predicted BZ work replaces the natural one when it is
+// strictly better.
+struct Proposal {
+    std::vector<float> scores;
+    int candidates = 0;
+    unsigned long long seed = 0;
+};
+
 struct BatchProblem {
     Problem a, b;
     bool pair = false;
     std::vector<int> order;
+    Proposal proposal;
 };
 
 struct BatchStats {

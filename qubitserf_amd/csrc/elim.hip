@@ -126,11 +126,15 @@ __global__ __launch_bounds__(64) void info_sets(const ElimJob* __restrict__ jobs
     int* live = reinterpret_cast<int*>(nzc + st);      // n ints
 
     if (J.nparts > 0) {   // sets of a given partition, each from the original generator
+        if (J.set_of_col)
+            for (int i = lane; i < n; i += 64) J.set_of_col[i] = -1;
         for (int p = 0; p < J.nparts; ++p) {
             for (int i = lane; i < K * st; i += 64) W[i] = J.gen[i];
             __syncthreads();
             const int o0 = J.part_off[p];
-            const int rank = eliminate(W, K, st, J.part_cols + o0, J.part_off[p + 1] - o0, [](int) {});
+            const int rank = eliminate(W, K, st, J.part_cols + o0, J.part_off[p + 1] - o0, [&](int c) {
+                if (J.set_of_col) J.set_of_col[c] = p;
+            });
             if (p < J.set_cap) {
                 u64* out = J.sets + (size_t)p * K * st;
                 for (int i = lane; i < K * st; i += 64) out[i] = W[i];
@@ -149,11 +153,16 @@ __global__ __launch_bounds__(64) void info_sets(const ElimJob* __restrict__ jobs
         used[w] = 0;
     }
     for (int i = lane; i < n; i += 64) live[i] = J.order ? J.order[i] : i;
+    if (J.set_of_col)
+        for (int i = lane; i < n; i += 64) J.set_of_col[i] = -1;
     __syncthreads();
 
     int nlive = n, nsets = 0;
     for (;;) {
-        const int rank = eliminate(W, K, st, live, nlive, [&](int c) { used[c >> 6] |= 1ull << (c & 63); });
+        const int rank = eliminate(W, K, st, live, nlive, [&](int c) {
+            used[c >> 6] |= 1ull << (c & 63);
+            if (J.set_of_col) J.set_of_col[c] = nsets;
+        });
         if (nsets < J.set_cap) {
             u64* out = J.sets + (size_t)nsets * K * st;
             for (int i = lane; i < K * st; i += 64) out[i] = W[i];

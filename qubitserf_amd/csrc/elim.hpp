@@ -34,6 +34,7 @@ struct ElimJob {
     const int* part_off;
     int nparts;
     const int* order;    // greedy peel: initial column order (a permutation of [0, n)); null = 0..n-1
+    int* set_of_col;     // out (n ints, may be null): the set whose pivot a column became, else -1
 };
 
 // Random information-set trials: trials x jobs workgroups of one wave. Trial t of a job

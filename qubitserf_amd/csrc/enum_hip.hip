@@ -675,7 +675,7 @@ struct Engine {
             nu += (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st + (want_sets ? (size_t)q.set_cap * q.K * st : 0);
             i_off[j] = ni;
             ni += 2 + (want_sets ? (size_t)q.set_cap : 0) + (q.nparts > 0 ? (size_t)q.nparts + 1 + q.part_off[q.nparts] : 0) +
-                  (q.order ? (size_t)q.n : 0);
+                  (q.order ? (size_t)q.n : 0) + (want_sets && q.set_of_col ? (size_t)q.n : 0);
             scratch_sets += elim_scratch_words(q.K, st, 1);
             scratch_trial += elim_scratch_words(q.K, st, 1);
             max_n = std::max(max_n, q.n);
@@ -726,7 +726,9 @@ struct Engine {
             e.nparts = q.nparts > 0 ? q.nparts : 0;
             e.part_off = e.nparts ? ib + 2 + e.set_cap : nullptr;
             e.part_cols = e.nparts ? ib + 2 + e.set_cap + e.nparts + 1 : nullptr;
-            e.order = q.order ? ib + 2 + e.set_cap + (e.nparts ? e.nparts + 1 + q.part_off[q.nparts] : 0) : nullptr;
+            int* tail = ib + 2 + e.set_cap + (e.nparts ? e.nparts + 1 + q.part_off[q.nparts] : 0);
+            e.order = q.order ? tail : nullptr;
+            e.set_of_col = (want_sets && q.set_of_col) ? tail + (q.order ? q.n : 0) : nullptr;
             e.scratch = elim_scratch_words(q.K, st, 1) ? el_scratch.p + sc : nullptr;
             sc += elim_scratch_words(q.K, st, 1) * (size_t)std::max(1, batch);
             lds_sets = std::max(lds_sets, info_sets_lds_bytes(e));
@@ -769,6 +771,10 @@ struct Engine {
             q.best = want_bound ? std::min(q.init, h[0]) : q.init;
             if (!want_sets) continue;
             q.nsets = h[1];
+            if (q.set_of_col) {
+                const int* tail = h + 2 + q.set_cap + (q.nparts > 0 ? q.nparts + 1 + q.part_off[q.nparts] : 0);
+                std::memcpy(q.set_of_col, tail + (q.order ? q.n : 0), (size_t)q.n * 4);
+            }
             const int keep = std::min(q.nsets, q.set_cap);
             std::memcpy(q.ranks, h + 2, (size_t)keep * 4);
             std::memcpy(q.sets, hu.data() + u_off[j] + (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st,

@@ -156,8 +156,10 @@ def last_trace():
 def batch_distance(problems, *, work_cap=0, verbose=False):
     """Distances of many codes in one batched solve (``distfind_batch_distance``).
 
-    Each problem is ``("classical", H)``, ``("classical", H, order)`` or
-    ``("css", Hx, Hz, which)``; ``order`` is a column order for the information-set peel.
+    Each problem is ``("classical", H)``, ``("classical", H, order)``,
+    ``("classical", H, None, (scores, candidates, seed))`` or ``("css", Hx, Hz, which)``;
+    ``order`` is a column order for the information-set peel, ``scores`` a proposal's
+    column scores from which ``candidates`` orders are sampled.
     Returns a list of ``DistfindResult`` (distance, lower_bound, proven, levels), identical
     to the single-code entry points' results; with ``work_cap`` > 0 a problem whose next
     level would exceed that many codewords stops with ``proven == 0``."""
@@ -175,6 +177,14 @@ def batch_distance(problems, *, work_cap=0, verbose=False):
                 o = np.ascontiguousarray(p[2], np.int32)
                 it.order = o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
                 keep.append(o)
+            if len(p) > 3 and p[3] is not None:
+                sc, cands, seed = p[3]
+                sv = np.ascontiguousarray(sc, np.float32)
+                if sv.shape != (H.shape[1],):
+                    raise ValueError("scores must hold one value per column")
+                it.scores = sv.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+                it.candidates, it.proposal_seed = int(cands), int(seed)
+                keep.append(sv)
         elif p[0] == "css":
             Hx, Hz = _as_css(p[1], p[2])
             it.kind, it.A, it.ar, it.ac = 1, _ptr(Hx), Hx.shape[0], Hx.shape[1]

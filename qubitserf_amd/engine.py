@@ -119,6 +119,9 @@ def eliminate(problems, *, what=ELIM_SETS | ELIM_BOUND, trials=256, seed=0x9E377
             o = np.ascontiguousarray(order, np.int32)
             q.order = o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
             keep.append(o)
+        soc = np.full(n, -2, np.int32)
+        q.set_of_col = soc.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        keep.append(soc)
         cap = len(parts) if parts else int(p.get("set_cap", n + 2))
         sets = np.zeros((max(cap, 1), max(K, 1), stride), np.uint64)
         ranks = np.zeros(max(cap, 1), np.int32)
@@ -127,18 +130,18 @@ def eliminate(problems, *, what=ELIM_SETS | ELIM_BOUND, trials=256, seed=0x9E377
         q.ranks = ranks.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         q.init = int(p.get("init", WEIGHT_NONE))
         keep += [g, sets, ranks]
-        p["_out"] = (sets, ranks)
+        p["_out"] = (sets, ranks, soc)
     rc = L.qsf_hip_eliminate(jobs, len(problems), int(trials), int(seed), int(what))
     if rc:
         _err(rc, "qsf_hip_eliminate")
     out = []
     for j, p in enumerate(problems):
-        sets, ranks = p.pop("_out")
+        sets, ranks, soc = p.pop("_out")
         q = jobs[j]
         r = {"best": q.best}
         if what & ELIM_SETS:
             ns = min(q.nsets, q.set_cap)
-            r.update(nsets=q.nsets, sets=sets[:ns, : q.K].copy(), ranks=ranks[:ns].tolist())
+            r.update(nsets=q.nsets, sets=sets[:ns, : q.K].copy(), ranks=ranks[:ns].tolist(), set_of_col=soc.copy())
         out.append(r)
     del keep
     return out
