@@ -17,6 +17,11 @@ Prints ONE JSON line. Fields beyond the common contract:
                       test), which the two-stage kernel undercuts
   cpu_baseline        the reference's own CPU enumerator (oracle/_ref) on one level of the
                       same workload, on this host's cores
+  batched_learned     plain (one code per call, natural information sets) vs batched +
+                      learned (one call, trained scorer's proposals) on 256 random codes
+                      (n 64..160, rate 0.3..0.6) and 16 quasi-cyclic codes (L 20..40), under
+                      a per-code work cap; the elimination kernel's HBM rate; the serving
+                      loop's p50/p99 latency and throughput at 70% device load
 """
 from __future__ import annotations
 
@@ -71,12 +76,72 @@ def cpu_baseline(threads: int):
     }
 
 
+def _certified(rs):
+    return sum(1 for r in rs if r.proven)
+
+
+def batched_learned(log):
+    """Plain vs batched+learned time-to-certify on the batched (256 random codes) and learned
+    (quasi-cyclic evaluation codes) workloads, the elimination phase's HBM rate, and the
+    serving loop. Plain = one code per call with the natural information sets; batched +
+    learned = one call for the whole set with the trained scorer's proposals (scorer
+    inference included). Every code carries the same work cap in both forms; a code over
+    the cap is reported unproven (the full sets need ~1e19 codewords)."""
+    import numpy as np
+    from qubitserf_amd import codes, engine, serving
+    from qubitserf_amd import distfind as df
+    from qubitserf_amd.proposal import MPScorer, choose_orders
+
+    wfile = ROOT / "qubitserf_amd" / "proposal" / "weights.npz"
+    scorer = MPScorer.load(wfile) if wfile.exists() else None
+    out = {"scorer": str(wfile.relative_to(ROOT)) if scorer else None}
+
+    def compare(name, Hs, cap, cands):
+        t0 = time.perf_counter()
+        plain = [df.batch_distance([("classical", H)], work_cap=cap)[0] for H in Hs]
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        learned = df.batch_distance(choose_orders(Hs, scorer, candidates=cands), work_cap=cap)
+        t_bl = time.perf_counter() - t0
+        st = df.last_batch_stats()
+        t0 = time.perf_counter()
+        rand = df.batch_distance(choose_orders(Hs, None, candidates=cands), work_cap=cap)
+        t_rand = time.perf_counter() - t0
+        agree = all(a.distance == b.distance for a, b in zip(plain, learned) if a.proven and b.proven)
+        r = {"codes": len(Hs), "work_cap": cap, "candidates": cands,
+             "plain_sequential_s": t_plain, "batched_learned_s": t_bl, "batched_random_s": t_rand,
+             "speedup": t_plain / t_bl, "certified_plain": _certified(plain), "certified_learned": _certified(learned),
+             "certified_random": _certified(rand), "distances_agree": agree,
+             "levels_plain": sum(x.levels for x in plain), "levels_learned": sum(x.levels for x in learned),
+             "batched_rounds": st["rounds"], "batched_codewords": st["codewords"],
+             "batched_device_ms": st["device_ms"]}
+        log(f"{name}: {json.dumps(r)}")
+        return r
+
+    c3 = codes.config3_batch(0, 256)
+    out["batched_c3"] = compare("batched", c3, int(1e10), 8)
+    out["learned_c4"] = compare("learned", codes.config4_eval(0, 16, 20, 40), int(1e11), 16)
+    # elimination phase: information sets of all 256 codes in one launch
+    probs = [dict(gen=engine.pack_rows(codes.gf2_kernel(H)), n=H.shape[1], set_cap=8) for H in c3]
+    engine.eliminate(probs, what=engine.ELIM_SETS)
+    es = engine.last_elim_stats()
+    gbps = es["sets_bytes"] / (es["sets_ms"] * 1e-3) / 1e9 if es["sets_ms"] > 0 else 0.0
+    out["elimination"] = {"jobs": es["jobs"], "bytes": es["sets_bytes"], "kernel_ms": es["sets_ms"],
+                          "GBps": gbps, "peak_GBps": 8000.0, "frac": gbps / 8000.0, "bound": "hbm"}
+    reqs = serving.make_requests(96, seed=0)
+    rate, stats = serving.calibrate_rate(reqs, 0.7, rounds=3)
+    out["serving"] = stats
+    log(f"serving: {json.dumps(stats)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched / learned / serving figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -150,6 +215,8 @@ def main():
             "five_op_model_TOPs": five_op,
         },
     }
+    if not args.no_extras:
+        line["batched_learned"] = batched_learned(lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(min(threads, 16))

@@ -123,6 +123,7 @@ typedef struct {
                               score + Gumbel noise); the peel with the least predicted BZ work
                               replaces the natural one when strictly better */
     uint64_t proposal_seed;
+    int64_t work_cap;      /* > 0: this problem's own work cap (else the call's work_cap) */
 } DistfindBatchItem;
 
 typedef struct {
@@ -135,9 +136,10 @@ typedef struct {
 } DistfindBatchStats;
 
 /* work_cap > 0: a problem stops before a level would take its enumerated codewords past
- * work_cap; it then reports its bounds so far with proven = 0. Return codes as
- * distfind_css_distance; out has `count` entries. */
-int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int verbose,
+ * work_cap; it then reports its bounds so far with proven = 0. is_trials: random
+ * information-set trials for each problem's upper bound (<= 0: 256, the reference's count,
+ * bz.cpp:128). Return codes as distfind_css_distance; out has `count` entries. */
+int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int is_trials, int verbose,
                             DistfindResult* out);
 void distfind_last_batch_stats(DistfindBatchStats* out);
 

@@ -96,7 +96,8 @@ void qsf_hip_free(void* p);
  *   `order` when given (a proposal's column order). With a partition (nparts > 0; part p is
  *   part_cols[part_off[p] .. part_off[p+1])): set p is gen made systematic over part p
  *   (css.cpp:358-361). Outputs: nsets, and the first min(nsets, set_cap) generators
- *   (set_cap x K x stride u64, in `sets`) and ranks (in `ranks`), caller-owned.
+ *   (set_cap x K x stride u64, in `sets`; sets may be NULL for ranks only) and ranks (in
+ *   `ranks`), caller-owned.
  * QSF_ELIM_BOUND: the random information-set upper bound (replaces random_is_seed,
  *   cpp/distfind/bz.cpp:24-70, run before level 1 at bz.cpp:128 and bz.cpp:252): best =
  *   min(init, lightest nonzero logical row over `trials` systematic forms). Trial t orders

@@ -109,7 +109,7 @@ class DistfindBatchItem(ctypes.Structure):
                 ("ac", ctypes.c_int), ("B", ctypes.POINTER(ctypes.c_uint8)), ("br", ctypes.c_int),
                 ("bc", ctypes.c_int), ("which", ctypes.c_char), ("order", ctypes.POINTER(ctypes.c_int)),
                 ("scores", ctypes.POINTER(ctypes.c_float)), ("candidates", ctypes.c_int),
-                ("proposal_seed", ctypes.c_uint64)]
+                ("proposal_seed", ctypes.c_uint64), ("work_cap", ctypes.c_int64)]
 
 
 class DistfindBatchStats(ctypes.Structure):
@@ -164,7 +164,7 @@ def load() -> ctypes.CDLL:
     sig("distfind_last_error", c_char_p, [])
     sig("distfind_last_trace", c_int, [ctypes.POINTER(DistfindLevel), c_int])
     sig("distfind_batch_distance", c_int,
-        [ctypes.POINTER(DistfindBatchItem), c_int, ctypes.c_int64, c_int, rp])
+        [ctypes.POINTER(DistfindBatchItem), c_int, ctypes.c_int64, c_int, c_int, rp])
     sig("distfind_last_batch_stats", None, [ctypes.POINTER(DistfindBatchStats)])
 
     vp = ctypes.c_void_p

@@ -2,6 +2,7 @@
 // include/codeaut_capi.h).
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -188,10 +189,11 @@ namespace {
 thread_local BatchStats g_batch;
 }
 
-int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int verbose,
+int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int is_trials, int verbose,
                             DistfindResult* out) {
     if (count < 0 || (count > 0 && (!items || !out))) { g_err = "null batch"; return RC_NULL; }
     if (int rc = check_args("bz", "gpu")) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<BatchProblem> probs(count);
     for (int i = 0; i < count; ++i) {
         const DistfindBatchItem& it = items[i];
@@ -214,6 +216,7 @@ int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t w
             g_err = "batch item kind must be 0 (classical) or 1 (CSS)";
             return RC_METHOD;
         }
+        bp.work_cap = it.work_cap > 0 ? (u64)it.work_cap : 0;
         if (it.scores && it.candidates > 0 && !bp.pair) {
             bp.proposal.scores.assign(it.scores, it.scores + it.ac);
             bp.proposal.candidates = std::min(it.candidates, 1024);
@@ -231,6 +234,10 @@ int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t w
     }
     BzOptions o = options(verbose);
     o.trace = nullptr;
+    if (is_trials > 0) o.is_trials = is_trials;
+    if (o.profile)
+        std::fprintf(stderr, "[prof batch] %d problems built in %.3f ms\n", count,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     return guarded([&] {
         std::vector<BzResult> r = bz_batch(probs, o, work_cap > 0 ? (u64)work_cap : 0, &g_batch);
         for (int i = 0; i < count; ++i) fill(out + i, r[i]);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <thread>
 #include <unordered_map>
 
 #include "../../include/qsf_hip.h"
@@ -52,6 +53,7 @@ struct SetRequest {
     const BitMat* gen;
     const std::vector<int>* order = nullptr;
     const std::vector<std::vector<int>>* parts = nullptr;
+    bool ranks_only = false;   // out.gamma stays empty
     Family out;
 };
 
@@ -88,9 +90,9 @@ void device_sets(std::vector<SetRequest*>& reqs) {
                 q.nparts = (int)r.parts->size();
             }
             const int c = cap[todo[k]];
-            sets[k].assign((size_t)c * g.rows * g.ld + 1, 0);
+            if (!r.ranks_only) sets[k].assign((size_t)c * g.rows * g.ld + 1, 0);
             ranks[k].assign(c, 0);
-            q.sets = sets[k].data();
+            q.sets = r.ranks_only ? nullptr : sets[k].data();
             q.set_cap = c;
             q.ranks = ranks[k].data();
         }
@@ -111,9 +113,10 @@ void device_sets(std::vector<SetRequest*>& reqs) {
             std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ranks[k][a] > ranks[k][b]; });
             r.out = Family{};
             r.out.ng = q.nsets;
-            r.out.gamma.resize((size_t)q.nsets * w);
+            if (!r.ranks_only) r.out.gamma.resize((size_t)q.nsets * w);
             for (int t = 0; t < q.nsets; ++t) {
-                std::memcpy(r.out.gamma.data() + (size_t)t * w, sets[k].data() + (size_t)idx[t] * w, w * 8);
+                if (!r.ranks_only)
+                    std::memcpy(r.out.gamma.data() + (size_t)t * w, sets[k].data() + (size_t)idx[t] * w, w * 8);
                 r.out.ranks.push_back(ranks[k][idx[t]]);
             }
         }
@@ -283,9 +286,12 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
     }
     if (live.empty()) return;
     const BzOptions& o = *live[0]->opt;
+    auto ta = clk::now();
     const int rc = qsf_hip_eliminate(jobs.data(), (int)jobs.size(), o.is_trials, o.seed, QSF_ELIM_BOUND);
     if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
     for (size_t k = 0; k < live.size(); ++k) live[k]->inner = jobs[k].best;
+    auto tb = clk::now();
+    const double t_bound = std::chrono::duration<double, std::milli>(tb - ta).count();
 
     // peels: the given order (or 0..n-1), plus every proposal candidate
     const size_t ns = sides.size();
@@ -305,34 +311,70 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
         for (size_t t = 0; t < cand[i].size(); ++t) {
             cand_req[i][t].gen = &sides[i]->prob->gen;
             cand_req[i][t].order = &cand[i][t];
+            cand_req[i][t].ranks_only = true;
             reqs.push_back(&cand_req[i][t]);
         }
     }
     device_sets(reqs);
+    auto tm = clk::now();
+    // candidates were peeled for their ranks only; the winner's generators are peeled again
+    std::vector<SetRequest*> winners;
     for (size_t i = 0; i < ns; ++i) {
         Side& S = *sides[i];
         if (!S.has || cand_req[i].empty()) continue;
         long double best = predicted_work(S.K, greedy[i].out.ranks, S.even, S.inner);
-        for (SetRequest& r : cand_req[i]) {
-            const long double w = predicted_work(S.K, r.out.ranks, S.even, S.inner);
-            if (w < best) { best = w; greedy[i].out = std::move(r.out); }
+        int win = -1;
+        for (size_t t = 0; t < cand_req[i].size(); ++t) {
+            const long double w = predicted_work(S.K, cand_req[i][t].out.ranks, S.even, S.inner);
+            if (w < best) { best = w; win = (int)t; }
+        }
+        if (win >= 0) {
+            greedy[i].order = &cand[i][win];
+            winners.push_back(&greedy[i]);
         }
     }
+    if (!winners.empty()) device_sets(winners);
+    const double t_sets = std::chrono::duration<double, std::milli>(clk::now() - tb).count();
+    // matroid partitions (host: an augmenting-path search) of the sides whose peel is short
+    // of full-rank sets, several sides at a time
     std::vector<std::vector<std::vector<int>>> parts(ns);
     std::vector<SetRequest> part_req(ns);
     std::vector<SetRequest*> preqs;
+    std::vector<size_t> want;
     for (size_t i = 0; i < ns; ++i) {
         Side& S = *sides[i];
         if (!S.has) continue;
         const BitMat& gen = S.prob->gen;
         const int gfull = greedy[i].out.full(S.K);
         if (gfull >= nonzero_columns(gen) / S.K || gen.cols > 200) continue;
-        if (!matroid_parts(gen, parts[i]) || parts[i].empty()) continue;
-        part_req[i].gen = &gen;
+        want.push_back(i);
+    }
+    std::vector<char> ok(ns, 0);
+    {
+        const size_t workers = std::min<size_t>(want.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (size_t k; (k = next++) < want.size();) {
+                const size_t i = want[k];
+                ok[i] = matroid_parts(sides[i]->prob->gen, parts[i]) && !parts[i].empty();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < workers; ++w) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    }
+    for (size_t i : want) {
+        if (!ok[i]) continue;
+        part_req[i].gen = &sides[i]->prob->gen;
         part_req[i].parts = &parts[i];
         preqs.push_back(&part_req[i]);
     }
+    const double t_matroid = std::chrono::duration<double, std::milli>(clk::now() - tm).count();
     if (!preqs.empty()) device_sets(preqs);
+    if (o.profile)
+        std::fprintf(stderr, "[prof setup] sides=%zu bound %.3f ms, peels %.3f ms, matroid %.3f ms (%zu partitions)\n",
+                     live.size(), t_bound, t_sets, t_matroid, preqs.size());
     for (size_t i = 0; i < ns; ++i) {
         Side& S = *sides[i];
         if (!S.has) continue;
@@ -423,6 +465,7 @@ struct Stepper {
     bool pair = false;
     int d = 1, turn = 0, best = kNoWeight, maxw = 0;
     bool done = false, capped = false;
+    u64 cap = 0;           // this problem's own work cap (0: the batch's)
     Side* cur = nullptr;
 
     void init() {
@@ -514,6 +557,7 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
         Stepper& S = *st[i];
         BatchProblem& bp = probs[i];
         S.pair = bp.pair;
+        S.cap = bp.work_cap;
         S.A.prepare(bp.a, opt, bp.pair ? "Z" : "");
         sides.push_back(&S.A);
         orders.push_back(!bp.pair && !bp.order.empty() ? &bp.order : nullptr);
@@ -547,7 +591,8 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
                 Side* side = S.cur;
                 const u64 w = side->level_work(S.d);
                 const u64 spent = S.A.work + (S.pair ? S.B.work : 0);
-                if (work_cap && (w > work_cap || spent > work_cap - w)) { S.capped = S.done = true; break; }
+                const u64 cap = S.cap ? S.cap : work_cap;
+                if (cap && (w > cap || spent > cap - w)) { S.capped = S.done = true; break; }
                 qsf_enum_plan pl;
                 if (side->plan(S.d, pl)) {
                     plans.push_back(pl);
@@ -569,6 +614,9 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
         qsf_hip_stats round{};
         qsf_hip_last_stats(&round);
         const double ms = std::chrono::duration<double, std::milli>(clk::now() - tr).count();
+        if (opt.profile)
+            std::fprintf(stderr, "[prof round] %d: %zu levels, %llu codewords, call %.3f ms, device %.3f ms\n", bs.rounds,
+                         plans.size(), (unsigned long long)round.codewords, ms, round.kernel_ms);
         ++bs.rounds;
         bs.codewords += round.codewords;
         bs.device_ms += round.kernel_ms;

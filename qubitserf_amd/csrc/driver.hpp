@@ -65,6 +65,7 @@ struct BatchProblem {
     bool pair = false;
     std::vector<int> order;
     Proposal proposal;
+    unsigned long long work_cap = 0;   // > 0: overrides the batch's cap for this problem
 };
 
 struct BatchStats {

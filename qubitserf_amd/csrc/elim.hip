@@ -136,8 +136,10 @@ __global__ __launch_bounds__(64) void info_sets(const ElimJob* __restrict__ jobs
                 if (J.set_of_col) J.set_of_col[c] = p;
             });
             if (p < J.set_cap) {
-                u64* out = J.sets + (size_t)p * K * st;
-                for (int i = lane; i < K * st; i += 64) out[i] = W[i];
+                if (J.sets) {
+                    u64* out = J.sets + (size_t)p * K * st;
+                    for (int i = lane; i < K * st; i += 64) out[i] = W[i];
+                }
                 if (lane == 0) J.ranks[p] = rank;
             }
             __syncthreads();
@@ -164,8 +166,10 @@ __global__ __launch_bounds__(64) void info_sets(const ElimJob* __restrict__ jobs
             if (J.set_of_col) J.set_of_col[c] = nsets;
         });
         if (nsets < J.set_cap) {
-            u64* out = J.sets + (size_t)nsets * K * st;
-            for (int i = lane; i < K * st; i += 64) out[i] = W[i];
+            if (J.sets) {
+                u64* out = J.sets + (size_t)nsets * K * st;
+                for (int i = lane; i < K * st; i += 64) out[i] = W[i];
+            }
             if (lane == 0) J.ranks[nsets] = rank;
         }
         ++nsets;

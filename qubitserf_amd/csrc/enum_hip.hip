@@ -12,6 +12,9 @@
 #include <list>
 #include <memory>
 #include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -487,8 +490,14 @@ struct Solve {
     std::vector<u32> htbl;        // host copy of the residual+syndrome table
     std::vector<u64> hflags;      // host copy of the flag masks
     std::vector<u64> binom;       // (K+1) x bk
-    DevBuf<u64> gamma, check, dbinom, flags;
-    DevBuf<u32> tbl;
+    // device copies, carved from one allocation: [gamma | check | binom | flags | tbl]
+    struct Ptr { u64* p = nullptr; };
+    struct Ptr32 { u32* p = nullptr; };
+    DevBuf<u64> arena;
+    std::vector<u64> stage;       // host image of the arena (source of its upload)
+    size_t n_gamma = 0, n_check = 0, n_binom = 0, n_flags = 0;
+    Ptr gamma, check, dbinom, flags;
+    Ptr32 tbl;
     u64 C(int nn, int kk) const {
         if (kk < 0 || nn < 0 || kk > nn) return 0;
         if (kk >= bk) return kBinInf;
@@ -571,6 +580,65 @@ struct Engine {
     }
 
     void build(Solve& s, const qsf_enum_plan& p) {
+        build_host(s, p);
+        upload(s);
+    }
+
+    // Solves for many plans: the missing ones' host tables are built on several threads,
+    // then allocated and uploaded one by one. cache_cap must exceed n.
+    std::vector<Solve*> solves_for(const qsf_enum_plan* plans, int n) {
+        std::vector<Solve*> out(n, nullptr);
+        std::vector<std::pair<Solve*, const qsf_enum_plan*>> fresh;
+        std::unordered_map<u64, Solve*> made;
+        for (int j = 0; j < n; ++j) {
+            const qsf_enum_plan& p = plans[j];
+            for (auto it = cache.begin(); it != cache.end(); ++it) {
+                Solve& c = **it;
+                if (p.buffers_key != 0 && c.key == p.buffers_key && c.n == p.n && c.K == p.K &&
+                    c.ngt == p.num_gamma_total && c.kcheck == p.kcheck && c.stride == p.stride) {
+                    cache.splice(cache.begin(), cache, it);
+                    out[j] = &c;
+                    break;
+                }
+            }
+            if (out[j]) continue;
+            if (p.buffers_key != 0) {
+                auto m = made.find(p.buffers_key);
+                if (m != made.end()) { out[j] = m->second; continue; }
+            }
+            cache.push_front(std::make_unique<Solve>());
+            out[j] = cache.front().get();
+            fresh.emplace_back(out[j], &p);
+            if (p.buffers_key != 0) made[p.buffers_key] = out[j];
+        }
+        const int workers = (int)std::min<size_t>(fresh.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+        if (workers > 1) {
+            std::atomic<size_t> next{0};
+            std::vector<std::thread> pool;
+            std::exception_ptr fail;
+            std::mutex fail_mu;
+            for (int w = 0; w < workers; ++w)
+                pool.emplace_back([&] {
+                    for (size_t i; (i = next++) < fresh.size();) {
+                        try {
+                            build_host(*fresh[i].first, *fresh[i].second);
+                        } catch (...) {
+                            std::lock_guard<std::mutex> lk(fail_mu);
+                            fail = std::current_exception();
+                        }
+                    }
+                });
+            for (auto& t : pool) t.join();
+            if (fail) std::rethrow_exception(fail);
+        } else {
+            for (auto& f : fresh) build_host(*f.first, *f.second);
+        }
+        for (auto& f : fresh) upload(*f.first);
+        while (cache.size() > cache_cap) cache.pop_back();
+        return out;
+    }
+
+    void build_host(Solve& s, const qsf_enum_plan& p) {
         s.key = p.buffers_key;
         s.n = p.n; s.stride = p.stride; s.K = p.K; s.ngt = p.num_gamma_total; s.kcheck = p.kcheck;
         const int K = p.K, st = p.stride, n = p.n;
@@ -636,20 +704,33 @@ struct Engine {
                 if (flagged[g][r] && r < kMaxTiledK) fl[2 * g + (r >> 6)] |= 1ull << (r & 63);
             }
         }
-        s.gamma.reserve((size_t)s.ngt * K * st);
-        HIPCHK(hipMemcpyAsync(s.gamma.p, p.gamma, (size_t)s.ngt * K * st * 8, hipMemcpyHostToDevice, stream));
-        s.check.reserve((size_t)std::max(1, p.kcheck) * st);
-        if (p.kcheck)
-            HIPCHK(hipMemcpyAsync(s.check.p, p.check, (size_t)p.kcheck * st * 8, hipMemcpyHostToDevice, stream));
-        s.dbinom.reserve(s.binom.size());
-        HIPCHK(hipMemcpyAsync(s.dbinom.p, s.binom.data(), s.binom.size() * 8, hipMemcpyHostToDevice, stream));
-        s.tbl.reserve(tbl.size());
-        HIPCHK(hipMemcpyAsync(s.tbl.p, tbl.data(), tbl.size() * 4, hipMemcpyHostToDevice, stream));
-        s.flags.reserve(fl.size());
-        HIPCHK(hipMemcpyAsync(s.flags.p, fl.data(), fl.size() * 8, hipMemcpyHostToDevice, stream));
-        HIPCHK(hipStreamSynchronize(stream));   // host vectors above go out of scope
+        // one staging vector, one allocation, one copy
+        const size_t n_gamma = (size_t)s.ngt * K * st, n_check = (size_t)std::max(1, p.kcheck) * st;
+        const size_t n_binom = s.binom.size(), n_flags = fl.size(), n_tbl = (tbl.size() + 1) / 2;
+        std::vector<u64>& stage = s.stage;   // kept with the solve: the async copy reads it
+        stage.assign(n_gamma + n_check + n_binom + n_flags + n_tbl, 0);
+        u64* at = stage.data();
+        std::memcpy(at, p.gamma, n_gamma * 8);
+        if (p.kcheck) std::memcpy(at + n_gamma, p.check, (size_t)p.kcheck * st * 8);
+        std::memcpy(at + n_gamma + n_check, s.binom.data(), n_binom * 8);
+        std::memcpy(at + n_gamma + n_check + n_binom, fl.data(), n_flags * 8);
+        if (!tbl.empty()) std::memcpy(at + n_gamma + n_check + n_binom + n_flags, tbl.data(), tbl.size() * 4);
+        s.n_gamma = n_gamma; s.n_check = n_check; s.n_binom = n_binom; s.n_flags = n_flags;
         s.htbl = std::move(tbl);
         s.hflags = std::move(fl);
+    }
+
+    void upload(Solve& s) {
+        const std::vector<u64>& stage = s.stage;
+        const size_t n_gamma = s.n_gamma, n_check = s.n_check, n_binom = s.n_binom, n_flags = s.n_flags;
+        s.arena.reserve(stage.size());
+        u64* d = s.arena.p;
+        s.gamma.p = d;
+        s.check.p = d + n_gamma;
+        s.dbinom.p = d + n_gamma + n_check;
+        s.flags.p = d + n_gamma + n_check + n_binom;
+        s.tbl.p = reinterpret_cast<u32*>(d + n_gamma + n_check + n_binom + n_flags);
+        HIPCHK(hipMemcpyAsync(d, stage.data(), stage.size() * 8, hipMemcpyHostToDevice, stream));
     }
 
     // ---- elimination: information sets and random-IS bounds, many problems per launch --
@@ -672,7 +753,7 @@ struct Engine {
             const qsf_elim_job& q = jobs[j];
             const int st = (q.n + 63) / 64;
             u_off[j] = nu;
-            nu += (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st + (want_sets ? (size_t)q.set_cap * q.K * st : 0);
+            nu += (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st + (want_sets && q.sets ? (size_t)q.set_cap * q.K * st : 0);
             i_off[j] = ni;
             ni += 2 + (want_sets ? (size_t)q.set_cap : 0) + (q.nparts > 0 ? (size_t)q.nparts + 1 + q.part_off[q.nparts] : 0) +
                   (q.order ? (size_t)q.n : 0) + (want_sets && q.set_of_col ? (size_t)q.n : 0);
@@ -721,7 +802,7 @@ struct Engine {
             e.best = ib;
             e.nsets = ib + 1;
             e.set_cap = want_sets ? q.set_cap : 0;
-            e.sets = want_sets ? base + (size_t)q.K * st + (size_t)e.kcheck * st : nullptr;
+            e.sets = want_sets && q.sets ? base + (size_t)q.K * st + (size_t)e.kcheck * st : nullptr;
             e.ranks = want_sets ? ib + 2 : nullptr;
             e.nparts = q.nparts > 0 ? q.nparts : 0;
             e.part_off = e.nparts ? ib + 2 + e.set_cap : nullptr;
@@ -735,7 +816,11 @@ struct Engine {
             lds_trials = std::max(lds_trials, is_trials_lds_bytes(e));
             if (want_sets) el_stats.sets_bytes += (u64)q.K * st * 8;   // generator read once
         }
-        HIPCHK(hipMemcpyAsync(el_u64.p, hu.data(), nu * 8, hipMemcpyHostToDevice, stream));
+        for (int j = 0; j < njobs; ++j) {   // inputs only: generator and detector rows
+            const qsf_elim_job& q = jobs[j];
+            const size_t w = (size_t)q.K * ((q.n + 63) / 64) + (size_t)std::max(0, q.kcheck) * ((q.n + 63) / 64);
+            if (w) HIPCHK(hipMemcpyAsync(el_u64.p + u_off[j], hu.data() + u_off[j], w * 8, hipMemcpyHostToDevice, stream));
+        }
         HIPCHK(hipMemcpyAsync(el_int.p, hi.data(), ni * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(el_jobs.p, el_hjobs.data(), njobs * sizeof(ElimJob), hipMemcpyHostToDevice, stream));
         if (want_sets) {
@@ -752,7 +837,7 @@ struct Engine {
                 HIPCHK(launch_is_trials(el_jobs.p, njobs, lds_trials, seed, t0, std::min(batch, trials - t0), stream));
             HIPCHK(hipEventRecord(b1, stream));
         }
-        HIPCHK(hipMemcpyAsync(hu.data(), el_u64.p, nu * 8, hipMemcpyDeviceToHost, stream));
+        if (want_sets) HIPCHK(hipMemcpyAsync(hu.data(), el_u64.p, nu * 8, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(hi.data(), el_int.p, ni * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         float ms = 0;
@@ -777,6 +862,7 @@ struct Engine {
             }
             const int keep = std::min(q.nsets, q.set_cap);
             std::memcpy(q.ranks, h + 2, (size_t)keep * 4);
+            if (!q.sets) continue;
             std::memcpy(q.sets, hu.data() + u_off[j] + (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st,
                         (size_t)keep * q.K * st * 8);
             el_stats.sets_bytes += (u64)keep * q.K * st * 8;   // systematic generators written
@@ -824,8 +910,7 @@ struct Engine {
     int enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out) {
         stats = qsf_hip_stats{};
         cache_cap = std::max(cache_cap, (size_t)n + 6);
-        std::vector<Solve*> solves(n, nullptr);
-        for (int j = 0; j < n; ++j) solves[j] = &solve_for(plans[j]);   // cache_cap >= n: none evicted
+        std::vector<Solve*> solves = solves_for(plans, n);   // cache_cap > n: none evicted
         std::vector<TItem> items;
         std::vector<int> item_of(n, -1), sweep;
         for (int j = 0; j < n; ++j) {
@@ -1329,7 +1414,7 @@ int qsf_hip_eliminate(qsf_elim_job* jobs, int njobs, int trials, uint64_t seed, 
         const qsf_elim_job& q = jobs[j];
         const bool sets = (what & QSF_ELIM_SETS) != 0;
         if (q.K < 0 || q.n <= 0 || q.n > qsa::kElimMaxN || (q.K > 0 && !q.gen) || q.kcheck < 0 ||
-            (q.kcheck > 0 && !q.check) || (sets && (q.set_cap < 1 || !q.sets || !q.ranks)) ||
+            (q.kcheck > 0 && !q.check) || (sets && (q.set_cap < 1 || !q.ranks)) ||
             (q.nparts > 0 && (!q.part_off || !q.part_cols)) || !qsa::is_permutation(q.order, q.n)) {
             engine().err = "qsf_hip_eliminate: bad job " + std::to_string(j) + " (n must be in [1, 4096])";
             return 1;

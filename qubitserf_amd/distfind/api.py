@@ -153,22 +153,27 @@ def last_trace():
     return out
 
 
-def batch_distance(problems, *, work_cap=0, verbose=False):
+def batch_distance(problems, *, work_cap=0, is_trials=0, verbose=False):
     """Distances of many codes in one batched solve (``distfind_batch_distance``).
 
     Each problem is ``("classical", H)``, ``("classical", H, order)``,
     ``("classical", H, None, (scores, candidates, seed))`` or ``("css", Hx, Hz, which)``;
     ``order`` is a column order for the information-set peel, ``scores`` a proposal's
-    column scores from which ``candidates`` orders are sampled.
+    column scores from which ``candidates`` orders are sampled. A problem may also be a
+    dict ``{"problem": <tuple>, "work_cap": cap}`` with its own work cap.
     Returns a list of ``DistfindResult`` (distance, lower_bound, proven, levels), identical
     to the single-code entry points' results; with ``work_cap`` > 0 a problem whose next
-    level would exceed that many codewords stops with ``proven == 0``."""
+    level would exceed that many codewords stops with ``proven == 0``; ``is_trials`` sets the
+    random information-set trials per problem (0: 256)."""
     L = _native.load()
     _normalize_backend("gpu")
     items = (_native.DistfindBatchItem * max(1, len(problems)))()
     keep = []
     for i, p in enumerate(problems):
         it = items[i]
+        if isinstance(p, dict):
+            it.work_cap = int(p.get("work_cap", 0))
+            p = p["problem"]
         if p[0] == "classical":
             H = _as_u8(p[1])
             it.kind, it.A, it.ar, it.ac = 0, _ptr(H), H.shape[0], H.shape[1]
@@ -194,7 +199,7 @@ def batch_distance(problems, *, work_cap=0, verbose=False):
         else:
             raise ValueError("problem kind must be 'classical' or 'css'")
     out = (_native.DistfindResult * max(1, len(problems)))()
-    rc = L.distfind_batch_distance(items, len(problems), int(work_cap), 1 if verbose else 0, out)
+    rc = L.distfind_batch_distance(items, len(problems), int(work_cap), int(is_trials), 1 if verbose else 0, out)
     _check(rc, "batch_distance")
     del keep
     return [out[i] for i in range(len(problems))]

@@ -33,7 +33,7 @@ def training_codes(count=2000, seed=0, n_lo=64, n_hi=200):
     return out
 
 
-def traces(Hs, orders=8, work_cap=1_000_000, seed=0, tau=1.0):
+def traces(Hs, orders=8, work_cap=1_000_000, seed=0, tau=1.0, chunk=1024, is_trials=16):
     """Per code: (targets y (n,), gains per order). Runs the capped solves in one batch."""
     rng = np.random.default_rng(seed + 7)
     items, meta = [], []
@@ -43,7 +43,9 @@ def traces(Hs, orders=8, work_cap=1_000_000, seed=0, tau=1.0):
         for o in ords:
             items.append(("classical", H, o))
             meta.append((c, o))
-    res = df.batch_distance(items, work_cap=work_cap)
+    res = []
+    for s in range(0, len(items), chunk):   # bounded batches: each holds its problems' device tables
+        res += df.batch_distance(items[s:s + chunk], work_cap=work_cap, is_trials=is_trials)
     gens = [engine.pack_rows(codes.gf2_kernel(H)) for H in Hs]
     probs = [dict(gen=gens[c], n=Hs[c].shape[1], order=o, set_cap=1) for c, o in meta]
     sets = engine.eliminate(probs, what=engine.ELIM_SETS)
