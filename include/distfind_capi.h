@@ -2,12 +2,13 @@
  * (python/qubitserf/distfind/_native.py:39-107), with the reference's signatures and
  * struct layouts (cpp/distfind/capi.h:12-111).
  *
- * What this engine answers: method "bz" with backend "gpu" or "auto", on the device.
- * Return codes: 0 ok; 1 null pointer (capi.cpp:35, :71); 2 odd column count for a
- * symplectic matrix (capi.cpp:171); 3 "cc" on a non-CSS code (capi.cpp:214); and, new
- * here, 4 method not provided by this engine ("mitm"/"cc"; the operator-weight and
+ * What this engine answers: methods "bz" and (CSS and classical codes) "mitm" with
+ * backend "gpu" or "auto", on the device. Return codes: 0 ok; 1 null pointer (capi.cpp:35,
+ * :71); 2 odd column count for a symplectic matrix (capi.cpp:171); 3 "cc" on a non-CSS code
+ * (capi.cpp:214); and, new here, 4 method not provided by this engine ("cc", symplectic
+ * "mitm", "mitm" beyond 63 parity checks / 64 detectors / n = 128; the operator-weight and
  * subsystem entry points), 5 backend not provided ("cpu", or no gfx950 device), 6 device
- * error (qsf_hip_last_error() / distfind_last_error() has the text). There is no silent
+ * error (distfind_last_error() has the text). There is no silent
  * CPU fallback: `DistfindResult.backend` is always "hip" on success.
  */
 #ifndef QSA_DISTFIND_CAPI_H
@@ -142,6 +143,17 @@ typedef struct {
 int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int is_trials, int verbose,
                             DistfindResult* out);
 void distfind_last_batch_stats(DistfindBatchStats* out);
+
+/* ---- text formats of the distfind tool (cpp/distfind/cli/distfind_cli.cpp:100-188) ----
+ * 0/1 matrix: one row per line, "0"/"1" tokens separated by whitespace, blank lines
+ * skipped; out gets rows x cols bytes (row-major, at most cap; NULL to size only).
+ * Pauli CSS stabilisers: one per line (I . _ space = identity, X, Z), a blank line ends
+ * the input; Y or mixed X/Z lines are rejected; all-identity lines are dropped; hx / hz get
+ * the X-type / Z-type rows (n bytes each, at most cap bytes per matrix).
+ * Return 0, or 2 with distfind_last_error() saying what is wrong (1 for NULL outputs). */
+int distfind_parse_matrix_text(const char* text, uint8_t* out, int64_t cap, int* rows, int* cols);
+int distfind_parse_pauli_css(const char* text, uint8_t* hx, uint8_t* hz, int64_t cap, int* hx_rows, int* hz_rows,
+                             int* n);
 
 #ifdef __cplusplus
 }

@@ -145,6 +145,16 @@ typedef struct {
 } qsf_hip_elim_stats;
 void qsf_hip_last_elim_stats(qsf_hip_elim_stats* out);
 
+/* Meet-in-the-middle test of one weight (replaces has_codeword_of_weight,
+ * cpp/distfind/mitm.cpp:241-314): *hit = 1 iff some word x of Hamming weight d has a zero
+ * syndrome and, unless count_all, a nonzero detector fingerprint. syn[j] holds column j of
+ * the code's parity checks (bit r = row r; at most 63 rows), log[j] column j of the detector
+ * rows (at most 64). Columns [0, nl) form the left half and [nl, n) the right, at most 64
+ * each. Returns 0; 1 bad arguments; 3 HIP error; 4 a weight split too large for the device
+ * table (more than 2^27 parts on the smaller side). */
+int qsf_hip_mitm_weight(const uint64_t* syn, const uint64_t* log, int n, int nl, int d, int count_all, int* hit);
+const char* qsf_hip_mitm_last_error(void);
+
 /* Statistics of the last enumerate/collect call on this thread's library instance:
  * codewords examined (C(K,d) x num_gamma, the reference's `work`, bz.cpp:151-154), device
  * time in milliseconds of the enumeration kernel, and which kernel ran

@@ -17,6 +17,10 @@
 
 using namespace qsa;
 
+namespace qsa {
+void set_last_error(const std::string& msg);
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -34,12 +38,14 @@ void fill(DistfindResult* out, const BzResult& r) {
     std::strncpy(out->backend, "hip", sizeof(out->backend) - 1);
 }
 
-// Shared argument policy: method "bz" only; backend "gpu"/"auto" only, device present.
+// Shared argument policy: method "bz" (and "mitm" for CSS and classical codes); backend
+// "gpu"/"auto" only, device present.
 int check_args(const char* method, const char* backend, bool noncss = false) {
     std::string m = method ? method : "bz";
     std::string b = backend ? backend : "auto";
     if (m == "cc" && noncss) { g_err = "connected cluster has no non-CSS form"; return RC_CC_NONCSS; }
-    if (m != "bz") { g_err = "method '" + m + "' is not provided by the gfx950 engine (it runs 'bz')"; return RC_METHOD; }
+    if (m == "mitm" && noncss) { g_err = "symplectic meet-in-the-middle is not provided by the gfx950 engine"; return RC_METHOD; }
+    if (m != "bz" && m != "mitm") { g_err = "method '" + m + "' is not provided by the gfx950 engine (it runs 'bz' and 'mitm')"; return RC_METHOD; }
     if (b != "gpu" && b != "auto") { g_err = "backend '" + b + "' is not provided by the gfx950 engine (use 'gpu')"; return RC_BACKEND; }
     if (!qsf_hip_available()) { g_err = std::string("no usable gfx950 device: ") + qsf_hip_last_error(); return RC_BACKEND; }
     return RC_OK;
@@ -60,18 +66,26 @@ int guarded(F&& f) {
         return f();
     } catch (const EngineError& e) {
         g_err = e.what;
-        return RC_DEVICE;
+        return e.code == 4 ? RC_METHOD : RC_DEVICE;   // 4: the problem exceeds what a kernel takes
     } catch (const std::exception& e) {
         g_err = e.what();
         return RC_DEVICE;
     }
 }
 
-int css_solve(const BitMat& Hx, const BitMat& Hz, char which, int verbose, DistfindResult* out) {
+bool is_mitm(const char* method) { return method && std::string(method) == "mitm"; }
+
+// css side / min dispatch, by method (cpp/distfind/capi.cpp:54-60)
+int css_solve(const BitMat& Hx, const BitMat& Hz, char which, int verbose, DistfindResult* out, bool mitm = false) {
     BzOptions o = options(verbose);
     BzResult r;
-    if (which == 'Z' || which == 'X') r = bz_distance(css_side(Hx, Hz, which), o);
-    else r = bz_min_interleaved(css_side(Hx, Hz, 'Z'), css_side(Hx, Hz, 'X'), o);
+    if (which == 'Z' || which == 'X') {
+        const Problem p = css_side(Hx, Hz, which);
+        r = mitm ? mitm_distance(p, o) : bz_distance(p, o);
+    } else {
+        const Problem pz = css_side(Hx, Hz, 'Z'), px = css_side(Hx, Hz, 'X');
+        r = mitm ? mitm_min_interleaved(pz, px, o) : bz_min_interleaved(pz, px, o);
+    }
     fill(out, r);
     return RC_OK;
 }
@@ -88,7 +102,7 @@ int distfind_css_distance(const uint8_t* Hx, int hx_rows, int hx_cols, const uin
     if (int rc = check_args(method, backend)) return rc;
     return guarded([&] {
         return css_solve(BitMat::from_bytes(Hx, hx_rows, hx_cols), BitMat::from_bytes(Hz, hz_rows, hz_cols), which,
-                         verbose, out);
+                         verbose, out, is_mitm(method));
     });
 }
 
@@ -99,7 +113,8 @@ int distfind_classical_distance(const uint8_t* H, int rows, int cols, const char
     if (int rc = check_args(method, backend)) return rc;
     return guarded([&] {
         BzOptions o = options(verbose);
-        fill(out, bz_distance(classical(BitMat::from_bytes(H, rows, cols)), o));
+        const Problem p = classical(BitMat::from_bytes(H, rows, cols));
+        fill(out, is_mitm(method) ? mitm_distance(p, o) : bz_distance(p, o));
         return RC_OK;
     });
 }
@@ -116,7 +131,7 @@ int distfind_stabilizer_distance(const uint8_t* S, int rows, int cols, const cha
         if (css) {
             BitMat Hx, Hz;
             split_css(m, Hx, Hz);
-            return css_solve(Hx, Hz, which, verbose, out);
+            return css_solve(Hx, Hz, which, verbose, out, is_mitm(method));
         }
         BzOptions o = options(verbose);
         if (o.verbose)
@@ -355,3 +370,7 @@ int32_t codeaut_backend_available(const char* name) {
 const char* codeaut_version(void) { return "0.1.0+gfx950"; }
 
 }  // extern "C"
+
+namespace qsa {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}

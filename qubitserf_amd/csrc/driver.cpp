@@ -641,4 +641,120 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
     return out;
 }
 
+// ---------------------------------------------------------------- meet in the middle
+
+namespace {
+
+// One MITM subproblem stepped one weight at a time (state as cpp/distfind/mitm.cpp:403-450).
+struct MitmSide {
+    bool has = false, resolved = false, confirmed = false, count_all = false;
+    int n = 0, nl = 0, target = 0, seed_upper = 0, found = -1, level = 0;
+    const char* tag = "";
+    std::vector<u64> syn, log;
+
+    void setup(const Problem& p, const char* t) {
+        tag = t;
+        n = p.n;
+        const int K = p.gen.rows;
+        has = !(K == 0 || (!p.every_nonzero && p.detector.rows == 0));
+        if (!has) { resolved = confirmed = true; return; }
+        const BitMat H = kernel_basis(p.gen);   // the code's parity checks
+        if (H.rows > 63 || p.detector.rows > 64 || n > 128 || n - n / 2 > 64)
+            throw EngineError{4, "mitm on the device needs at most 63 parity checks, 64 detector rows and n <= 128"};
+        count_all = p.every_nonzero;
+        syn.assign(n, 0);
+        log.assign(n, 0);
+        for (int r = 0; r < H.rows; ++r)
+            for (int j = 0; j < n; ++j)
+                if (H.bit(r, j)) syn[j] |= 1ull << r;
+        if (!count_all)
+            for (int c = 0; c < p.detector.rows; ++c)
+                for (int j = 0; j < n; ++j)
+                    if (p.detector.bit(c, j)) log[j] |= 1ull << c;
+        nl = n / 2;
+        seed_upper = p.seed_upper;
+        target = seed_upper > 0 ? std::min(seed_upper, n) : n;
+    }
+
+    bool step(int d) {
+        ++level;
+        int hit = 0;
+        const int rc = qsf_hip_mitm_weight(syn.data(), log.data(), n, nl, d, count_all ? 1 : 0, &hit);
+        if (rc != 0) throw EngineError{rc, qsf_hip_mitm_last_error()};
+        if (hit) found = d;
+        return hit != 0;
+    }
+};
+
+}  // namespace
+
+BzResult mitm_distance(const Problem& p, const BzOptions& opt) {
+    auto t0 = clk::now();
+    BzResult r;
+    MitmSide S;
+    S.setup(p, "");
+    if (S.has) {
+        for (int d = 1; d <= S.target; ++d) {
+            if (S.step(d)) {
+                if (opt.verbose) say_final(d, std::chrono::duration<double>(clk::now() - t0).count(), "");
+                break;
+            }
+        }
+        if (S.found < 0 && S.seed_upper > 0 && S.target == S.seed_upper) S.found = S.seed_upper;
+        r.distance = S.found;
+        r.levels = S.level;
+        r.lower_bound = S.found >= 0 ? S.found : 0;
+        r.proven = S.found >= 0;
+    }
+    r.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+    return r;
+}
+
+BzResult mitm_min_interleaved(const Problem& pz, const Problem& px, const BzOptions& opt) {
+    auto t0 = clk::now();
+    MitmSide Z, X;
+    Z.setup(pz, "Z");
+    X.setup(px, "X");
+    MitmSide* sides[2] = {&Z, &X};
+    int best = kNoWeight;
+    if (Z.has && Z.seed_upper > 0) best = std::min(best, Z.seed_upper);
+    if (X.has && X.seed_upper > 0) best = std::min(best, X.seed_upper);
+    const int maxd = std::max(Z.has ? Z.target : 0, X.has ? X.target : 0);
+    for (int d = 1; d <= maxd; ++d) {
+        for (MitmSide* S : sides) {
+            if (S->resolved) continue;
+            if (d > S->target) {   // its own ceiling without a hit: the seed certifies it
+                if (S->seed_upper > 0 && S->target == S->seed_upper) {
+                    S->found = S->seed_upper;
+                    S->confirmed = true;
+                    best = std::min(best, S->found);
+                }
+                S->resolved = true;
+                continue;
+            }
+            if (d >= best) {   // cannot lower the running minimum
+                S->confirmed = true;
+                if (S->seed_upper > 0 && best == S->seed_upper) S->found = best;
+                S->resolved = true;
+                continue;
+            }
+            if (S->step(d)) {
+                best = std::min(best, d);
+                S->confirmed = S->resolved = true;
+                if (opt.verbose) say_final(d, std::chrono::duration<double>(clk::now() - t0).count(), S->tag);
+            }
+        }
+        if (Z.resolved && X.resolved) break;
+    }
+    auto v = [](const MitmSide& S) { return S.found >= 0 ? S.found : kNoWeight; };
+    const int dist = std::min(v(Z), v(X));
+    BzResult r;
+    r.distance = dist >= kNoWeight ? -1 : dist;
+    r.lower_bound = dist >= kNoWeight ? 0 : dist;
+    r.levels = std::max(Z.level, X.level);
+    r.proven = Z.confirmed && X.confirmed && dist < kNoWeight;
+    r.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+    return r;
+}
+
 }  // namespace qsa

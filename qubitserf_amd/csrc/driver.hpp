@@ -48,6 +48,14 @@ BzResult bz_distance(const Problem& p, const BzOptions& opt);
 // lower bound reaches the best weight found so far (cpp/distfind/bz.cpp:296-357).
 BzResult bz_min_interleaved(const Problem& pz, const Problem& px, const BzOptions& opt);
 
+// Meet-in-the-middle distance of one subproblem on the device (cpp/distfind/mitm.cpp:318-392):
+// weights d = 1, 2, ... up to the seeded upper bound, first hit wins. Needs at most 63
+// parity checks, 64 detector rows and n <= 128 (EngineError 4 otherwise).
+BzResult mitm_distance(const Problem& p, const BzOptions& opt);
+// The interleaved CSS minimum of the two sides, capping a side once it cannot lower the
+// best (cpp/distfind/mitm.cpp:463-537).
+BzResult mitm_min_interleaved(const Problem& pz, const Problem& px, const BzOptions& opt);
+
 // One problem of a batch: a single side (classical, or one CSS side), or the two CSS
 // sides of a min-distance solve (pair), with an optional proposal order for a single side.
 // A learned proposal for one single-side problem: a score per column; `candidates` column
