@@ -144,6 +144,12 @@ int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t w
                             DistfindResult* out);
 void distfind_last_batch_stats(DistfindBatchStats* out);
 
+/* Host logic exposed for tests (no device needed): the Edmonds matroid partition of the
+ * columns of G (rows x cols uint8) into independent classes (cpp/distfind/css.cpp:274-328),
+ * classes ordered by size as the reference sorts them (css.cpp:354-355), empty classes
+ * dropped; part_of_col[c] = class of column c. Return 0, 1 bad arguments, 6 failed check. */
+int distfind_matroid_parts(const uint8_t* G, int rows, int cols, int* part_of_col);
+
 /* ---- text formats of the distfind tool (cpp/distfind/cli/distfind_cli.cpp:100-188) ----
  * 0/1 matrix: one row per line, "0"/"1" tokens separated by whitespace, blank lines
  * skipped; out gets rows x cols bytes (row-major, at most cap; NULL to size only).

@@ -274,6 +274,8 @@ static bool matroid_partition(const Mat& gen, std::vector<std::vector<int>>& out
     return true;
 }
 
+bool matroid_partition_public(const Mat& gen, std::vector<std::vector<int>>& out) { return matroid_partition(gen, out); }
+
 static int nfull(const std::vector<Gamma>& s, int K) { int f = 0; for (auto& g : s) if (g.rank == K) ++f; return f; }
 
 std::vector<Gamma> gamma_sequence(const Mat& gen) {
@@ -971,6 +973,24 @@ void oracle_part_sets(const uint64_t* gen, int K, int n, const int* cols, const 
         ranks[p] = (int)orc::rref_cols(W, c).size();
         std::memcpy(out + (size_t)p * G.w.size(), W.w.data(), G.w.size() * 8);
     }
+}
+
+// The matroid partition of gen's columns (css.cpp:274-328), classes sorted by size as
+// css.cpp:354-355 sorts them, empty classes dropped: part_of_col[c] = class of column c.
+int oracle_matroid_parts(const uint64_t* gen, int K, int n, int* part_of_col) {
+    orc::Mat G(K, n);
+    std::memcpy(G.w.data(), gen, G.w.size() * 8);
+    std::vector<std::vector<int>> part;
+    if (!orc::matroid_partition_public(G, part)) return -1;
+    std::sort(part.begin(), part.end(), [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
+    std::fill(part_of_col, part_of_col + n, -1);
+    int p = 0;
+    for (auto& c : part) {
+        if (c.empty()) continue;
+        for (int j : c) part_of_col[j] = p;
+        ++p;
+    }
+    return p;
 }
 
 // Problem dimensions and information-set ranks, for the host-logic tests.

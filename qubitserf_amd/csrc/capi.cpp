@@ -198,6 +198,18 @@ int distfind_last_trace(DistfindLevel* out, int cap) {
     return n;
 }
 
+// ---------------------------------------------------------------- host logic, for tests
+
+int distfind_matroid_parts(const uint8_t* G, int rows, int cols, int* part_of_col) {
+    if (!G || !part_of_col || rows < 0 || cols <= 0) { g_err = "null or empty generator"; return RC_NULL; }
+    std::vector<std::vector<int>> parts;
+    if (!matroid_parts(BitMat::from_bytes(G, rows, cols), parts)) { g_err = "partition check failed"; return RC_DEVICE; }
+    std::fill(part_of_col, part_of_col + cols, -1);
+    for (size_t p = 0; p < parts.size(); ++p)
+        for (int c : parts[p]) part_of_col[c] = (int)p;
+    return RC_OK;
+}
+
 // ---------------------------------------------------------------- batched solves
 
 namespace {

@@ -224,7 +224,8 @@ namespace {
 // information sets and the BZ lower bounds) match the reference's.
 class Partitioner {
   public:
-    explicit Partitioner(const BitMat& gen) : n_(gen.cols), kw_(words(gen.rows)), col_((size_t)gen.cols * words(gen.rows), 0) {
+    explicit Partitioner(const BitMat& gen)
+        : n_(gen.cols), kw_(words(gen.rows)), ow_(words(gen.cols)), col_((size_t)gen.cols * words(gen.rows), 0) {
         for (int j = 0; j < n_; ++j) {
             bool any = false;
             for (int i = 0; i < gen.rows; ++i)
@@ -251,42 +252,57 @@ class Partitioner {
     }
 
   private:
-    static std::vector<int> symdiff(const std::vector<int>& a, const std::vector<int>& b) {
-        std::vector<int> r;
-        std::set_symmetric_difference(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(r));
-        return r;
-    }
     const u64* colv(int j) const { return col_.data() + (size_t)j * kw_; }
-    int lowbit(const std::vector<u64>& v) const {
-        for (int k = 0; k < kw_; ++k) if (v[k]) return k * 64 + __builtin_ctzll(v[k]);
-        return -1;
-    }
-    // true iff column z lies in the span of class `cls`; then *circuit = class columns
-    // whose XOR is z.
-    bool spans(const std::vector<int>& cls, int z, std::vector<int>* circuit) const {
-        std::vector<std::vector<u64>> bv;
-        std::vector<int> bl;
-        std::vector<std::vector<int>> bo;
-        for (int idx : cls) {
-            std::vector<u64> v(colv(idx), colv(idx) + kw_);
-            std::vector<int> o{idx};
-            for (size_t b = 0; b < bv.size(); ++b)
-                if ((v[bl[b] >> 6] >> (bl[b] & 63)) & 1) {
-                    for (int k = 0; k < kw_; ++k) v[k] ^= bv[b][k];
-                    o = symdiff(o, bo[b]);
+    // true iff column z lies in the span of class `cls`; then *circuit = the class columns
+    // whose XOR is z, ascending. A class is always independent, so that set is unique and
+    // equals the one the reference's origin-tracking reduction finds (css.cpp:231-268).
+    // Basis vectors and their origins (bitsets over columns) live in reused flat buffers.
+    bool spans(const std::vector<int>& cls, int z, std::vector<int>* circuit) {
+        const int kw = kw_, ow = ow_;
+        const bool track = circuit != nullptr;
+        bv_.resize(cls.size() * kw);
+        bl_.resize(cls.size());
+        if (track) bo_.resize(cls.size() * ow);
+        size_t nb = 0;
+        tv_.resize(kw);
+        to_.resize(ow);
+        auto reduce = [&](int col) {   // tv_ = column reduced by the basis so far, to_ its origin
+            std::copy(colv(col), colv(col) + kw, tv_.begin());
+            if (track) std::fill(to_.begin(), to_.end(), 0ull);
+            for (size_t b = 0; b < nb; ++b) {
+                const int l = bl_[b];
+                if ((tv_[l >> 6] >> (l & 63)) & 1) {
+                    const u64* v = bv_.data() + b * kw;
+                    for (int k = 0; k < kw; ++k) tv_[k] ^= v[k];
+                    if (track) {
+                        const u64* o = bo_.data() + b * ow;
+                        for (int k = 0; k < ow; ++k) to_[k] ^= o[k];
+                    }
                 }
-            int l = lowbit(v);
-            if (l >= 0) { bv.push_back(std::move(v)); bl.push_back(l); bo.push_back(std::move(o)); }
-        }
-        std::vector<u64> v(colv(z), colv(z) + kw_);
-        std::vector<int> o;
-        for (size_t b = 0; b < bv.size(); ++b)
-            if ((v[bl[b] >> 6] >> (bl[b] & 63)) & 1) {
-                for (int k = 0; k < kw_; ++k) v[k] ^= bv[b][k];
-                o = symdiff(o, bo[b]);
             }
-        for (u64 x : v) if (x) return false;
-        if (circuit) *circuit = std::move(o);
+        };
+        for (int idx : cls) {
+            reduce(idx);
+            int l = -1;
+            for (int k = 0; k < kw && l < 0; ++k)
+                if (tv_[k]) l = k * 64 + __builtin_ctzll(tv_[k]);
+            if (l < 0) continue;
+            std::copy(tv_.begin(), tv_.end(), bv_.begin() + nb * kw);
+            bl_[nb] = l;
+            if (track) {
+                to_[idx >> 6] ^= 1ull << (idx & 63);
+                std::copy(to_.begin(), to_.end(), bo_.begin() + nb * ow);
+            }
+            ++nb;
+        }
+        reduce(z);
+        for (u64 x : tv_)
+            if (x) return false;
+        if (track) {
+            circuit->clear();
+            for (int k = 0; k < ow; ++k)
+                for (u64 m = to_[k]; m; m &= m - 1) circuit->push_back(k * 64 + __builtin_ctzll(m));
+        }
         return true;
     }
     bool place(int z, int forbidden) {
@@ -310,8 +326,10 @@ class Partitioner {
         return false;
     }
 
-    int n_, kw_;
+    int n_, kw_, ow_;
     std::vector<u64> col_;
+    std::vector<u64> bv_, bo_, tv_, to_;   // span-test scratch
+    std::vector<int> bl_;
     std::vector<int> nonzero_, zero_;
     std::vector<std::vector<int>> classes_;
     std::vector<char> seen_;

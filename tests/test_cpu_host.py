@@ -133,3 +133,27 @@ def test_config3_law():
         assert 64 <= n <= 160
         assert 0.29 <= (n - m) / n <= 0.61
         assert codes.gf2_rank(H) == m
+
+
+def test_matroid_partition_matches_oracle():
+    """The host matroid partition (product) equals the oracle's restatement of the
+    reference's (cpp/distfind/css.cpp:274-328) class for class, on generators where the
+    greedy peel is short of full-rank sets and the partition decides the lower bounds."""
+    from tests import oracle as O
+    L = _native.load()
+    L.distfind_matroid_parts.argtypes = [_native.u8p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    ip = ctypes.POINTER(ctypes.c_int)
+    O.lib().oracle_matroid_parts.argtypes = [O.u64p, ctypes.c_int, ctypes.c_int, ip]
+    rng = np.random.default_rng(12)
+    cases = [codes.gf2_kernel(codes.random_rate_code(rng, 40, 120, 0.3, 0.45)) for _ in range(25)]
+    cases += [codes.gf2_kernel(codes.quasi_cyclic(rng, L)) for L in (5, 7, 9)]
+    cases += [codes.gf2_kernel(codes.hamming_parity(4)), codes.reed_muller_generator(1, 5)]
+    for G in cases:
+        G = np.ascontiguousarray(G, np.uint8)
+        K, n = G.shape
+        a = np.zeros(n, np.int32)
+        assert L.distfind_matroid_parts(G.ctypes.data_as(_native.u8p), K, n, a.ctypes.data_as(ip)) == 0
+        b = np.zeros(n, np.int32)
+        packed = engine.pack_rows(G)
+        assert O.lib().oracle_matroid_parts(packed.ctypes.data_as(O.u64p), K, n, b.ctypes.data_as(ip)) >= 0
+        assert np.array_equal(a, b), (K, n)
