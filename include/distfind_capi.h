@@ -144,6 +144,21 @@ int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t w
                             DistfindResult* out);
 void distfind_last_batch_stats(DistfindBatchStats* out);
 
+/* Continuous batching (a session): problems join between rounds and leave when done.
+ * distfind_session_new returns NULL when the engine is unusable (distfind_last_error).
+ * add: builds the items' problems and their information sets and bounds on the device,
+ * writes one id per item. step: one round — one level of every unfinished problem in one
+ * batched device call; *active = problems still unfinished. poll: up to cap finished
+ * problems (id, result; result.seconds = time since its add), returns the count. Results
+ * equal the single-call results. A session is not thread-safe. */
+void* distfind_session_new(int64_t work_cap, int is_trials);
+int distfind_session_add(void* session, const DistfindBatchItem* items, int count, int64_t* ids);
+int distfind_session_step(void* session, int* active);
+int distfind_session_poll(void* session, int64_t* ids, DistfindResult* out, int cap);
+int distfind_session_active(void* session);
+void distfind_session_stats(void* session, DistfindBatchStats* out);
+void distfind_session_free(void* session);
+
 /* Host logic exposed for tests (no device needed): the Edmonds matroid partition of the
  * columns of G (rows x cols uint8) into independent classes (cpp/distfind/css.cpp:274-328),
  * classes ordered by size as the reference sorts them (css.cpp:354-355), empty classes

@@ -166,6 +166,14 @@ def load() -> ctypes.CDLL:
     sig("distfind_batch_distance", c_int,
         [ctypes.POINTER(DistfindBatchItem), c_int, ctypes.c_int64, c_int, c_int, rp])
     sig("distfind_last_batch_stats", None, [ctypes.POINTER(DistfindBatchStats)])
+    hp = ctypes.c_void_p
+    sig("distfind_session_new", hp, [ctypes.c_int64, c_int])
+    sig("distfind_session_add", c_int, [hp, ctypes.POINTER(DistfindBatchItem), c_int, ctypes.POINTER(ctypes.c_int64)])
+    sig("distfind_session_step", c_int, [hp, ctypes.POINTER(c_int)])
+    sig("distfind_session_poll", c_int, [hp, ctypes.POINTER(ctypes.c_int64), rp, c_int])
+    sig("distfind_session_active", c_int, [hp])
+    sig("distfind_session_stats", None, [hp, ctypes.POINTER(DistfindBatchStats)])
+    sig("distfind_session_free", None, [hp])
 
     vp = ctypes.c_void_p
     sig("codeaut_bz_collect", vp,

@@ -216,12 +216,13 @@ namespace {
 thread_local BatchStats g_batch;
 }
 
-int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int is_trials, int verbose,
-                            DistfindResult* out) {
-    if (count < 0 || (count > 0 && (!items || !out))) { g_err = "null batch"; return RC_NULL; }
-    if (int rc = check_args("bz", "gpu")) return rc;
-    const auto t0 = std::chrono::steady_clock::now();
-    std::vector<BatchProblem> probs(count);
+}  // extern "C"
+
+namespace {
+
+// Batch items -> problems (problem build on the host, as the single entry points do).
+int build_problems(const DistfindBatchItem* items, int count, std::vector<BatchProblem>& probs) {
+    probs.assign(count, BatchProblem{});
     for (int i = 0; i < count; ++i) {
         const DistfindBatchItem& it = items[i];
         if (!it.A || (it.kind == 1 && !it.B)) { g_err = "null matrix in batch item " + std::to_string(i); return RC_NULL; }
@@ -259,6 +260,26 @@ int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t w
             }
         }
     }
+    return RC_OK;
+}
+
+struct SessionHandle {
+    BzOptions opt;
+    std::unique_ptr<BatchSession> s;
+    std::vector<std::pair<long long, BzResult>> pending;   // finished, not yet polled
+};
+
+}  // namespace
+
+extern "C" {
+
+int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int is_trials, int verbose,
+                            DistfindResult* out) {
+    if (count < 0 || (count > 0 && (!items || !out))) { g_err = "null batch"; return RC_NULL; }
+    if (int rc = check_args("bz", "gpu")) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<BatchProblem> probs;
+    if (int rc = build_problems(items, count, probs)) return rc;
     BzOptions o = options(verbose);
     o.trace = nullptr;
     if (is_trials > 0) o.is_trials = is_trials;
@@ -271,6 +292,71 @@ int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t w
         return RC_OK;
     });
 }
+
+void* distfind_session_new(int64_t work_cap, int is_trials) {
+    if (check_args("bz", "gpu")) return nullptr;
+    auto* h = new SessionHandle();
+    h->opt = options(0);
+    h->opt.trace = nullptr;
+    if (is_trials > 0) h->opt.is_trials = is_trials;
+    h->s = std::make_unique<BatchSession>(h->opt, work_cap > 0 ? (u64)work_cap : 0);
+    return h;
+}
+
+int distfind_session_add(void* handle, const DistfindBatchItem* items, int count, int64_t* ids) {
+    auto* h = static_cast<SessionHandle*>(handle);
+    if (!h || count < 0 || (count > 0 && (!items || !ids))) { g_err = "null session or items"; return RC_NULL; }
+    std::vector<BatchProblem> probs;
+    if (int rc = build_problems(items, count, probs)) return rc;
+    return guarded([&] {
+        std::vector<long long> got = h->s->add(std::move(probs));
+        for (int i = 0; i < count; ++i) ids[i] = got[i];
+        return RC_OK;
+    });
+}
+
+int distfind_session_step(void* handle, int* active) {
+    auto* h = static_cast<SessionHandle*>(handle);
+    if (!h) { g_err = "null session"; return RC_NULL; }
+    return guarded([&] {
+        const int a = h->s->step();
+        if (active) *active = a;
+        return RC_OK;
+    });
+}
+
+int distfind_session_poll(void* handle, int64_t* ids, DistfindResult* out, int cap) {
+    auto* h = static_cast<SessionHandle*>(handle);
+    if (!h || cap < 0 || (cap > 0 && (!ids || !out))) { g_err = "null session or outputs"; return -1; }
+    auto& pending = h->pending;
+    for (auto& f : h->s->take_finished()) pending.push_back(f);
+    const int n = std::min<int>(cap, (int)pending.size());
+    for (int i = 0; i < n; ++i) {
+        ids[i] = pending[i].first;
+        fill(out + i, pending[i].second);
+    }
+    pending.erase(pending.begin(), pending.begin() + n);
+    return n;
+}
+
+int distfind_session_active(void* handle) {
+    auto* h = static_cast<SessionHandle*>(handle);
+    return h ? h->s->active() : -1;
+}
+
+void distfind_session_stats(void* handle, DistfindBatchStats* out) {
+    auto* h = static_cast<SessionHandle*>(handle);
+    if (!h || !out) return;
+    const BatchStats& b = h->s->stats;
+    out->rounds = b.rounds;
+    out->capped = b.capped;
+    out->codewords = b.codewords;
+    out->device_ms = b.device_ms;
+    out->setup_s = b.setup_s;
+    out->seconds = b.seconds;
+}
+
+void distfind_session_free(void* handle) { delete static_cast<SessionHandle*>(handle); }
 
 void distfind_last_batch_stats(DistfindBatchStats* out) {
     if (!out) return;

@@ -543,70 +543,105 @@ struct Stepper {
 
 }  // namespace
 
-std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions& opt, unsigned long long work_cap,
-                               BatchStats* stats) {
+// A batch as a session: problems join between rounds, every round enumerates one level of
+// every unfinished problem, finished problems leave with their results.
+struct BatchSession::Impl {
+    struct Entry {
+        long long id;
+        std::unique_ptr<BatchProblem> prob;
+        std::unique_ptr<Stepper> st;
+        clk::time_point added;
+    };
+    BzOptions opt;
+    u64 work_cap;
+    long long next_id = 0;
+    std::vector<Entry> live;
+    std::vector<std::pair<long long, BzResult>> finished;
+
+    void retire_done() {
+        std::vector<Entry> keep;
+        for (Entry& e : live) {
+            if (!e.st->done) { keep.push_back(std::move(e)); continue; }
+            BzResult r = e.st->result();
+            r.seconds = std::chrono::duration<double>(clk::now() - e.added).count();
+            finished.emplace_back(e.id, r);
+        }
+        live.swap(keep);
+    }
+};
+
+BatchSession::BatchSession(const BzOptions& opt, unsigned long long work_cap) : impl_(std::make_unique<Impl>()) {
+    impl_->opt = opt;
+    impl_->work_cap = work_cap;
+}
+BatchSession::~BatchSession() = default;
+
+int BatchSession::active() const { return (int)impl_->live.size(); }
+
+std::vector<long long> BatchSession::add(std::vector<BatchProblem>&& probs) {
+    Impl& I = *impl_;
     auto t0 = clk::now();
-    BatchStats bs;
-    const size_t np = probs.size();
-    std::vector<std::unique_ptr<Stepper>> st(np);
+    std::vector<long long> ids;
     std::vector<Side*> sides;
     std::vector<const std::vector<int>*> orders;
     std::vector<const Proposal*> props;
-    for (size_t i = 0; i < np; ++i) {
-        st[i] = std::make_unique<Stepper>();
-        Stepper& S = *st[i];
-        BatchProblem& bp = probs[i];
+    const size_t first = I.live.size();
+    for (BatchProblem& bp0 : probs) {
+        Impl::Entry e{I.next_id++, std::make_unique<BatchProblem>(std::move(bp0)), std::make_unique<Stepper>(), t0};
+        BatchProblem& bp = *e.prob;
+        Stepper& S = *e.st;
         S.pair = bp.pair;
         S.cap = bp.work_cap;
-        S.A.prepare(bp.a, opt, bp.pair ? "Z" : "");
+        S.A.prepare(bp.a, I.opt, bp.pair ? "Z" : "");
         sides.push_back(&S.A);
         orders.push_back(!bp.pair && !bp.order.empty() ? &bp.order : nullptr);
         props.push_back(!bp.pair && bp.proposal.candidates > 0 ? &bp.proposal : nullptr);
         if (bp.pair) {
-            S.B.prepare(bp.b, opt, "X");
+            S.B.prepare(bp.b, I.opt, "X");
             sides.push_back(&S.B);
             orders.push_back(nullptr);
             props.push_back(nullptr);
         }
+        ids.push_back(e.id);
+        I.live.push_back(std::move(e));
     }
     setup_sides(sides, orders, props);
-    bs.setup_s = std::chrono::duration<double>(clk::now() - t0).count();
-    for (auto& S : st) {
-        S->init();
-        S->next();
+    for (size_t k = first; k < I.live.size(); ++k) {
+        I.live[k].st->init();
+        I.live[k].st->next();
     }
+    stats.setup_s += std::chrono::duration<double>(clk::now() - t0).count();
+    I.retire_done();
+    return ids;
+}
+
+int BatchSession::step() {
+    Impl& I = *impl_;
     std::vector<qsf_enum_plan> plans;
     std::vector<int> floors, found;
     std::vector<Stepper*> who;
     std::vector<u64> cw;
-    for (;;) {
-        plans.clear();
-        floors.clear();
-        who.clear();
-        cw.clear();
-        bool any = false;
-        for (auto& up : st) {
-            Stepper& S = *up;
-            while (!S.done && S.cur) {
-                Side* side = S.cur;
-                const u64 w = side->level_work(S.d);
-                const u64 spent = S.A.work + (S.pair ? S.B.work : 0);
-                const u64 cap = S.cap ? S.cap : work_cap;
-                if (cap && (w > cap || spent > cap - w)) { S.capped = S.done = true; break; }
-                qsf_enum_plan pl;
-                if (side->plan(S.d, pl)) {
-                    plans.push_back(pl);
-                    floors.push_back(side->outer);
-                    who.push_back(&S);
-                    cw.push_back(w);
-                    any = true;
-                    break;
-                }
-                S.after(kNoWeight, 0.0, qsf_hip_stats{});   // no generator active at this level
-                S.next();
+    for (auto& e : I.live) {
+        Stepper& S = *e.st;
+        while (!S.done && S.cur) {
+            Side* side = S.cur;
+            const u64 w = side->level_work(S.d);
+            const u64 spent = S.A.work + (S.pair ? S.B.work : 0);
+            const u64 cap = S.cap ? S.cap : I.work_cap;
+            if (cap && (w > cap || spent > cap - w)) { S.capped = S.done = true; break; }
+            qsf_enum_plan pl;
+            if (side->plan(S.d, pl)) {
+                plans.push_back(pl);
+                floors.push_back(side->outer);
+                who.push_back(&S);
+                cw.push_back(w);
+                break;
             }
+            S.after(kNoWeight, 0.0, qsf_hip_stats{});   // no generator active at this level
+            S.next();
         }
-        if (!any) break;
+    }
+    if (!plans.empty()) {
         found.assign(plans.size(), kNoWeight);
         auto tr = clk::now();
         const int rc = qsf_hip_enumerate_batch(plans.data(), floors.data(), (int)plans.size(), found.data());
@@ -614,12 +649,12 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
         qsf_hip_stats round{};
         qsf_hip_last_stats(&round);
         const double ms = std::chrono::duration<double, std::milli>(clk::now() - tr).count();
-        if (opt.profile)
-            std::fprintf(stderr, "[prof round] %d: %zu levels, %llu codewords, call %.3f ms, device %.3f ms\n", bs.rounds,
+        if (I.opt.profile)
+            std::fprintf(stderr, "[prof round] %d: %zu levels, %llu codewords, call %.3f ms, device %.3f ms\n", stats.rounds,
                          plans.size(), (unsigned long long)round.codewords, ms, round.kernel_ms);
-        ++bs.rounds;
-        bs.codewords += round.codewords;
-        bs.device_ms += round.kernel_ms;
+        ++stats.rounds;
+        stats.codewords += round.codewords;
+        stats.device_ms += round.kernel_ms;
         for (size_t k = 0; k < plans.size(); ++k) {
             qsf_hip_stats one{};
             one.codewords = cw[k];
@@ -629,15 +664,34 @@ std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions
             who[k]->next();
         }
     }
+    for (auto& e : I.live) stats.capped += (e.st->done && e.st->capped) ? 1 : 0;
+    I.retire_done();
+    return (int)I.live.size();
+}
+
+std::vector<std::pair<long long, BzResult>> BatchSession::take_finished() {
+    std::vector<std::pair<long long, BzResult>> out;
+    out.swap(impl_->finished);
+    return out;
+}
+
+std::vector<BzResult> bz_batch(std::vector<BatchProblem>& probs, const BzOptions& opt, unsigned long long work_cap,
+                               BatchStats* stats) {
+    auto t0 = clk::now();
+    BatchSession S(opt, work_cap);
+    const size_t np = probs.size();
+    std::vector<long long> ids = S.add(std::move(probs));
+    while (S.active() > 0) S.step();
     std::vector<BzResult> out(np);
+    std::unordered_map<long long, size_t> at;
+    for (size_t i = 0; i < np; ++i) at[ids[i]] = i;
     const double secs = std::chrono::duration<double>(clk::now() - t0).count();
-    for (size_t i = 0; i < np; ++i) {
-        out[i] = st[i]->result();
-        out[i].seconds = secs;
-        bs.capped += st[i]->capped ? 1 : 0;
+    for (auto& f : S.take_finished()) {
+        out[at[f.first]] = f.second;
+        out[at[f.first]].seconds = secs;
     }
-    bs.seconds = secs;
-    if (stats) *stats = bs;
+    S.stats.seconds = secs;
+    if (stats) *stats = S.stats;
     return out;
 }
 

@@ -1,6 +1,8 @@
 // Brouwer–Zimmermann weight-level driver over the gfx950 engine (host side).
 #pragma once
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 #include "problem.hpp"
 #include "../../include/qsf_hip.h"
@@ -83,6 +85,26 @@ struct BatchStats {
     double device_ms = 0.0;         // enumeration kernels
     double setup_s = 0.0;           // information sets and upper bounds (device) incl. host set-up
     double seconds = 0.0;
+};
+
+// A batch that problems can join between rounds (continuous batching): add() builds the
+// new problems' information sets and bounds on the device; step() enumerates one level of
+// every unfinished problem in one batched device call; finished problems (with
+// seconds = time since their add) are collected by take_finished(). Results equal the
+// single-call results.
+class BatchSession {
+  public:
+    BatchSession(const BzOptions& opt, unsigned long long work_cap);
+    ~BatchSession();
+    std::vector<long long> add(std::vector<BatchProblem>&& probs);
+    int step();                                  // returns the problems still unfinished
+    int active() const;
+    std::vector<std::pair<long long, BzResult>> take_finished();
+    BatchStats stats;
+
+  private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
 };
 
 // Many problems at once: each is stepped level by level exactly as bz_distance /

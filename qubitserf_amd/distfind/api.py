@@ -153,20 +153,7 @@ def last_trace():
     return out
 
 
-def batch_distance(problems, *, work_cap=0, is_trials=0, verbose=False):
-    """Distances of many codes in one batched solve (``distfind_batch_distance``).
-
-    Each problem is ``("classical", H)``, ``("classical", H, order)``,
-    ``("classical", H, None, (scores, candidates, seed))`` or ``("css", Hx, Hz, which)``;
-    ``order`` is a column order for the information-set peel, ``scores`` a proposal's
-    column scores from which ``candidates`` orders are sampled. A problem may also be a
-    dict ``{"problem": <tuple>, "work_cap": cap}`` with its own work cap.
-    Returns a list of ``DistfindResult`` (distance, lower_bound, proven, levels), identical
-    to the single-code entry points' results; with ``work_cap`` > 0 a problem whose next
-    level would exceed that many codewords stops with ``proven == 0``; ``is_trials`` sets the
-    random information-set trials per problem (0: 256)."""
-    L = _native.load()
-    _normalize_backend("gpu")
+def _batch_items(problems):
     items = (_native.DistfindBatchItem * max(1, len(problems)))()
     keep = []
     for i, p in enumerate(problems):
@@ -198,11 +185,83 @@ def batch_distance(problems, *, work_cap=0, is_trials=0, verbose=False):
             keep += [Hx, Hz]
         else:
             raise ValueError("problem kind must be 'classical' or 'css'")
+    return items, keep
+
+
+def batch_distance(problems, *, work_cap=0, is_trials=0, verbose=False):
+    """Distances of many codes in one batched solve (``distfind_batch_distance``).
+
+    Each problem is ``("classical", H)``, ``("classical", H, order)``,
+    ``("classical", H, None, (scores, candidates, seed))`` or ``("css", Hx, Hz, which)``;
+    ``order`` is a column order for the information-set peel, ``scores`` a proposal's
+    column scores from which ``candidates`` orders are sampled. A problem may also be a
+    dict ``{"problem": <tuple>, "work_cap": cap}`` with its own work cap.
+    Returns a list of ``DistfindResult`` (distance, lower_bound, proven, levels), identical
+    to the single-code entry points' results; with ``work_cap`` > 0 a problem whose next
+    level would exceed that many codewords stops with ``proven == 0``; ``is_trials`` sets the
+    random information-set trials per problem (0: 256)."""
+    L = _native.load()
+    _normalize_backend("gpu")
+    items, keep = _batch_items(problems)
     out = (_native.DistfindResult * max(1, len(problems)))()
     rc = L.distfind_batch_distance(items, len(problems), int(work_cap), int(is_trials), 1 if verbose else 0, out)
     _check(rc, "batch_distance")
     del keep
     return [out[i] for i in range(len(problems))]
+
+
+class Session:
+    """Continuous batching over ``distfind_session_*``: ``add`` problems (same forms as
+    ``batch_distance``) at any time, ``step`` runs one round (one level of every unfinished
+    problem), ``poll`` returns ``(id, DistfindResult)`` of the problems finished since."""
+
+    def __init__(self, work_cap=0, is_trials=0):
+        _normalize_backend("gpu")
+        self.L = _native.load()
+        self.h = self.L.distfind_session_new(int(work_cap), int(is_trials))
+        if not self.h:
+            raise RuntimeError(f"session: {_native.last_error()}")
+
+    def add(self, problems):
+        items, keep = _batch_items(problems)
+        ids = (ctypes.c_int64 * max(1, len(problems)))()
+        _check(self.L.distfind_session_add(self.h, items, len(problems), ids), "session.add")
+        del keep
+        return [ids[i] for i in range(len(problems))]
+
+    def step(self) -> int:
+        a = ctypes.c_int(0)
+        _check(self.L.distfind_session_step(self.h, ctypes.byref(a)), "session.step")
+        return a.value
+
+    def poll(self):
+        out = []
+        while True:
+            ids = (ctypes.c_int64 * 64)()
+            res = (_native.DistfindResult * 64)()
+            n = self.L.distfind_session_poll(self.h, ids, res, 64)
+            if n < 0:
+                raise RuntimeError(f"session.poll: {_native.last_error()}")
+            out += [(ids[i], res[i]) for i in range(n)]
+            if n < 64:
+                return out
+
+    @property
+    def active(self) -> int:
+        return self.L.distfind_session_active(self.h)
+
+    def stats(self) -> dict:
+        s = _native.DistfindBatchStats()
+        self.L.distfind_session_stats(self.h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def close(self):
+        if self.h:
+            self.L.distfind_session_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def last_batch_stats() -> dict:

@@ -1,11 +1,11 @@
 """Serving loop over the batched engine: mixed distance requests arrive as a Poisson
-process, the server admits what has arrived (up to a device-memory budget and a batch
-size), solves it in one batched call, and records each request's latency.
+process, the server admits what has arrived (within a device-memory budget) into a
+continuous batch, and records each request's latency.
 
-The arrival clock is virtual and the service is real: arrivals are drawn in advance; when
-the server is idle it jumps the clock to the next arrival, takes every request that has
-arrived by then, runs the batch on the device and advances the clock by the batch's
-measured wall time. Latencies and device load are thus those of the real engine under that
+The arrival clock is virtual and the service is real: arrivals are drawn in advance; the
+server admits arrived requests into a continuous batch (distfind.Session), runs one round
+at a time on the device and advances the clock by each admission's and round's measured
+wall time; when idle it jumps the clock to the next arrival. Latencies and device load are thus those of the real engine under that
 arrival process, reproducible from the seed. The rate is tuned so that the device is busy
 for the target fraction of the time (busy = sum of batch times / makespan).
 
@@ -47,45 +47,68 @@ def device_bytes(req):
     return sets * n * (st * 8 + 8) + 64 * 1024
 
 
-def serve(requests, rate, seed=1, max_batch=64, mem_budget=1 << 30):
-    """Run the loop; returns per-request latencies and load statistics."""
+def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30):
+    """Run the loop with continuous batching; returns latency and load statistics.
+
+    Between rounds the server admits every request that has arrived, in arrival order,
+    while the device memory of the admitted solves stays within `mem_budget` (a request
+    that alone exceeds it is admitted only onto an idle device) and at most `max_active`
+    are in flight. Each round enumerates one level of every solve in flight; a solve
+    leaves as soon as it finishes, so small requests do not wait for large ones."""
+    from collections import deque
+
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
-    clock = 0.0
-    busy = 0.0
+    sess = df.Session()
+    clock = busy = 0.0
     nxt = 0
-    queue = []
+    queue = deque()
+    mem = 0
+    owner = {}                       # session id -> (request index, bytes)
     lat = np.zeros(len(requests))
     proven = np.zeros(len(requests), bool)
-    batches = []
-    while nxt < len(requests) or queue:
-        if not queue:
-            clock = max(clock, arrivals[nxt])
+    done = rounds = admitted_batches = 0
+    while done < len(requests):
         while nxt < len(requests) and arrivals[nxt] <= clock:
             queue.append(nxt)
             nxt += 1
-        take, mem = [], 0
-        for r in queue:   # admission: first come, within the batch size and memory budget
-            b = device_bytes(requests[r])
-            if take and (len(take) >= max_batch or mem + b > mem_budget):
+        take = []
+        while queue and len(owner) + len(take) < max_active:
+            b = device_bytes(requests[queue[0]])
+            if mem + b > mem_budget and (owner or take):
                 break
+            r = queue.popleft()
             take.append(r)
             mem += b
-        queue = queue[len(take):]
-        t0 = time.perf_counter()
-        res = df.batch_distance([{"problem": requests[r]["problem"], "work_cap": requests[r]["work_cap"]} for r in take])
-        dt = time.perf_counter() - t0
-        clock += dt
-        busy += dt
-        batches.append((len(take), dt))
-        for r, x in zip(take, res):
+        if take:
+            t0 = time.perf_counter()
+            ids = sess.add([{"problem": requests[r]["problem"], "work_cap": requests[r]["work_cap"]} for r in take])
+            dt = time.perf_counter() - t0
+            clock += dt
+            busy += dt
+            admitted_batches += 1
+            for i, r in zip(ids, take):
+                owner[i] = (r, device_bytes(requests[r]))
+        if sess.active > 0:
+            t0 = time.perf_counter()
+            sess.step()
+            dt = time.perf_counter() - t0
+            clock += dt
+            busy += dt
+            rounds += 1
+        for i, res in sess.poll():
+            r, b = owner.pop(i)
+            mem -= b
             lat[r] = clock - arrivals[r]
-            proven[r] = bool(x.proven)
+            proven[r] = bool(res.proven)
+            done += 1
+        if not owner and not queue and nxt < len(requests):
+            clock = max(clock, arrivals[nxt])   # idle until the next arrival
+    sess.close()
     makespan = clock - arrivals[0] if len(requests) else 0.0
     return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
                 mean_ms=float(lat.mean() * 1e3), throughput_rps=len(requests) / max(makespan, 1e-9),
-                busy=busy / max(makespan, 1e-9), batches=len(batches),
-                mean_batch=float(np.mean([b for b, _ in batches])) if batches else 0.0,
+                busy=busy / max(makespan, 1e-9), rounds=rounds, admissions=admitted_batches,
                 proven=int(proven.sum()), requests=len(requests), rate_rps=rate)
 
 

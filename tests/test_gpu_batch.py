@@ -96,3 +96,26 @@ def test_empty_and_degenerate(hip):
     full = np.eye(6, dtype=np.uint8)              # k = 0: no codewords
     r = df.batch_distance([("classical", full), ("classical", codes.repetition_parity(5))])
     assert r[0].distance == -1 and r[1].distance == 5
+
+
+def test_session_matches_batch(hip):
+    """Continuous batching: problems added at different rounds finish with the same
+    results as a one-shot batch."""
+    from qubitserf_amd import distfind as df
+    probs = _mixed_problems()
+    want = [(r.distance, r.lower_bound, r.proven, r.levels) for r in df.batch_distance(probs)]
+    s = df.Session()
+    ids = s.add(probs[:5])
+    got = {}
+    rounds = 0
+    pending = list(range(5, len(probs)))
+    while s.active or pending:
+        if pending and rounds % 2 == 1:
+            ids += s.add([probs[pending.pop(0)]])
+        if s.active:
+            s.step()
+        rounds += 1
+        for i, r in s.poll():
+            got[i] = (r.distance, r.lower_bound, r.proven, r.levels)
+    assert [got[i] for i in ids] == want
+    s.close()
