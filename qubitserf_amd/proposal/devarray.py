@@ -1,5 +1,6 @@
 """Minimal device arrays for the scorer, on the library's own HIP runtime (qsf_dev_*):
-float32 or bfloat16 (stored as uint16 bit patterns), row-major, contiguous."""
+float32 or bfloat16 (stored as uint16 bit patterns), row-major, contiguous. Blocks come
+from a size-class pool, since a forward pass makes a few dozen short-lived tensors."""
 from __future__ import annotations
 
 import ctypes
@@ -39,6 +40,22 @@ def bf16_to_f32(b) -> np.ndarray:
     return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
+_POOL: dict = {}          # rounded byte size -> free device blocks (reused, never freed)
+
+
+def _take(nbytes):
+    size = 256
+    while size < nbytes:
+        size <<= 1
+    free = _POOL.setdefault(size, [])
+    if free:
+        return free.pop(), size
+    p = _lib().qsf_dev_alloc(size)
+    if not p:
+        raise MemoryError(f"device allocation of {size} bytes failed")
+    return p, size
+
+
 class DevArray:
     def __init__(self, shape, dtype=F32, zero=True):
         self.shape = tuple(int(s) for s in shape)
@@ -47,9 +64,7 @@ class DevArray:
         self.itemsize = 4 if dtype == F32 else 2
         self.nbytes = self.numel * self.itemsize
         self.L = _lib()
-        self.ptr = self.L.qsf_dev_alloc(max(1, self.nbytes))
-        if not self.ptr:
-            raise MemoryError(f"device allocation of {self.nbytes} bytes failed")
+        self.ptr, self._block = _take(max(1, self.nbytes))
         if zero and self.nbytes:
             self._ok(self.L.qsf_dev_zero(self.ptr, self.nbytes), "qsf_dev_zero")
 
@@ -83,6 +98,6 @@ class DevArray:
 
     def __del__(self):
         p = getattr(self, "ptr", None)
-        if p:
-            self.L.qsf_dev_free(p)
+        if p:   # back to the pool: kernels queued on the null stream finish before any reuse
+            _POOL.setdefault(self._block, []).append(p)
             self.ptr = None
