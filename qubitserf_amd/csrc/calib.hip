@@ -6,6 +6,11 @@
 //                             achieved int32 lane-ops/s.
 //   qsa_calib copy <MiB>      a plain device copy of <MiB> MiB (one read + one write of each
 //                             byte); used to scale the FETCH_SIZE / WRITE_SIZE counters.
+//   qsa_calib mfma            int8 matrix-core peak: each wave chains 8 independent
+//                             v_mfma_i32_32x32x32_i8 accumulators; prints int8 ops/s
+//                             (2 per multiply-add).
+//   qsa_calib mfma_max        the int8 hot-test pattern: per step 8 products (zero
+//                             accumulator) reduced with v_max3; prints products/s.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -35,6 +40,54 @@ __global__ __launch_bounds__(256) void valu_chain(unsigned* out, unsigned key, i
         x7 = __popc(x7 ^ key) + x7;
     }
     out[t] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7;
+}
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(64) void mfma_chain(int* out, int iters) {
+    const int l = threadIdx.x;
+    v4i a[8], b = {l, l * 3, l * 5, l * 7};
+    v16i c[8];
+    for (int k = 0; k < 8; ++k) {
+        a[k] = v4i{l + k, l ^ k, l * k, l - k};
+        c[k] = v16i{};
+    }
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[k], b, c[k], 0, 0, 0);
+    int r = 0;
+    for (int k = 0; k < 8; ++k)
+        for (int v = 0; v < 16; ++v) r ^= c[k][v];
+    out[blockIdx.x * 64 + l] = r;
+}
+
+__global__ __launch_bounds__(64) void mfma_max(int* out, int iters) {
+    const int l = threadIdx.x;
+    v4i a[8], b = {l, l * 3, l * 5, l * 7};
+    for (int k = 0; k < 8; ++k) a[k] = v4i{l + k, l ^ k, l * k, l - k};
+    int hit = 0;
+    for (int i = 0; i < iters; ++i) {
+        b[0] += 1;
+        int mx = -(1 << 30);
+        const v16i z = {};
+#pragma unroll
+        for (int k0 = 0; k0 < 8; k0 += 4) {
+            const v16i p0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[k0 + 0], b, z, 0, 0, 0);
+            const v16i p1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[k0 + 1], b, z, 0, 0, 0);
+            const v16i p2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[k0 + 2], b, z, 0, 0, 0);
+            const v16i p3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[k0 + 3], b, z, 0, 0, 0);
+            auto red = [](const v16i& c) {
+                int m = max(max(c[0], c[1]), c[2]);
+#pragma unroll
+                for (int v = 3; v < 15; v += 2) m = max(max(m, c[v]), c[v + 1]);
+                return max(m, c[15]);
+            };
+            mx = max(mx, max(max(red(p0), red(p1)), max(red(p2), red(p3))));
+        }
+        hit += __ballot(mx > (1 << 29)) != 0ull;
+    }
+    out[blockIdx.x * 64 + l] = hit;
 }
 
 __global__ __launch_bounds__(256) void copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
@@ -86,6 +139,33 @@ int main(int argc, char** argv) {
                     2.0 * bytes / (ms * 1e-3) / 1e9);
         CHECK(hipFree(a));
         CHECK(hipFree(b));
+        return 0;
+    }
+    if (std::strcmp(argv[1], "mfma") == 0 || std::strcmp(argv[1], "mfma_max") == 0) {
+        const bool chain = std::strcmp(argv[1], "mfma") == 0;
+        const int blocks = 256 * 12, iters = chain ? 4000 : 2000;
+        int* out;
+        CHECK(hipMalloc(&out, (size_t)blocks * 64 * sizeof(int)));
+        auto launch = [&](int it) {
+            if (chain) hipLaunchKernelGGL(mfma_chain, dim3(blocks), dim3(64), 0, 0, out, it);
+            else hipLaunchKernelGGL(mfma_max, dim3(blocks), dim3(64), 0, 0, out, it);
+        };
+        launch(10);
+        CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(e0, 0));
+        launch(iters);
+        CHECK(hipEventRecord(e1, 0));
+        CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double mfmas = (double)blocks * iters * 8;
+        if (chain)
+            std::printf("{\"probe\": \"mfma\", \"int8_ops\": %.6e, \"ms\": %.3f, \"tops\": %.1f}\n", mfmas * 65536, ms,
+                        mfmas * 65536 / (ms * 1e-3) / 1e12);
+        else
+            std::printf("{\"probe\": \"mfma_max\", \"products\": %.6e, \"ms\": %.3f, \"products_per_s\": %.4e}\n",
+                        mfmas * 1024, ms, mfmas * 1024 / (ms * 1e-3));
+        CHECK(hipFree(out));
         return 0;
     }
     std::fprintf(stderr, "unknown probe\n");

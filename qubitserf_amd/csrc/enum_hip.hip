@@ -360,6 +360,296 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
 }
 
 // =====================================================================================
+// bz_mfma: the hot test as int8 matrix products (min mode)
+// =====================================================================================
+//
+// The weight of a codeword on its flagged rows and on 30 of its residual columns is a lower
+// bound on its weight, and it is a dot product: for residual bits a (lane element) and b
+// (stream element), hd(a, b) = |a| + sum_i (1 - 2 a_i) b_i. With the lane side as +1 / -1
+// bytes plus the two dimensions (1, off_lane + |a|) and the stream side as 0 / 1 bytes plus
+// (off_stream, 1), one v_mfma_i32_32x32x32_i8 yields w30 = off_lane + off_stream + hd for a
+// 32 x 32 block of codewords. Residual bit j + 8 i (j < 8, i < 4) sits in byte i of dword j
+// of a codeword's 32-byte vector, so a stream element's bytes are (w >> j) & 0x01010101;
+// bits 30 and 31 (dword 6 and 7, byte 3) carry the two extra dimensions. Any byte order
+// shared by both operands gives the same products, so the instruction's lane layout does
+// not enter.
+//
+// A workgroup is four waves. Each stream chunk of kMfmaStream elements is made once in LDS
+// (two elements per thread) and tested by all four waves, each against its own 256 lane
+// elements held as eight A operands (lane l: row l % 32, the 16 dimensions of half l / 32).
+// Per group of 32 stream elements a wave reads its B operand once, issues eight products,
+// and reduces them with v_min3; only a group whose minimum is below the bound is tested
+// exactly (XD residual dwords, then the full residual and syndrome).
+template <int XD>
+__global__ __launch_bounds__(256) void bz_mfma(TiledArgs a) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef int v16i __attribute__((ext_vector_type(16)));
+    __shared__ u32 rows_fast[kMaxTiledK * XD];
+    __shared__ SElem<XD> stream[kMfmaStream];
+    __shared__ v4i sbytes[2 * kMfmaStream];                  // half h of element e at [h * kMfmaStream + e]
+    __shared__ u64 sbinom[64 * (kMfmaSliceK + 1)];            // C(c, i), c < sm, i <= sk of the stream list
+    __shared__ u64 lmk[kMfmaWaves * 256];                     // each wave's lane-element subsets
+    __shared__ unsigned short lay[64 * 16];                   // product layout: (row, column) of lane l, value v
+    __shared__ unsigned long long s_tile;
+    extern __shared__ u32 rows_full[];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, row = lane & 31;
+    if (wave == 0) {
+        // The (row, column) that value v of lane l holds, read off two products whose
+        // entries are their own row and column index (row index x ones, ones x column
+        // index, both on dimension 0), so the exact tests need no assumed layout.
+        const v4i idx = v4i{half == 0 ? row : 0, 0, 0, 0};
+        const v4i one = v4i{half == 0 ? 1 : 0, 0, 0, 0};
+        const v16i zero = {};
+        const v16i pr = __builtin_amdgcn_mfma_i32_32x32x32_i8(idx, one, zero, 0, 0, 0);
+        const v16i pc = __builtin_amdgcn_mfma_i32_32x32x32_i8(one, idx, zero, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) lay[lane * 16 + v] = (unsigned short)((pr[v] & 31) | ((pc[v] & 31) << 8));
+    }
+    int cur_job = -1, cur_g = -1;
+    int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
+    const TJob* J = nullptr;
+    unsigned slow_entries = 0, medium_entries = 0;
+
+    for (;;) {
+        __syncthreads();   // every wave is done with the previous tile's LDS
+        if (tid == 0) s_tile = atomicAdd(a.next_tile, 1ull);
+        __syncthreads();
+        const u64 t = uni64(s_tile);
+        if (t >= a.total_tiles) break;
+        int lo = 0, hi = a.nrects - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (a.rects[mid].tile_base <= t) lo = mid; else hi = mid - 1;
+        }
+        const Rect& Rr = a.rects[uni(lo)];
+        const int job = uni(Rr.job);
+        if (job != cur_job) {
+            J = a.jobs + job;
+            K = uni(J->K); RW = uni(J->RW); RD = uni(J->RD); SD = uni(J->SD); floor = uni(J->floor);
+            bound = uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            cur_job = job;
+            cur_g = -1;
+        }
+        bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+        if (__syncthreads_or(bound <= floor)) {
+            if (a.njobs == 1) break;
+            continue;
+        }
+        const u64 tile_base = uni64(Rr.tile_base), stream_groups = uni64(Rr.stream_groups);
+        const u64 stream_chunks = uni64(Rr.stream_chunks), n_lane = uni64(Rr.n_lane), n_stream = uni64(Rr.n_stream);
+        const int fixed = uni(Rr.fixed);
+        const u64 local = t - tile_base;
+        const u64 bx = local / stream_groups;
+        const u64 bg = local - bx * stream_groups;
+        const int g = uni(Rr.g);
+        const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
+        const int sk = uni(Rr.sk), sbase = uni(Rr.sbase), sm = uni(Rr.sm);
+        const u64 flo = J->flags[2 * g], fhi = J->flags[2 * g + 1];
+        const u64 lflags = range_bits(flo, fhi, lbase, lm);
+        const u64 sflags = range_bits(flo, fhi, sbase, sm);
+        const u32* fx = fixed >= 0 ? J->fixed + (size_t)fixed * (RW + 1) : nullptr;
+        u32 fw[XD];
+#pragma unroll
+        for (int f = 0; f < XD; ++f) fw[f] = (fx && f < RD) ? fx[f] : 0u;
+        const int foff = fx ? (int)fx[RW] : 0;
+        const u32* gtbl = J->tbl + (size_t)g * K * RW;
+        const bool full_in_lds = K * RW <= a.lds_full;
+        const u32* slow_tbl = full_in_lds ? rows_full : gtbl;
+        if (g != cur_g) {   // same tile in every wave: the branch is workgroup-uniform
+            for (int i = tid; i < K * XD; i += 256) rows_fast[i] = (i % XD) < RD ? gtbl[(i / XD) * RW + (i % XD)] : 0u;
+            if (full_in_lds)
+                for (int i = tid; i < K * RW; i += 256) rows_full[i] = gtbl[i];
+            cur_g = g;
+        }
+        // unranks of stream positions read a slice of the binomial table from LDS
+        const bool slice = sk <= kMfmaSliceK;
+        const u64* sbin = slice ? sbinom : a.binom;
+        const int sbk = slice ? kMfmaSliceK + 1 : a.bk;
+        if (slice)
+            for (int i = tid; i < sm * (kMfmaSliceK + 1); i += 256) {
+                const int c = i / (kMfmaSliceK + 1), k = i - c * (kMfmaSliceK + 1);
+                sbinom[i] = k <= sk ? a.binom[c * a.bk + k] : 0ull;
+            }
+        __syncthreads();
+
+        // ---- this wave's 256 lane elements: row r of block b is element
+        // bx * kMfmaLane + 256 wave + 8 r + b (lanes l and l + 32 share a row)
+        v4i av[kMfmaBlocks];
+        u64* wlmk = lmk + 256 * wave;
+        const u64 first = bx * (u64)kMfmaLane + (u64)(256 * wave + kMfmaBlocks * row);
+        u64 mnext = (first < n_lane && lk) ? colex_mask(first, lk, lm, a.binom, a.bk) : 0ull;
+#pragma unroll
+        for (int b = 0; b < kMfmaBlocks; ++b) {
+            const u64 idx = first + (u64)b;
+            const bool valid = idx < n_lane;
+            const u64 m = valid ? mnext : 0ull;
+            if (valid && lk && idx + 1 < n_lane) mnext = colex_next(mnext);
+            if (half == 0) wlmk[kMfmaBlocks * row + b] = m;
+            u32 x0 = 0;
+            for (u64 mm = m; mm; mm &= mm - 1) x0 ^= rows_fast[(lbase + __builtin_ctzll(mm)) * XD];
+            const int xoff = __popcll(m & lflags);
+            u32 q[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {   // bytes 0 / 1 -> +1 / -1 (0xFE * x = (x << 8) - (x << 1))
+                const u32 x = (x0 >> (4 * half + i)) & 0x01010101u;
+                q[i] = 0x01010101u ^ ((x << 8) - (x << 1));
+            }
+            if (half == 1) {   // dimension 27: 1, dimension 31: off_lane + |a|
+                const int w = min(xoff + __popc(x0 & 0x3fffffffu), 127);
+                q[2] = (q[2] & 0x00ffffffu) | (1u << 24);
+                q[3] = (q[3] & 0x00ffffffu) | ((u32)(valid ? w : 127) << 24);
+            }
+            if (!valid) {
+                q[0] = q[1] = 0u;
+                if (half == 0) q[2] = q[3] = 0u; else { q[2] = 1u << 24; q[3] = 127u << 24; }
+            }
+            av[b] = v4i{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+        }
+
+        const u64 ch0 = bg * (u64)a.G;
+        const u64 ch1 = min(ch0 + (u64)a.G, stream_chunks);
+        for (u64 ch = ch0; ch < ch1; ++ch) {
+            if (ch != ch0) {
+                bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                if (__syncthreads_or(bound <= floor)) break;   // also: the previous chunk is consumed
+            }
+            const u64 c0 = ch * (u64)kMfmaStream;
+            const int len = (int)min((u64)kMfmaStream, n_stream - c0);
+            const int groups = (len + 31) >> 5;
+            {   // two consecutive stream elements per thread
+                const int i0 = 2 * tid;
+                if (i0 < len) {
+                    u64 m = sk ? colex_mask(c0 + (u64)i0, sk, sm, sbin, sbk) : 0ull;
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int i = i0 + e;
+                        if (i < len) {
+                            SElem<XD> el;
+#pragma unroll
+                            for (int f = 0; f < XD; ++f) el.w[f] = fw[f];
+                            for (u64 mm = m; mm; mm &= mm - 1) {
+                                const int r = __builtin_ctzll(mm);
+#pragma unroll
+                                for (int f = 0; f < XD; ++f) el.w[f] ^= rows_fast[(sbase + r) * XD + f];
+                            }
+                            el.off = foff + __popcll(m & sflags);
+                            stream[i] = el;
+                            u32 q[8];
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) q[j] = (el.w[0] >> j) & 0x01010101u;
+                            q[6] = (q[6] & 0x00ffffffu) | ((u32)min(el.off, 127) << 24);   // dimension 27: off_stream
+                            q[7] = (q[7] & 0x00ffffffu) | (1u << 24);                      // dimension 31: 1
+                            sbytes[i] = v4i{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+                            sbytes[kMfmaStream + i] = v4i{(int)q[4], (int)q[5], (int)q[6], (int)q[7]};
+                            if (e == 0 && sk && i + 1 < len) m = colex_next(m);
+                        }
+                    }
+                }
+                // padding up to a whole group: products of 127, at or above any threshold
+                for (int i = len + tid; i < groups * 32; i += 256) {
+                    sbytes[i] = v4i{0, 0, 0, 0};
+                    sbytes[kMfmaStream + i] = v4i{0, 0, (int)(127u << 24), 0};
+                }
+            }
+            __syncthreads();
+
+            int thrd = min(bound, 127);
+            // B operands one group ahead; the lane blocks in two passes of four independent
+            // products, each reduced by a chain of v_min3
+            const v4i* sb = sbytes + half * kMfmaStream + row;
+            v4i bnext = sb[0];
+            for (int gi = 0; gi < groups; ++gi) {
+                const v4i bv = bnext;
+                if (gi + 1 < groups) bnext = sb[(gi + 1) * 32];
+                const v16i zero = {};
+                auto red = [](const v16i& c) {
+                    int m = min(min(c[0], c[1]), c[2]);
+#pragma unroll
+                    for (int v = 3; v < 15; v += 2) m = min(min(m, c[v]), c[v + 1]);
+                    return min(m, c[15]);
+                };
+                int mn = 1 << 30;
+#pragma unroll
+                for (int b0 = 0; b0 < kMfmaBlocks; b0 += 4) {
+                    const v16i p0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 0], bv, zero, 0, 0, 0);
+                    const v16i p1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 1], bv, zero, 0, 0, 0);
+                    const v16i p2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 2], bv, zero, 0, 0, 0);
+                    const v16i p3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 3], bv, zero, 0, 0, 0);
+                    mn = min(mn, min(min(red(p0), red(p1)), min(red(p2), red(p3))));
+                }
+                if (__builtin_expect(__ballot(mn < thrd) == 0ull, 1)) continue;
+                // ---- exact tests of this group: the products are formed again block by block,
+                // and each lane tests the (lane element, stream element) pairs of its values
+                // below the threshold (both elements are then real: padding gives >= 127)
+                ++medium_entries;
+                int mine = QSF_WEIGHT_NONE;
+#pragma unroll
+                for (int b = 0; b < kMfmaBlocks; ++b) {
+                    const v16i p = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b], bv, zero, 0, 0, 0);
+                    if (__ballot(red(p) < thrd) == 0ull) continue;
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        if (p[v] >= thrd) continue;
+                        const u32 ij = lay[lane * 16 + v];
+                        const u64 lmask = wlmk[kMfmaBlocks * (ij & 31) + b];
+                        const int si = gi * 32 + (int)(ij >> 8);
+                        u32 ew[XD];
+#pragma unroll
+                        for (int f = 0; f < XD; ++f) ew[f] = stream[si].w[f];
+                        const int base = __popcll(lmask & lflags) + stream[si].off;
+                        int tk = base;
+                        {
+                            u32 xw[XD];
+#pragma unroll
+                            for (int f = 0; f < XD; ++f) xw[f] = 0;
+                            for (u64 mm = lmask; mm; mm &= mm - 1) {
+                                const int r = __builtin_ctzll(mm);
+#pragma unroll
+                                for (int f = 0; f < XD; ++f) xw[f] ^= rows_fast[(lbase + r) * XD + f];
+                            }
+#pragma unroll
+                            for (int f = 0; f < XD; ++f) tk += __popc(xw[f] ^ ew[f]);
+                        }
+                        if (tk >= bound || tk >= mine) continue;
+                        // exact weight and logical test of the pair
+                        ++slow_entries;
+                        const u64 smask = sk ? colex_mask(c0 + (u64)si, sk, sm, sbin, sbk) : 0ull;
+                        int w = base;
+                        for (int f = 0; f < RD && w < bound && w < mine; ++f) {
+                            u32 x = fx ? fx[f] : 0u;
+                            for (u64 mm = lmask; mm; mm &= mm - 1) x ^= slow_tbl[(lbase + __builtin_ctzll(mm)) * RW + f];
+                            for (u64 mm = smask; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
+                            w += __popc(x);
+                        }
+                        if (w >= bound || w >= mine) continue;
+                        bool logical = (SD == 0);
+                        for (int f = RD; f < RW && !logical; ++f) {
+                            u32 x = fx ? fx[f] : 0u;
+                            for (u64 mm = lmask; mm; mm &= mm - 1) x ^= slow_tbl[(lbase + __builtin_ctzll(mm)) * RW + f];
+                            for (u64 mm = smask; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
+                            logical = x != 0;
+                        }
+                        if (logical) mine = w;
+                    }
+                }
+                const int wm = uni(wave_min(mine));
+                if (wm < bound) {
+                    bound = wm;
+                    if (lane == 0) atomicMin(J->best, wm);
+                    thrd = min(bound, 127);
+                }
+                if (bound <= floor) break;
+            }
+        }
+    }
+    if (lane == 0 && slow_entries) atomicAdd(a.slow_hits, (unsigned long long)slow_entries);
+    if (lane == 0 && medium_entries) atomicAdd(a.slow_hits + 1, (unsigned long long)medium_entries);
+}
+
+// =====================================================================================
 // bz_rowsweep: full rows, lexicographic runs (small levels, K > 128, collect mode)
 // =====================================================================================
 
@@ -513,7 +803,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
-    hipEvent_t gev[2 * 8] = {};          // start/stop per kernel-variant launch of a tiled run
+    hipEvent_t gev[2 * 11] = {};         // start/stop per kernel-variant launch of a tiled run
     DevBuf<u64> gbinom;                  // (kMaxTiledK + 1) x (kMaxD + 1) saturating C(n, k) for the tiled kernel
     DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
     unsigned char* pin = nullptr;        // its pinned host staging
@@ -970,14 +1260,15 @@ struct Engine {
         for (long k = 0; k < thr; ++k) { cdf += pmf; pmf = pmf * (bits - k) / (k + 1); }
         return cdf;
     }
-    static Variant choose_variant(const Solve& s, const qsf_enum_plan& p) {
+    // fd == 0 marks the int8-MFMA hot test (bz_mfma<xd>; rpl = 4 gives its 256-element lane tile)
+    static Variant choose_variant(const Solve& s, const qsf_enum_plan& p, bool allow_mfma = true) {
         static const Variant table[] = {{1, 1, 32}, {1, 2, 16}, {2, 2, 16}, {2, 4, 12}, {3, 4, 12}, {4, 4, 12},
-                                        {1, 2, 24}, {1, 2, 32}};
+                                        {1, 2, 24}, {1, 2, 32}, {0, 1, 4},  {0, 2, 4},  {0, 4, 4}};
         if (const char* env = std::getenv("QSA_TILED_VARIANT")) {   // tuning: "fd,xd,rpl"
             int f = 0, x = 0, r = 0;
             if (std::sscanf(env, "%d,%d,%d", &f, &x, &r) == 3)
                 for (const Variant& v : table)
-                    if (v.fd == f && v.xd == x && v.rpl == r && (x <= s.RD || x == 4)) return v;
+                    if (v.fd == f && v.xd == x && v.rpl == r && (x <= s.RD || x == 4) && (f > 0 || allow_mfma)) return v;
         }
         if (const char* env = std::getenv("QSA_TILED_FD")) {   // tuning/testing override
             int f = std::atoi(env);
@@ -985,6 +1276,16 @@ struct Engine {
             if (f == 2) return s.RD == 2 ? table[2] : table[3];
             if (f == 3) return table[4];
             if (f == 4) return table[5];
+        }
+        if (allow_mfma && !std::getenv("QSA_NO_MFMA")) {
+            // 30 residual bits + the flagged counts per codeword; a 32 x 256 group of codewords
+            // on fair-coin bits should reach the exact tests less than 2% of the time
+            int min_off = p.d;
+            for (int g = 0; g < p.num_gamma; ++g) min_off = std::min(min_off, std::max(0, p.d - (p.K - s.nflag[g])));
+            const long thr = (long)p.current_best - min_off;
+            const int res = std::max(1, s.maxres);
+            if (8192.0 * binom_cdf_below(std::min(30, res), thr) < 0.02)
+                return s.RD >= 3 ? table[10] : s.RD == 2 ? table[9] : table[8];
         }
         if (s.RD == 1) return table[0];
         int min_off = p.d;
@@ -1097,24 +1398,26 @@ struct Engine {
     // one persistent launch per variant (tiles pulled from the group's counter), one
     // read-back of counters and bounds. res[i] = the bound job i ended with.
     void run_tiled(std::vector<TItem>& items, bool collect, int keep, u64 cap, std::vector<int>& res) {
-        static const Variant vtab[8] = {{1, 1, 32}, {1, 2, 16}, {1, 2, 24}, {1, 2, 32},
-                                        {2, 2, 16}, {2, 4, 12}, {3, 4, 12}, {4, 4, 12}};
+        constexpr int NV = 11;
+        static const Variant vtab[NV] = {{1, 1, 32}, {1, 2, 16}, {1, 2, 24}, {1, 2, 32}, {2, 2, 16}, {2, 4, 12},
+                                         {3, 4, 12}, {4, 4, 12}, {0, 1, 4},  {0, 2, 4},  {0, 4, 4}};
         auto vidx = [&](const Variant& v) {
-            for (int k = 0; k < 8; ++k)
+            for (int k = 0; k < NV; ++k)
                 if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl) return k;
             return 7;
         };
         const int ni = (int)items.size();
-        std::vector<std::vector<Rect>> grect(8);
+        std::vector<std::vector<Rect>> grect(NV);
         std::vector<u32> hfixed;
         std::vector<size_t> fixed_off(ni, 0);
-        std::vector<int> glds(8, 0);
+        std::vector<int> glds(NV, 0);
         for (int i = 0; i < ni; ++i) {
             TItem& it = items[i];
             const Solve& s = *it.s;
             const int K = it.p.K, d = it.p.d, h = s.h;
             const int vi = vidx(it.V);
-            RectBuilder B{s, 0, s.RW, kStreamMax, 64ull * it.V.rpl, {}, {}};
+            RectBuilder B{s, 0, s.RW, it.V.fd == 0 ? kMfmaStream : kStreamMax,
+                          it.V.fd == 0 ? (u64)kMfmaLane : 64ull * it.V.rpl, {}, {}};
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
                 for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
@@ -1127,15 +1430,18 @@ struct Engine {
             if (K * s.RW <= kLdsFullRows) glds[vi] = std::max(glds[vi], K * s.RW);
         }
         // tiles: a lane block x G stream chunks; G large enough to amortise the register
-        // tile, small enough to leave >= 8 tiles per resident wave for load balance
+        // tile, small enough to leave >= 8 tiles per resident workgroup for load balance
+        // (16 one-wave workgroups per CU for bz_tiled, 4 four-wave ones for bz_mfma)
         const u64 waves = (u64)std::max(1, cus) * 16;
-        std::vector<int> gG(8, 1);
-        std::vector<u64> gtiles(8, 0);
+        const u64 mfma_groups = (u64)std::max(1, cus) * 4;
+        std::vector<int> gG(NV, 1);
+        std::vector<u64> gtiles(NV, 0);
         size_t nrect = 0;
-        for (int v = 0; v < 8; ++v) {
+        for (int v = 0; v < NV; ++v) {
             u64 chunks = 0;
             for (const Rect& r : grect[v]) chunks += r.lane_blocks * r.stream_chunks;
-            gG[v] = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * waves)));
+            const u64 resident = vtab[v].fd == 0 ? mfma_groups : waves;
+            gG[v] = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * resident)));
             u64 tiles = 0;
             for (Rect& r : grect[v]) {
                 r.stream_groups = (r.stream_chunks + gG[v] - 1) / gG[v];
@@ -1145,7 +1451,7 @@ struct Engine {
             gtiles[v] = tiles;
             nrect += grect[v].size();
         }
-        const size_t o_cnt = 0, o_best = 8 * 4 * sizeof(u64);
+        const size_t o_cnt = 0, o_best = NV * 4 * sizeof(u64);
         const size_t o_jobs = align16(o_best + (size_t)ni * sizeof(int));
         const size_t o_rects = align16(o_jobs + (size_t)ni * sizeof(TJob));
         const size_t o_fixed = align16(o_rects + nrect * sizeof(Rect));
@@ -1170,10 +1476,10 @@ struct Engine {
             J.floor = it.floor;
             hj[i] = J;
         }
-        std::vector<size_t> grect_off(8, 0);
+        std::vector<size_t> grect_off(NV, 0);
         {
             size_t at = o_rects;
-            for (int v = 0; v < 8; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 grect_off[v] = at;
                 if (!grect[v].empty()) std::memcpy(pin + at, grect[v].data(), grect[v].size() * sizeof(Rect));
                 at += grect[v].size() * sizeof(Rect);
@@ -1185,8 +1491,8 @@ struct Engine {
         auto* cnt = reinterpret_cast<unsigned long long*>(dev + o_cnt);
         stats.kernel = 2;
         stats.tiles = 0;
-        int used[8] = {0};
-        for (int v = 0; v < 8; ++v) {
+        int used[NV] = {0};
+        for (int v = 0; v < NV; ++v) {
             if (gtiles[v] == 0) continue;
             used[v] = 1;
             const Variant& V = vtab[v];
@@ -1198,7 +1504,7 @@ struct Engine {
             a.total_tiles = gtiles[v];
             a.binom = gbinom.p;
             a.bk = kMaxD + 1;
-            a.S = kStreamMax;
+            a.S = V.fd == 0 ? kMfmaStream : kStreamMax;
             a.G = gG[v];
             a.lds_full = glds[v];
             a.next_tile = cnt + 4 * v;
@@ -1208,8 +1514,8 @@ struct Engine {
             a.out_rows = cnt + 4 * v + 3;
             a.out = out.p;
             a.out_cap = cap;
-            const u64 grid = std::min<u64>(gtiles[v], waves);
-            const dim3 gd((unsigned)grid), bd(64);
+            const u64 grid = std::min<u64>(gtiles[v], V.fd == 0 ? mfma_groups : waves);
+            const dim3 gd((unsigned)grid), bd(V.fd == 0 ? 64 * kMfmaWaves : 64);
             const size_t lds = (size_t)glds[v] * sizeof(u32);
             HIPCHK(hipEventRecord(gev[2 * v], stream));
             switch (v) {
@@ -1220,7 +1526,10 @@ struct Engine {
                 case 4: hipLaunchKernelGGL((bz_tiled<2, 2, 16>), gd, bd, lds, stream, a); break;
                 case 5: hipLaunchKernelGGL((bz_tiled<2, 4, 12>), gd, bd, lds, stream, a); break;
                 case 6: hipLaunchKernelGGL((bz_tiled<3, 4, 12>), gd, bd, lds, stream, a); break;
-                default: hipLaunchKernelGGL((bz_tiled<4, 4, 12>), gd, bd, lds, stream, a); break;
+                case 7: hipLaunchKernelGGL((bz_tiled<4, 4, 12>), gd, bd, lds, stream, a); break;
+                case 8: hipLaunchKernelGGL((bz_mfma<1>), gd, bd, lds, stream, a); break;
+                case 9: hipLaunchKernelGGL((bz_mfma<2>), gd, bd, lds, stream, a); break;
+                default: hipLaunchKernelGGL((bz_mfma<4>), gd, bd, lds, stream, a); break;
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(gev[2 * v + 1], stream));
@@ -1228,7 +1537,9 @@ struct Engine {
             stats.fd = V.fd;
             stats.xd = V.xd;
             stats.rpl = V.rpl;
-            stats.ops_per_codeword = (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
+            // VALU lane-ops per codeword of the hot loop; the int8-MFMA test spends 0.5
+            // (v_max3 over its products) besides 32 int8 multiply-adds on the matrix cores
+            stats.ops_per_codeword = V.fd == 0 ? 0.5 : (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
         }
         HIPCHK(hipMemcpyAsync(pin, dev, o_jobs, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -1237,7 +1548,7 @@ struct Engine {
         double ms_total = 0;
         stats.slow_hits = stats.medium_hits = 0;
         out_rows_last = 0;
-        for (int v = 0; v < 8; ++v) {
+        for (int v = 0; v < NV; ++v) {
             if (!used[v]) continue;
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, gev[2 * v], gev[2 * v + 1]));
@@ -1293,7 +1604,7 @@ struct Engine {
             if (tiled) {
                 qsf_enum_plan pv = p;
                 pv.current_best = keep + 1;
-                std::vector<TItem> items{TItem{&s, p, -1, choose_variant(s, pv)}};
+                std::vector<TItem> items{TItem{&s, p, -1, choose_variant(s, pv, false)}};
                 std::vector<int> res;
                 run_tiled(items, true, keep, cap, res);
                 got = out_rows_last;

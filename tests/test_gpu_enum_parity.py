@@ -170,3 +170,30 @@ def test_tiled_large_levels_with_groups(hip):
         want = O.enumerate_plan(gamma, stride * 64, d, check=check, current_best=best)
         assert got == want, (K, d, got, want, st)
         assert st["kernel"] == 2
+
+
+@pytest.mark.parametrize("xd", [1, 2, 4])
+def test_mfma_hot_test(hip, xd, monkeypatch):
+    """The int8-MFMA hot test (bz_mfma) with each medium-test width, on random and
+    systematic generators, loose and tight bounds, against the oracle."""
+    monkeypatch.setenv("QSA_TILED_VARIANT", f"0,{xd},4")
+    rng = np.random.default_rng(900 + xd)
+    with hip.kernel_policy("tiled"):
+        for i in range(10):
+            K = int(rng.integers(24, 121)); stride = 3 if xd > 1 else int(rng.integers(1, 4))
+            d = int(rng.integers(3, 6)) if K < 90 else 4
+            kcheck = int(rng.integers(0, 4))
+            best = [NONE, 70, 45, 30, 22, 14, 9, 60, 35, 25][i]
+            run_case(hip, rng, K, stride, d, 1 + i % 2, kcheck, best, systematic=bool(i % 3))
+            assert hip.last_stats()["fd"] == 0
+
+
+def test_mfma_chosen_on_large_levels(hip):
+    """Automatic choice: a large level with a good bound runs the MFMA test; results equal
+    the oracle's and the VALU kernel's."""
+    rng = np.random.default_rng(3)
+    gamma, check = rand_plan(rng, 80, 3, 1, 2, systematic=True)
+    n = 192
+    want = O.enumerate_plan(gamma, n, 5, check=check, current_best=8)
+    got = hip.enumerate_level(gamma, n, 5, check=check, current_best=8)
+    assert hip.last_stats()["fd"] == 0 and got == want
