@@ -374,20 +374,23 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
 // shared by both operands gives the same products, so the instruction's lane layout does
 // not enter.
 //
-// A workgroup is four waves. Each stream chunk of kMfmaStream elements is made once in LDS
-// (two elements per thread) and tested by all four waves, each against its own 256 lane
-// elements held as eight A operands (lane l: row l % 32, the 16 dimensions of half l / 32).
-// Per group of 32 stream elements a wave reads its B operand once, issues eight products,
-// and reduces them with v_min3; only a group whose minimum is below the bound is tested
-// exactly (XD residual dwords, then the full residual and syndrome).
+// A workgroup is four waves; a tile is 1024 lane elements (256 per wave, held as eight A
+// operands: lane l holds row l % 32 and the 16 dimensions of half l / 32) times the stream
+// range of nch chunks of kMfmaStream elements. Thread t owns the run of 2 nch consecutive
+// stream elements starting at 2 nch t (one unrank per tile, then colex successors whose
+// first residual dword changes by the rows that enter or leave) and contributes two of
+// them to each chunk. A chunk is made once in LDS and tested by all four waves: per group
+// of 32 stream elements a wave reads its B operand once, issues eight products and reduces
+// them with v_min3. A group whose minimum is below the bound is tested exactly: the
+// products are formed again and each lane tests the pairs of its values below the bound
+// (XD residual dwords, then the full residual and syndrome).
 template <int XD>
-__global__ __launch_bounds__(256) void bz_mfma(TiledArgs a) {
+__global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
     typedef int v4i __attribute__((ext_vector_type(4)));
     typedef int v16i __attribute__((ext_vector_type(16)));
     __shared__ u32 rows_fast[kMaxTiledK * XD];
-    __shared__ SElem<XD> stream[kMfmaStream];
-    __shared__ v4i sbytes[2 * kMfmaStream];                  // half h of element e at [h * kMfmaStream + e]
-    __shared__ u64 sbinom[64 * (kMfmaSliceK + 1)];            // C(c, i), c < sm, i <= sk of the stream list
+    __shared__ v4i sbytes[2 * kMfmaStream];                  // half h of chunk position e at [h * kMfmaStream + e]
+    __shared__ u64 smk[kMfmaStream];                          // the chunk's stream-element subsets
     __shared__ u64 lmk[kMfmaWaves * 256];                     // each wave's lane-element subsets
     __shared__ unsigned short lay[64 * 16];                   // product layout: (row, column) of lane l, value v
     __shared__ unsigned long long s_tile;
@@ -451,9 +454,7 @@ __global__ __launch_bounds__(256) void bz_mfma(TiledArgs a) {
         const u64 lflags = range_bits(flo, fhi, lbase, lm);
         const u64 sflags = range_bits(flo, fhi, sbase, sm);
         const u32* fx = fixed >= 0 ? J->fixed + (size_t)fixed * (RW + 1) : nullptr;
-        u32 fw[XD];
-#pragma unroll
-        for (int f = 0; f < XD; ++f) fw[f] = (fx && f < RD) ? fx[f] : 0u;
+        const u32 fw0 = fx ? fx[0] : 0u;
         const int foff = fx ? (int)fx[RW] : 0;
         const u32* gtbl = J->tbl + (size_t)g * K * RW;
         const bool full_in_lds = K * RW <= a.lds_full;
@@ -464,94 +465,80 @@ __global__ __launch_bounds__(256) void bz_mfma(TiledArgs a) {
                 for (int i = tid; i < K * RW; i += 256) rows_full[i] = gtbl[i];
             cur_g = g;
         }
-        // unranks of stream positions read a slice of the binomial table from LDS
-        const bool slice = sk <= kMfmaSliceK;
-        const u64* sbin = slice ? sbinom : a.binom;
-        const int sbk = slice ? kMfmaSliceK + 1 : a.bk;
-        if (slice)
-            for (int i = tid; i < sm * (kMfmaSliceK + 1); i += 256) {
-                const int c = i / (kMfmaSliceK + 1), k = i - c * (kMfmaSliceK + 1);
-                sbinom[i] = k <= sk ? a.binom[c * a.bk + k] : 0ull;
-            }
         __syncthreads();
 
         // ---- this wave's 256 lane elements: row r of block b is element
         // bx * kMfmaLane + 256 wave + 8 r + b (lanes l and l + 32 share a row)
         v4i av[kMfmaBlocks];
         u64* wlmk = lmk + 256 * wave;
-        const u64 first = bx * (u64)kMfmaLane + (u64)(256 * wave + kMfmaBlocks * row);
-        u64 mnext = (first < n_lane && lk) ? colex_mask(first, lk, lm, a.binom, a.bk) : 0ull;
+        {
+            const u64 first = bx * (u64)kMfmaLane + (u64)(256 * wave + kMfmaBlocks * row);
+            u64 mnext = (first < n_lane && lk) ? colex_mask(first, lk, lm, a.binom, a.bk) : 0ull;
 #pragma unroll
-        for (int b = 0; b < kMfmaBlocks; ++b) {
-            const u64 idx = first + (u64)b;
-            const bool valid = idx < n_lane;
-            const u64 m = valid ? mnext : 0ull;
-            if (valid && lk && idx + 1 < n_lane) mnext = colex_next(mnext);
-            if (half == 0) wlmk[kMfmaBlocks * row + b] = m;
-            u32 x0 = 0;
-            for (u64 mm = m; mm; mm &= mm - 1) x0 ^= rows_fast[(lbase + __builtin_ctzll(mm)) * XD];
-            const int xoff = __popcll(m & lflags);
-            u32 q[4];
+            for (int b = 0; b < kMfmaBlocks; ++b) {
+                const u64 idx = first + (u64)b;
+                const bool valid = idx < n_lane;
+                const u64 m = valid ? mnext : 0ull;
+                if (valid && lk && idx + 1 < n_lane) mnext = colex_next(mnext);
+                if (half == 0) wlmk[kMfmaBlocks * row + b] = m;
+                u32 x0 = 0;
+                for (u64 mm = m; mm; mm &= mm - 1) x0 ^= rows_fast[(lbase + __builtin_ctzll(mm)) * XD];
+                const int xoff = __popcll(m & lflags);
+                u32 q[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {   // bytes 0 / 1 -> +1 / -1 (0xFE * x = (x << 8) - (x << 1))
-                const u32 x = (x0 >> (4 * half + i)) & 0x01010101u;
-                q[i] = 0x01010101u ^ ((x << 8) - (x << 1));
+                for (int i = 0; i < 4; ++i) {   // bytes 0 / 1 -> +1 / -1 (0xFE * x = (x << 8) - (x << 1))
+                    const u32 x = (x0 >> (4 * half + i)) & 0x01010101u;
+                    q[i] = 0x01010101u ^ ((x << 8) - (x << 1));
+                }
+                if (half == 1) {   // dimension 27: 1, dimension 31: off_lane + |a|
+                    const int w = min(xoff + __popc(x0 & 0x3fffffffu), 127);
+                    q[2] = (q[2] & 0x00ffffffu) | (1u << 24);
+                    q[3] = (q[3] & 0x00ffffffu) | ((u32)(valid ? w : 127) << 24);
+                }
+                if (!valid) {
+                    q[0] = q[1] = 0u;
+                    if (half == 0) q[2] = q[3] = 0u; else { q[2] = 1u << 24; q[3] = 127u << 24; }
+                }
+                av[b] = v4i{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
             }
-            if (half == 1) {   // dimension 27: 1, dimension 31: off_lane + |a|
-                const int w = min(xoff + __popc(x0 & 0x3fffffffu), 127);
-                q[2] = (q[2] & 0x00ffffffu) | (1u << 24);
-                q[3] = (q[3] & 0x00ffffffu) | ((u32)(valid ? w : 127) << 24);
-            }
-            if (!valid) {
-                q[0] = q[1] = 0u;
-                if (half == 0) q[2] = q[3] = 0u; else { q[2] = 1u << 24; q[3] = 127u << 24; }
-            }
-            av[b] = v4i{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
         }
 
+        // ---- this thread's stream run: elements s0 + 2 nch tid + [0, 2 nch)
         const u64 ch0 = bg * (u64)a.G;
-        const u64 ch1 = min(ch0 + (u64)a.G, stream_chunks);
-        for (u64 ch = ch0; ch < ch1; ++ch) {
-            if (ch != ch0) {
+        const int nch = (int)(min(ch0 + (u64)a.G, stream_chunks) - ch0);
+        const u64 s0 = ch0 * (u64)kMfmaStream;
+        u64 sidx = s0 + (u64)(2 * nch) * (u64)tid;
+        u64 smask = (sidx < n_stream && sk) ? colex_mask(sidx, sk, sm, a.binom, a.bk) : 0ull;
+        u32 sx0 = fw0;
+        for (u64 mm = smask; mm; mm &= mm - 1) sx0 ^= rows_fast[(sbase + __builtin_ctzll(mm)) * XD];
+
+        for (int k = 0; k < nch; ++k) {
+            if (k != 0) {
                 bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
                 if (__syncthreads_or(bound <= floor)) break;   // also: the previous chunk is consumed
             }
-            const u64 c0 = ch * (u64)kMfmaStream;
-            const int len = (int)min((u64)kMfmaStream, n_stream - c0);
-            const int groups = (len + 31) >> 5;
-            {   // two consecutive stream elements per thread
-                const int i0 = 2 * tid;
-                if (i0 < len) {
-                    u64 m = sk ? colex_mask(c0 + (u64)i0, sk, sm, sbin, sbk) : 0ull;
 #pragma unroll
-                    for (int e = 0; e < 2; ++e) {
-                        const int i = i0 + e;
-                        if (i < len) {
-                            SElem<XD> el;
+            for (int e = 0; e < 2; ++e) {   // chunk positions 2 tid, 2 tid + 1
+                const int pos = 2 * tid + e;
+                if (sidx < n_stream) {
+                    u32 q[8];
 #pragma unroll
-                            for (int f = 0; f < XD; ++f) el.w[f] = fw[f];
-                            for (u64 mm = m; mm; mm &= mm - 1) {
-                                const int r = __builtin_ctzll(mm);
-#pragma unroll
-                                for (int f = 0; f < XD; ++f) el.w[f] ^= rows_fast[(sbase + r) * XD + f];
-                            }
-                            el.off = foff + __popcll(m & sflags);
-                            stream[i] = el;
-                            u32 q[8];
-#pragma unroll
-                            for (int j = 0; j < 8; ++j) q[j] = (el.w[0] >> j) & 0x01010101u;
-                            q[6] = (q[6] & 0x00ffffffu) | ((u32)min(el.off, 127) << 24);   // dimension 27: off_stream
-                            q[7] = (q[7] & 0x00ffffffu) | (1u << 24);                      // dimension 31: 1
-                            sbytes[i] = v4i{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
-                            sbytes[kMfmaStream + i] = v4i{(int)q[4], (int)q[5], (int)q[6], (int)q[7]};
-                            if (e == 0 && sk && i + 1 < len) m = colex_next(m);
-                        }
+                    for (int j = 0; j < 8; ++j) q[j] = (sx0 >> j) & 0x01010101u;
+                    const int off = foff + __popcll(smask & sflags);
+                    q[6] = (q[6] & 0x00ffffffu) | ((u32)min(off, 127) << 24);   // dimension 27: off_stream
+                    q[7] = (q[7] & 0x00ffffffu) | (1u << 24);                  // dimension 31: 1
+                    sbytes[pos] = v4i{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+                    sbytes[kMfmaStream + pos] = v4i{(int)q[4], (int)q[5], (int)q[6], (int)q[7]};
+                    smk[pos] = smask;
+                    if (sk && sidx + 1 < n_stream) {   // colex successor; XOR the rows that change
+                        const u64 nm = colex_next(smask);
+                        for (u64 mm = nm ^ smask; mm; mm &= mm - 1) sx0 ^= rows_fast[(sbase + __builtin_ctzll(mm)) * XD];
+                        smask = nm;
                     }
-                }
-                // padding up to a whole group: products of 127, at or above any threshold
-                for (int i = len + tid; i < groups * 32; i += 256) {
-                    sbytes[i] = v4i{0, 0, 0, 0};
-                    sbytes[kMfmaStream + i] = v4i{0, 0, (int)(127u << 24), 0};
+                    ++sidx;
+                } else {   // past the list: products of 127, at or above any threshold
+                    sbytes[pos] = v4i{0, 0, 0, 0};
+                    sbytes[kMfmaStream + pos] = v4i{0, 0, (int)(127u << 24), 0};
                 }
             }
             __syncthreads();
@@ -561,9 +548,10 @@ __global__ __launch_bounds__(256) void bz_mfma(TiledArgs a) {
             // products, each reduced by a chain of v_min3
             const v4i* sb = sbytes + half * kMfmaStream + row;
             v4i bnext = sb[0];
-            for (int gi = 0; gi < groups; ++gi) {
+            constexpr int kGroups = kMfmaStream / 32;
+            for (int gi = 0; gi < kGroups; ++gi) {
                 const v4i bv = bnext;
-                if (gi + 1 < groups) bnext = sb[(gi + 1) * 32];
+                if (gi + 1 < kGroups) bnext = sb[(gi + 1) * 32];
                 const v16i zero = {};
                 auto red = [](const v16i& c) {
                     int m = min(min(c[0], c[1]), c[2]);
@@ -595,33 +583,35 @@ __global__ __launch_bounds__(256) void bz_mfma(TiledArgs a) {
                         if (p[v] >= thrd) continue;
                         const u32 ij = lay[lane * 16 + v];
                         const u64 lmask = wlmk[kMfmaBlocks * (ij & 31) + b];
-                        const int si = gi * 32 + (int)(ij >> 8);
-                        u32 ew[XD];
-#pragma unroll
-                        for (int f = 0; f < XD; ++f) ew[f] = stream[si].w[f];
-                        const int base = __popcll(lmask & lflags) + stream[si].off;
+                        const int pos = gi * 32 + (int)(ij >> 8);
+                        const u64 sm_ = smk[pos];
+                        const int base = __popcll(lmask & lflags) + foff + __popcll(sm_ & sflags);
                         int tk = base;
-                        {
+                        {   // XD residual dwords of the pair's codeword
                             u32 xw[XD];
 #pragma unroll
-                            for (int f = 0; f < XD; ++f) xw[f] = 0;
+                            for (int f = 0; f < XD; ++f) xw[f] = (fx && f < RD) ? fx[f] : 0u;
                             for (u64 mm = lmask; mm; mm &= mm - 1) {
-                                const int r = __builtin_ctzll(mm);
+                                const int r = lbase + __builtin_ctzll(mm);
 #pragma unroll
-                                for (int f = 0; f < XD; ++f) xw[f] ^= rows_fast[(lbase + r) * XD + f];
+                                for (int f = 0; f < XD; ++f) xw[f] ^= rows_fast[r * XD + f];
+                            }
+                            for (u64 mm = sm_; mm; mm &= mm - 1) {
+                                const int r = sbase + __builtin_ctzll(mm);
+#pragma unroll
+                                for (int f = 0; f < XD; ++f) xw[f] ^= rows_fast[r * XD + f];
                             }
 #pragma unroll
-                            for (int f = 0; f < XD; ++f) tk += __popc(xw[f] ^ ew[f]);
+                            for (int f = 0; f < XD; ++f) tk += __popc(xw[f]);
                         }
                         if (tk >= bound || tk >= mine) continue;
                         // exact weight and logical test of the pair
                         ++slow_entries;
-                        const u64 smask = sk ? colex_mask(c0 + (u64)si, sk, sm, sbin, sbk) : 0ull;
                         int w = base;
                         for (int f = 0; f < RD && w < bound && w < mine; ++f) {
                             u32 x = fx ? fx[f] : 0u;
                             for (u64 mm = lmask; mm; mm &= mm - 1) x ^= slow_tbl[(lbase + __builtin_ctzll(mm)) * RW + f];
-                            for (u64 mm = smask; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
+                            for (u64 mm = sm_; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
                             w += __popc(x);
                         }
                         if (w >= bound || w >= mine) continue;
@@ -629,7 +619,7 @@ __global__ __launch_bounds__(256) void bz_mfma(TiledArgs a) {
                         for (int f = RD; f < RW && !logical; ++f) {
                             u32 x = fx ? fx[f] : 0u;
                             for (u64 mm = lmask; mm; mm &= mm - 1) x ^= slow_tbl[(lbase + __builtin_ctzll(mm)) * RW + f];
-                            for (u64 mm = smask; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
+                            for (u64 mm = sm_; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
                             logical = x != 0;
                         }
                         if (logical) mine = w;

@@ -39,7 +39,6 @@ constexpr int kMfmaBlocks = 8;      // bz_mfma: 32-element lane blocks per wave
 constexpr int kMfmaWaves = 4;       // bz_mfma: waves per workgroup, sharing each stream chunk
 constexpr int kMfmaLane = 32 * kMfmaBlocks * kMfmaWaves;   // bz_mfma: lane elements per tile
 constexpr int kMfmaStream = 512;    // bz_mfma: stream chunk length (two elements per thread)
-constexpr int kMfmaSliceK = 11;     // bz_mfma: stream lists up to this k unrank from an LDS binomial slice
 constexpr int kMaxRW = 64;          // residual + syndrome dwords per row in the tiled kernel
 constexpr int kLdsFullRows = 1024;  // dwords: the whole K x RW table is kept in LDS up to this size
 constexpr int kBigOffset = (1 << 30) + (1 << 29);  // masks lanes past the end of a list
