@@ -569,19 +569,27 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
                     mn = min(mn, min(min(red(p0), red(p1)), min(red(p2), red(p3))));
                 }
                 if (__builtin_expect(__ballot(mn < thrd) == 0ull, 1)) continue;
-                // ---- exact tests of this group: the products are formed again block by block,
-                // and each lane tests the (lane element, stream element) pairs of its values
-                // below the threshold (both elements are then real: padding gives >= 127)
+                // ---- exact tests of this group: the products are formed again, each lane
+                // marks its values below the threshold (bit 16 (b % 4) + v of cand[b / 4]),
+                // then tests those (lane element, stream element) pairs; both elements are
+                // then real (padding gives products >= 127)
                 ++medium_entries;
                 int mine = QSF_WEIGHT_NONE;
+                u64 cand[2] = {0ull, 0ull};
 #pragma unroll
                 for (int b = 0; b < kMfmaBlocks; ++b) {
                     const v16i p = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b], bv, zero, 0, 0, 0);
                     if (__ballot(red(p) < thrd) == 0ull) continue;
+                    u64 bits = 0;
 #pragma unroll
-                    for (int v = 0; v < 16; ++v) {
-                        if (p[v] >= thrd) continue;
-                        const u32 ij = lay[lane * 16 + v];
+                    for (int v = 0; v < 16; ++v) bits |= (p[v] < thrd ? 1ull : 0ull) << (16 * (b & 3) + v);
+                    cand[b >> 2] |= bits;
+                }
+                for (int part = 0; part < 2; ++part) {
+                    for (u64 cm = part ? cand[1] : cand[0]; cm; cm &= cm - 1) {
+                        const int q = __builtin_ctzll(cm);
+                        const int b = 4 * part + (q >> 4);
+                        const u32 ij = lay[lane * 16 + (q & 15)];
                         const u64 lmask = wlmk[kMfmaBlocks * (ij & 31) + b];
                         const int pos = gi * 32 + (int)(ij >> 8);
                         const u64 sm_ = smk[pos];
@@ -1308,7 +1316,12 @@ struct Engine {
         void emit(u64 f_lo, u64 f_hi, int xb, int xm, int xk, int yb, int ym, int yk, u64 nx, u64 ny) {
             Rect r{};
             r.g = g;
-            const bool x_lanes = nx >= ny;
+            // the longer list on the lanes; with the MFMA test's 1024-element lane tiles, the
+            // orientation that pads less (lane tiles x stream chunks)
+            auto padded = [&](u64 nl, u64 ns) {
+                return (double)((nl + lane_tile - 1) / lane_tile) * (double)lane_tile * (double)((ns + S - 1) / S) * (double)S;
+            };
+            const bool x_lanes = lane_tile == (u64)kMfmaLane ? padded(nx, ny) <= padded(ny, nx) : nx >= ny;
             r.lbase = x_lanes ? xb : yb; r.lm = x_lanes ? xm : ym; r.lk = x_lanes ? xk : yk;
             r.sbase = x_lanes ? yb : xb; r.sm = x_lanes ? ym : xm; r.sk = x_lanes ? yk : xk;
             r.n_lane = x_lanes ? nx : ny;
