@@ -42,14 +42,15 @@ VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12     # int32 lane-ops/s, gfx950
 MFMA_I8_PEAK_TOPS = 5000.0                          # dense int8 MFMA, MI355X (ops = 2 x multiply-adds)
 MFMA_OPS_PER_CODEWORD = 64                          # 32 int8 multiply-adds per codeword (bz_mfma)
 OPS_PER_CODEWORD = 5                                # 2 xor + 2 bcnt + 1 compare (64-bit test)
-TRAFFIC_FILE = ROOT / "profiles" / "bz_tiled_gross_traffic.json"
+TRAFFIC_FILES = {"bz_mfma": ROOT / "profiles" / "bz_mfma_gross_traffic.json",
+                 "bz_tiled": ROOT / "profiles" / "bz_tiled_gross_traffic.json"}
 
 
-def traffic_bytes():
-    """HBM bytes per launch of the dominant tiled dispatch, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-kernel scaled), or None."""
+def traffic_bytes(kernel: str):
+    """HBM bytes per launch of the kernel's largest gross dispatch, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel scaled), or None."""
     try:
-        return json.loads(TRAFFIC_FILE.read_text())["bytes_per_launch"]
+        return json.loads(TRAFFIC_FILES[kernel].read_text())["bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         return None
 
@@ -226,7 +227,7 @@ def main():
             "peak": rf_peak,
             "unit": "TOP/s",
             "frac": rf_achieved / rf_peak,
-            "traffic": traffic_bytes(),
+            "traffic": traffic_bytes(rf_kernel),
             "kernel": rf_kernel,
             "mfma_codewords_per_s": mfma_cw / (mfma_ms * 1e-3) if mfma_ms > 0 else None,
             "mfma_ms_per_step": mfma_ms / args.steps,

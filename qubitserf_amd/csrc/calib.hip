@@ -11,6 +11,11 @@
 //                             (2 per multiply-add).
 //   qsa_calib mfma_max        the int8 hot-test pattern: per step 8 products (zero
 //                             accumulator) reduced with v_max3; prints products/s.
+//   qsa_calib mfma16          as mfma with v_mfma_i32_16x16x64_i8.
+//   qsa_calib mfma16_and      the packed hot-test pattern: per step 8 16x16x64 products
+//                             (bias accumulator) AND-reduced, two codewords per entry;
+//                             prints codewords/s.
+//   An optional last argument sets the waves per CU of the mfma probes (default 12).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -90,6 +95,42 @@ __global__ __launch_bounds__(64) void mfma_max(int* out, int iters) {
     out[blockIdx.x * 64 + l] = hit;
 }
 
+__global__ __launch_bounds__(64) void mfma16_chain(int* out, int iters) {
+    const int l = threadIdx.x;
+    v4i a[8], b = {l, l * 3, l * 5, l * 7};
+    v4i c[8];
+    for (int k = 0; k < 8; ++k) {
+        a[k] = v4i{l + k, l ^ k, l * k, l - k};
+        c[k] = v4i{};
+    }
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], b, c[k], 0, 0, 0);
+    int r = 0;
+    for (int k = 0; k < 8; ++k)
+        for (int v = 0; v < 4; ++v) r ^= c[k][v];
+    out[blockIdx.x * 64 + l] = r;
+}
+
+__global__ __launch_bounds__(64) void mfma16_and(int* out, int iters) {
+    const int l = threadIdx.x;
+    v4i a[8], b = {l, l * 3, l * 5, l * 7};
+    for (int k = 0; k < 8; ++k) a[k] = v4i{l + k, l ^ k, l * k, l - k};
+    const v4i bias = {l, l, l, l};
+    int hit = 0;
+    for (int i = 0; i < iters; ++i) {
+        b[0] += 1;
+        unsigned acc = ~0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const v4i p = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], b, bias, 0, 0, 0);
+            acc &= (unsigned)p[0] & (unsigned)p[1] & (unsigned)p[2] & (unsigned)p[3];
+        }
+        hit += __ballot((acc & 0x820u) != 0x820u) != 0ull;
+    }
+    out[blockIdx.x * 64 + l] = hit;
+}
+
 __global__ __launch_bounds__(256) void copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -141,14 +182,18 @@ int main(int argc, char** argv) {
         CHECK(hipFree(b));
         return 0;
     }
-    if (std::strcmp(argv[1], "mfma") == 0 || std::strcmp(argv[1], "mfma_max") == 0) {
-        const bool chain = std::strcmp(argv[1], "mfma") == 0;
-        const int blocks = 256 * 12, iters = chain ? 4000 : 2000;
+    const bool m32c = std::strcmp(argv[1], "mfma") == 0, m32r = std::strcmp(argv[1], "mfma_max") == 0;
+    const bool m16c = std::strcmp(argv[1], "mfma16") == 0, m16r = std::strcmp(argv[1], "mfma16_and") == 0;
+    if (m32c || m32r || m16c || m16r) {
+        const int per_cu = argc >= 3 ? std::atoi(argv[2]) : 12;
+        const int blocks = 256 * per_cu, iters = (m32c || m16c) ? 4000 : 2000;
         int* out;
         CHECK(hipMalloc(&out, (size_t)blocks * 64 * sizeof(int)));
         auto launch = [&](int it) {
-            if (chain) hipLaunchKernelGGL(mfma_chain, dim3(blocks), dim3(64), 0, 0, out, it);
-            else hipLaunchKernelGGL(mfma_max, dim3(blocks), dim3(64), 0, 0, out, it);
+            if (m32c) hipLaunchKernelGGL(mfma_chain, dim3(blocks), dim3(64), 0, 0, out, it);
+            else if (m32r) hipLaunchKernelGGL(mfma_max, dim3(blocks), dim3(64), 0, 0, out, it);
+            else if (m16c) hipLaunchKernelGGL(mfma16_chain, dim3(blocks), dim3(64), 0, 0, out, it);
+            else hipLaunchKernelGGL(mfma16_and, dim3(blocks), dim3(64), 0, 0, out, it);
         };
         launch(10);
         CHECK(hipDeviceSynchronize());
@@ -159,12 +204,15 @@ int main(int argc, char** argv) {
         float ms = 0;
         CHECK(hipEventElapsedTime(&ms, e0, e1));
         const double mfmas = (double)blocks * iters * 8;
-        if (chain)
-            std::printf("{\"probe\": \"mfma\", \"int8_ops\": %.6e, \"ms\": %.3f, \"tops\": %.1f}\n", mfmas * 65536, ms,
-                        mfmas * 65536 / (ms * 1e-3) / 1e12);
-        else
-            std::printf("{\"probe\": \"mfma_max\", \"products\": %.6e, \"ms\": %.3f, \"products_per_s\": %.4e}\n",
-                        mfmas * 1024, ms, mfmas * 1024 / (ms * 1e-3));
+        const double ops_each = (m32c || m32r) ? 65536.0 : 32768.0;
+        if (m32c || m16c)
+            std::printf("{\"probe\": \"%s\", \"waves_per_cu\": %d, \"int8_ops\": %.6e, \"ms\": %.3f, \"tops\": %.1f}\n", argv[1],
+                        per_cu, mfmas * ops_each, ms, mfmas * ops_each / (ms * 1e-3) / 1e12);
+        else {
+            const double cw = mfmas * (m32r ? 1024.0 : 512.0);
+            std::printf("{\"probe\": \"%s\", \"waves_per_cu\": %d, \"products\": %.6e, \"ms\": %.3f, \"products_per_s\": %.4e}\n",
+                        argv[1], per_cu, cw, ms, cw / (ms * 1e-3));
+        }
         CHECK(hipFree(out));
         return 0;
     }
