@@ -82,6 +82,15 @@ class DevArray:
             d._ok(d.L.qsf_dev_copy(d.ptr, host.ctypes.data, d.nbytes, 1), "qsf_dev_copy")
         return d
 
+    @classmethod
+    def from_bits(cls, bits):
+        """A bfloat16 array from its bit patterns (uint16), uploaded as they are."""
+        bits = np.ascontiguousarray(bits, np.uint16)
+        d = cls(bits.shape, BF16, zero=False)
+        if d.nbytes:
+            d._ok(d.L.qsf_dev_copy(d.ptr, bits.ctypes.data, d.nbytes, 1), "qsf_dev_copy")
+        return d
+
     def numpy(self) -> np.ndarray:
         host = np.empty(self.shape, np.float32 if self.dtype == F32 else np.uint16)
         if self.nbytes:

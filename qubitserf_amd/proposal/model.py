@@ -134,6 +134,43 @@ def pack_batch(Hs, seeds=None):
     return X, Ac, Av, mask
 
 
+def pack_batch_bits(Hs, seeds=None):
+    """pack_batch computed for the whole batch at once, with the two averaging operators
+    built directly as bfloat16 bit patterns (uint16): an entry of Ac is 1 / deg(check) and
+    of Av 1 / deg(column), so both come from one table of the patterns of 1 / k. Equal, bit
+    for bit, to pack_batch with bf16_bits applied to its float32 operators."""
+    from .devarray import bf16_bits
+
+    B = len(Hs)
+    m = max(h.shape[0] for h in Hs)
+    n = max(h.shape[1] for h in Hs)
+    Hb = np.zeros((B, m, n), bool)
+    hms = np.zeros(B, np.int64)
+    hns = np.zeros(B, np.int64)
+    for b, H in enumerate(Hs):
+        H = np.asarray(H)
+        Hb[b, : H.shape[0], : H.shape[1]] = (H.astype(np.uint8) & 1).astype(bool)
+        hms[b], hns[b] = H.shape
+    mask = (np.arange(n)[None, :] < hns[:, None]).astype(np.float32)
+    dc_raw = Hb.sum(2).astype(np.float32)                         # (B, m)
+    dv_raw = Hb.sum(1).astype(np.float32)                         # (B, n)
+    inv = bf16_bits(np.float32(1.0) / np.arange(1, max(m, n) + 1, dtype=np.float32))
+    dc = np.maximum(dc_raw, 1).astype(np.int64)
+    dv = np.maximum(dv_raw, 1).astype(np.int64)
+    Ac = np.where(Hb, inv[dc - 1][:, :, None], np.uint16(0))
+    Av = np.where(Hb.transpose(0, 2, 1), inv[dv - 1][:, :, None], np.uint16(0))
+    X = np.zeros((B, n, F), np.float32)
+    X[:, :, 0] = mask
+    X[:, :, 1] = dv_raw / np.maximum(np.float32(1.0), dv_raw.max(1, keepdims=True))
+    mean_dc = (Hb * dc_raw[:, :, None]).sum(1, dtype=np.float32) / np.maximum(dv_raw, np.float32(1.0))
+    X[:, :, 2] = mean_dc / np.maximum(np.float32(1.0), dc_raw.max(1, keepdims=True))
+    X[:, :, 3] = (hms / hns).astype(np.float32)[:, None] * mask
+    for b in range(B):
+        X[b, : hns[b], 4:8] = np.random.default_rng(0 if seeds is None else seeds[b]).standard_normal(
+            (int(hns[b]), 4)).astype(np.float32) * 0.5
+    return X, Ac, Av, mask
+
+
 def param_shapes():
     shp = {"Win": (F, D), "bin": (D,)}
     for l in range(LAYERS):
@@ -175,8 +212,9 @@ class MPScorer:
         m = Ac.shape[1]
         c = {"B": B, "n": n, "m": m}
         c["X"] = DevArray.from_numpy(X, BF16)
-        c["Ac"] = DevArray.from_numpy(Ac, BF16)
-        c["Av"] = DevArray.from_numpy(Av, BF16)
+        # operators as float32 values, or already as bfloat16 patterns (pack_batch_bits)
+        c["Ac"] = DevArray.from_bits(Ac) if Ac.dtype == np.uint16 else DevArray.from_numpy(Ac, BF16)
+        c["Av"] = DevArray.from_bits(Av) if Av.dtype == np.uint16 else DevArray.from_numpy(Av, BF16)
         h = DevArray((B, n, D), BF16, zero=False)
         dv.gemm(c["X"], self.wb["Win"], h, B * n, D, F, bias=self.w["bin"], relu=1)
         c["h0"] = h
@@ -258,7 +296,7 @@ class MPScorer:
 
     def scores(self, Hs, seeds=None):
         """Column logits of each code (numpy, length n each)."""
-        X, Ac, Av, mask = pack_batch(Hs, seeds)
+        X, Ac, Av, mask = pack_batch_bits(Hs, seeds)
         s, _ = self.forward(X, Ac, Av)
         s = s.numpy()
         return [s[b, : H.shape[1]].copy() for b, H in enumerate(Hs)]

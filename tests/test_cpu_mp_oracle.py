@@ -69,3 +69,18 @@ def test_bf16_mode_close_to_exact():
     s64, _ = M.forward(P, X, Ac, Av, bf=False)
     s16, _ = M.forward(P, X, Ac, Av, bf=True)
     assert np.abs(s64 - s16).max() <= 0.05 * (1 + np.abs(s64).max())
+
+
+def test_pack_batch_bits_equals_pack_batch():
+    """The batched bit-pattern packing used for inference equals pack_batch followed by
+    the float32 -> bfloat16 rounding, bit for bit, on ragged codes."""
+    from qubitserf_amd import codes
+    from qubitserf_amd.proposal.devarray import bf16_bits
+    from qubitserf_amd.proposal.model import pack_batch, pack_batch_bits
+
+    Hs = codes.config3_batch(5, 12) + [np.eye(3, 7, dtype=np.uint8), np.zeros((2, 5), np.uint8)]
+    seeds = list(range(100, 100 + len(Hs)))
+    X, Ac, Av, mask = pack_batch(Hs, seeds)
+    X2, A2, V2, m2 = pack_batch_bits(Hs, seeds)
+    assert np.array_equal(X, X2) and np.array_equal(mask, m2)
+    assert A2.dtype == np.uint16 and np.array_equal(bf16_bits(Ac), A2) and np.array_equal(bf16_bits(Av), V2)
