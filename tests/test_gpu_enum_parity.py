@@ -183,9 +183,12 @@ def test_mfma_hot_test(hip, xd, monkeypatch):
             K = int(rng.integers(24, 121)); stride = 3 if xd > 1 else int(rng.integers(1, 4))
             d = int(rng.integers(3, 6)) if K < 90 else 4
             kcheck = int(rng.integers(0, 4))
-            best = [NONE, 70, 45, 30, 22, 14, 9, 60, 35, 25][i]
+            best = [100, 70, 45, 30, 22, 14, 9, 60, 35, 25][i]
             run_case(hip, rng, K, stride, d, 1 + i % 2, kcheck, best, systematic=bool(i % 3))
             assert hip.last_stats()["fd"] == 0
+        # above 100 (products are compared with min(bound, 127)) the VALU kernel takes over
+        run_case(hip, rng, 60, 3, 4, 1, 1, NONE, systematic=True)
+        assert hip.last_stats()["fd"] > 0
 
 
 def test_mfma_chosen_on_large_levels(hip):
@@ -197,3 +200,4 @@ def test_mfma_chosen_on_large_levels(hip):
     want = O.enumerate_plan(gamma, n, 5, check=check, current_best=8)
     got = hip.enumerate_level(gamma, n, 5, check=check, current_best=8)
     assert hip.last_stats()["fd"] == 0 and got == want
+
