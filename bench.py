@@ -9,9 +9,12 @@ summed over levels and sides), with the time-to-certify alongside.
 
 Prints ONE JSON line. Fields beyond the common contract:
   time_to_certify_s   mean wall time of one solve
-  roofline            of the dominant enumeration kernel. bz_mfma (int8 matrix-core hot
+  roofline            of the enumeration kernel that holds most of the device time, with
+                      every kernel's figures under "kernels". bz_mfma (int8 matrix-core hot
                       test): achieved = 64 int8 ops (32 multiply-adds) x its codewords / its
-                      device time, peak = 5.0 POP/s dense int8. bz_tiled (VALU hot test):
+                      device time, peak = 5.0 POP/s dense int8. bz_f4 (fp4 products,
+                      32x32x64 / 16x16x128): 128 / 256 ops per codeword, peak 10.1 POP/s
+                      dense fp4. bz_tiled (VALU hot test):
                       achieved = sum over levels of (the variant's hot-loop lane-ops per
                       codeword x codewords) / device time (HIP events on its stream), peak =
                       256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-ops/s.
@@ -39,9 +42,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12     # int32 lane-ops/s, gfx950
-MFMA_I8_PEAK_TOPS = 5000.0                          # dense int8 MFMA, MI355X (ops = 2 x multiply-adds)
-MFMA_OPS_PER_CODEWORD = 64                          # 32 int8 multiply-adds per codeword (bz_mfma)
 OPS_PER_CODEWORD = 5                                # 2 xor + 2 bcnt + 1 compare (64-bit test)
+# Matrix-core hot tests by the trace's `mfma` field: kernel, ops per codeword (2 per
+# multiply-add: 32 int8 dimensions, or 64 / 128 fp4 ones), dense MI355X peak in TOP/s.
+MATRIX = {1: ("bz_mfma", 64, 5000.0), 2: ("bz_f4_32x32x64", 128, 10100.0), 3: ("bz_f4_16x16x128", 256, 10100.0)}
 TRAFFIC_FILES = {"bz_mfma": ROOT / "profiles" / "bz_mfma_gross_traffic.json",
                  "bz_tiled": ROOT / "profiles" / "bz_tiled_gross_traffic.json"}
 
@@ -173,7 +177,7 @@ def main():
         solve()
 
     walls, codewords, tiled_cw, tiled_ms, tiled_ops = [], 0, 0, 0.0, 0.0
-    mfma_cw, mfma_ms, valu_cw, valu_ms = 0, 0.0, 0, 0.0
+    per = {}          # kernel -> [codewords, device ms, algorithmic ops]
     result = None
     for _ in range(args.steps):
         t0 = time.perf_counter()
@@ -186,24 +190,29 @@ def main():
                 tiled_cw += t["codewords"]
                 tiled_ms += t["kernel_ms"]
                 tiled_ops += t["ops_per_codeword"] * t["codewords"]
-                if t["fd"] == 0:      # int8-MFMA hot test
-                    mfma_cw += t["codewords"]
-                    mfma_ms += t["kernel_ms"]
-                else:
-                    valu_cw += t["codewords"]
-                    valu_ms += t["kernel_ms"]
+                if t["mfma"]:      # matrix-core hot test: its products are the work
+                    name, ops, _ = MATRIX[t["mfma"]]
+                else:              # VALU hot test: the variant's hot-loop lane-ops
+                    name, ops = "bz_tiled", t["ops_per_codeword"]
+                k = per.setdefault(name, [0, 0.0, 0.0])
+                k[0] += t["codewords"]
+                k[1] += t["kernel_ms"]
+                k[2] += ops * t["codewords"]
     total_s = sum(walls)
     value = codewords / total_s
     achieved = tiled_ops / (tiled_ms * 1e-3) / 1e12 if tiled_ms > 0 else 0.0
     five_op = OPS_PER_CODEWORD * tiled_cw / (tiled_ms * 1e-3) / 1e12 if tiled_ms > 0 else 0.0
-    # The MFMA hot test spends 32 int8 multiply-adds (64 ops) per codeword on the matrix
-    # cores; when its levels dominate the device time, the roofline is the int8 MFMA peak.
-    mfma_bound = mfma_ms > valu_ms
-    if mfma_bound:
-        rf_achieved = MFMA_OPS_PER_CODEWORD * mfma_cw / (mfma_ms * 1e-3) / 1e12
-        rf_peak, rf_bound, rf_kernel = MFMA_I8_PEAK_TOPS, "mfma", "bz_mfma"
-    else:
-        rf_achieved, rf_peak, rf_bound, rf_kernel = achieved, VALU_PEAK_TOPS, "valu", "bz_tiled"
+    peaks = {name: pk for name, _, pk in MATRIX.values()}
+    peaks["bz_tiled"] = VALU_PEAK_TOPS
+    kernels = {name: {"codewords_per_s": cw / (ms * 1e-3), "ms_per_step": ms / args.steps,
+                      "TOPs": o / (ms * 1e-3) / 1e12, "peak": peaks[name],
+                      "frac": o / (ms * 1e-3) / 1e12 / peaks[name]}
+               for name, (cw, ms, o) in per.items() if ms > 0}
+    # the roofline of the kernel that holds most of the device time
+    rf_kernel = max(kernels, key=lambda n: kernels[n]["ms_per_step"]) if kernels else "bz_tiled"
+    rf = kernels.get(rf_kernel, {"TOPs": 0.0, "peak": VALU_PEAK_TOPS})
+    rf_achieved, rf_peak = rf["TOPs"], rf["peak"]
+    rf_bound = "valu" if rf_kernel == "bz_tiled" else "mfma"
 
     line = {
         "metric": "codewords/s + time-to-certify exact d (plain vs batched+learned, 1 GPU)",
@@ -233,9 +242,7 @@ def main():
             "frac": rf_achieved / rf_peak,
             "traffic": traffic_bytes(rf_kernel),
             "kernel": rf_kernel,
-            "mfma_codewords_per_s": mfma_cw / (mfma_ms * 1e-3) if mfma_ms > 0 else None,
-            "mfma_ms_per_step": mfma_ms / args.steps,
-            "valu_kernel_ms_per_step": valu_ms / args.steps,
+            "kernels": kernels,
             "valu_TOPs_all_tiled": achieved,
             "ops_per_codeword": tiled_ops / tiled_cw if tiled_cw else None,
             "kernel_ms_per_step": tiled_ms / args.steps,

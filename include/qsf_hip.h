@@ -172,6 +172,8 @@ typedef struct {
     double ops_per_codeword; /* tiled kernel: hot-loop int32 VALU lane-ops per codeword,
                                 (2 FD RPL + RPL/2 + 3) / RPL: FD xor+bcnt pairs per codeword,
                                 a v_min3 per two codewords, 3 per stream step */
+    int mfma;             /* tiled kernel: matrix-core hot test of the last launch: 0 none (VALU),
+                             1 int8 32x32x32 products, 2 fp4 32x32x64, 3 fp4 16x16x128 */
 } qsf_hip_stats;
 void qsf_hip_last_stats(qsf_hip_stats* out);
 

@@ -72,6 +72,7 @@ class HipStats(ctypes.Structure):
         ("medium_hits", ctypes.c_uint64),
         ("rpl", ctypes.c_int),
         ("ops_per_codeword", ctypes.c_double),
+        ("mfma", ctypes.c_int),
     ]
 
 
@@ -119,7 +120,7 @@ class DistfindBatchStats(ctypes.Structure):
 
 class DistfindLevel(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("side", "d", "active", "ngamma", "K", "inner", "outer", "kernel",
-                                              "fd", "xd", "rpl", "pad")] + [
+                                              "fd", "xd", "rpl", "mfma")] + [
         ("codewords", ctypes.c_uint64), ("medium_hits", ctypes.c_uint64), ("slow_hits", ctypes.c_uint64),
         ("ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("ops_per_codeword", ctypes.c_double)]
 

@@ -191,7 +191,7 @@ int distfind_last_trace(DistfindLevel* out, int cap) {
         std::memset(&o, 0, sizeof(o));
         o.side = t.side; o.d = t.d; o.active = t.active; o.ngamma = t.ngamma; o.K = t.K;
         o.inner = t.inner; o.outer = t.outer; o.kernel = t.st.kernel;
-        o.fd = t.st.fd; o.xd = t.st.xd; o.rpl = t.st.rpl;
+        o.fd = t.st.fd; o.xd = t.st.xd; o.rpl = t.st.rpl; o.mfma = t.st.mfma;
         o.codewords = t.st.codewords; o.medium_hits = t.st.medium_hits; o.slow_hits = t.st.slow_hits;
         o.ms = t.ms; o.kernel_ms = t.st.kernel_ms; o.ops_per_codeword = t.st.ops_per_codeword;
     }

@@ -648,6 +648,400 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
 }
 
 // =====================================================================================
+// bz_f4: the hot test as fp4 (e2m1) matrix products (min mode)
+// =====================================================================================
+//
+// As bz_mfma, on the fp4 matrix instructions with unit block scales:
+//   MM = 2  v_mfma_scale_f32_32x32x64_f8f6f4   32 x 32 codewords per product, 64 dimensions
+//   MM = 3  v_mfma_scale_f32_16x16x128_f8f6f4  16 x 16 codewords per product, 128 dimensions
+// gfx950 runs fp4 products at twice the int8 rate, so a product tests 60 (MM = 2) or 124
+// (MM = 3) residual bits for about the time the int8 test spends on 30. The lane side holds
+// +1 / -1 (codes 0x2 / 0xA) per test bit, the stream side 1 / 0 (0x2 / 0x0). Dimensions
+// 28..31 of the last 32-dimension block carry the lane constant L = off_lane + |a| as four
+// fp4 digits (d0, d1, d2, d3) against the fixed stream values (6, 6, 6, 1), so the f32
+// products are exact small integers:
+//     out = L + sum_i (1 - 2 a_i) b_i = off_lane + hd(a, b)   over the test bits.
+// Every value a lane holds belongs to the stream element of its column, lane mod NR: that
+// is the instructions' output layout, read off index products at start-up and checked (a
+// launch that finds another layout reports it and the host raises). The stream constant
+// off_stream then enters as a per-lane threshold, min(bound, 127) - off_stream.
+// Block kb of the 32-dimension blocks (lanes kb NR .. kb NR + NR - 1) holds residual word
+// kb of both elements: nibble i of dword j is bit j + 4 i, i.e. (w >> j) & 0x11111111
+// moved into place (two VALU per dword); the last word keeps its bits 0..27.
+// Padding: a lane element past its list holds only the constant 114 (digits 6, 6, 6, 6), a
+// stream element past its list only zeros with off_stream = 127; the host uses these
+// kernels only while the bound is at most 100, so neither can pass.
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int MM> struct F4Prod;
+template <> struct F4Prod<2> {
+    typedef f32x16 acc;
+    static constexpr int NR = 32;
+    __device__ static acc mm(i32x4 a, i32x4 b) {
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(i32x8{a[0], a[1], a[2], a[3], 0, 0, 0, 0},
+                                                               i32x8{b[0], b[1], b[2], b[3], 0, 0, 0, 0}, acc{},
+                                                               4, 4, 0, 127, 0, 127);
+    }
+};
+template <> struct F4Prod<3> {
+    typedef f32x4 acc;
+    static constexpr int NR = 16;
+    __device__ static acc mm(i32x4 a, i32x4 b) {
+        return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(i32x8{a[0], a[1], a[2], a[3], 0, 0, 0, 0},
+                                                                i32x8{b[0], b[1], b[2], b[3], 0, 0, 0, 0}, acc{},
+                                                                4, 4, 0, 127, 0, 127);
+    }
+};
+
+// fp4 e2m1 code of v in {-1, 0, 1, 2, 3, 4, 6}
+__device__ __forceinline__ u32 f4_code(int v) {
+    return v < 0 ? 0xAu : v == 0 ? 0u : v == 1 ? 2u : v == 2 ? 4u : v == 3 ? 5u : v == 4 ? 6u : 7u;
+}
+
+// The lane constant L (clamped to [0, 100]; a smaller value only weakens the test) as
+// 6 (d0 + d1 + d2) + d3, digits d0..d2 in {0, 1, 2, 3, 4, 6} and d3 in {-1, 0, .., 4}:
+// nibble j of the result is the code of d_j.
+__device__ __forceinline__ u32 f4_lane_const(int L) {
+    L = max(0, min(L, 100));
+    int t = L / 6, u = L - 6 * t;
+    if (u == 5) { t += 1; u = -1; }
+    const int d0 = t >= 6 ? 6 : (t == 5 ? 4 : t);
+    const int r = t - d0;
+    const int d1 = r >= 6 ? 6 : (r == 5 ? 4 : r);
+    return f4_code(d0) | (f4_code(d1) << 4) | (f4_code(r - d1) << 8) | (f4_code(u) << 12);
+}
+
+template <int XD, int MM>
+__global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
+    typedef F4Prod<MM> P;
+    typedef typename P::acc acc;
+    constexpr int NR = P::NR;                     // rows (lane elements) = columns (stream elements) per product
+    constexpr int NKB = 64 / NR;                  // 32-dimension blocks across the lanes
+    constexpr int NV = NR * NR / 64;              // product values per lane
+    constexpr int LW = kMfmaBlocks * NR;          // lane elements per wave
+    constexpr int SL = f4_stream(MM);             // stream chunk length
+    constexpr int EPT = SL / (64 * kMfmaWaves);   // stream elements each thread makes per chunk
+    static_assert(XD >= NKB, "the held residual dwords must cover the test bits");
+    __shared__ u32 rows_fast[kMaxTiledK * XD];
+    __shared__ i32x4 sbytes[NKB * SL];            // block kb of chunk position e at [kb * SL + e]
+    __shared__ float soff[SL];                    // off_stream of each chunk position (127: padding)
+    __shared__ u64 smk[SL];                       // the chunk's stream-element subsets
+    __shared__ u64 lmk[kMfmaWaves * LW];          // each wave's lane-element subsets
+    __shared__ unsigned short lay[64 * NV];       // product layout: (row, column) of lane l, value v
+    __shared__ int s_bad;
+    __shared__ unsigned long long s_tile;
+    extern __shared__ u32 rows_full[];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int kb = lane / NR, row = lane % NR;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    if (wave == 0) {
+        // Index products on dimensions 0..2 (block 0): row-index digits (r & 3, (r >> 2) & 3,
+        // 4 (r >> 4)) against weights (1, 4, 4) give r; weights against column digits give c.
+        const u32 idx = kb == 0 ? (f4_code(row & 3) | (f4_code((row >> 2) & 3) << 4) | (f4_code(4 * (row >> 4)) << 8)) : 0u;
+        const u32 wts = kb == 0 ? (f4_code(1) | (f4_code(4) << 4) | (f4_code(4) << 8)) : 0u;
+        const i32x4 I = {(int)idx, 0, 0, 0}, W = {(int)wts, 0, 0, 0};
+        const acc pr = P::mm(I, W), pc = P::mm(W, I);
+        bool bad = false;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int r = (int)pr[v], c = (int)pc[v];
+            bad = bad || c != row || r < 0 || r >= NR;
+            lay[lane * NV + v] = (unsigned short)((r & 31) | ((c & 31) << 8));
+        }
+        if (__ballot(bad) != 0ull && lane == 0) s_bad = 1;
+    }
+    __syncthreads();
+    if (s_bad) {   // another output layout: report it, enumerate nothing
+        if (tid == 0) atomicAdd(a.out_rows, 1ull);
+        return;
+    }
+
+    int cur_job = -1, cur_g = -1;
+    int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
+    const TJob* J = nullptr;
+    unsigned slow_entries = 0, medium_entries = 0;
+
+    for (;;) {
+        __syncthreads();   // every wave is done with the previous tile's LDS
+        if (tid == 0) s_tile = atomicAdd(a.next_tile, 1ull);
+        __syncthreads();
+        const u64 t = uni64(s_tile);
+        if (t >= a.total_tiles) break;
+        int lo = 0, hi = a.nrects - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (a.rects[mid].tile_base <= t) lo = mid; else hi = mid - 1;
+        }
+        const Rect& Rr = a.rects[uni(lo)];
+        const int job = uni(Rr.job);
+        if (job != cur_job) {
+            J = a.jobs + job;
+            K = uni(J->K); RW = uni(J->RW); RD = uni(J->RD); SD = uni(J->SD); floor = uni(J->floor);
+            bound = uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            cur_job = job;
+            cur_g = -1;
+        }
+        bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+        if (__syncthreads_or(bound <= floor)) {
+            if (a.njobs == 1) break;
+            continue;
+        }
+        const u64 tile_base = uni64(Rr.tile_base), stream_groups = uni64(Rr.stream_groups);
+        const u64 stream_chunks = uni64(Rr.stream_chunks), n_lane = uni64(Rr.n_lane), n_stream = uni64(Rr.n_stream);
+        const int fixed = uni(Rr.fixed);
+        const u64 local = t - tile_base;
+        const u64 bx = local / stream_groups;
+        const u64 bg = local - bx * stream_groups;
+        const int g = uni(Rr.g);
+        const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
+        const int sk = uni(Rr.sk), sbase = uni(Rr.sbase), sm = uni(Rr.sm);
+        const u64 flo = J->flags[2 * g], fhi = J->flags[2 * g + 1];
+        const u64 lflags = range_bits(flo, fhi, lbase, lm);
+        const u64 sflags = range_bits(flo, fhi, sbase, sm);
+        const u32* fx = fixed >= 0 ? J->fixed + (size_t)fixed * (RW + 1) : nullptr;
+        const int foff = fx ? (int)fx[RW] : 0;
+        const u32* gtbl = J->tbl + (size_t)g * K * RW;
+        const bool full_in_lds = K * RW <= a.lds_full;
+        const u32* slow_tbl = full_in_lds ? rows_full : gtbl;
+        if (g != cur_g) {   // same tile in every wave: the branch is workgroup-uniform
+            for (int i = tid; i < K * XD; i += 256) rows_fast[i] = (i % XD) < RD ? gtbl[(i / XD) * RW + (i % XD)] : 0u;
+            if (full_in_lds)
+                for (int i = tid; i < K * RW; i += 256) rows_full[i] = gtbl[i];
+            cur_g = g;
+        }
+        __syncthreads();
+
+        // ---- this wave's LW lane elements: row r of block b is element
+        // bx * (4 LW) + LW wave + 8 r + b (the NKB lanes of a row share it)
+        i32x4 av[kMfmaBlocks];
+        u64* wlmk = lmk + LW * wave;
+        {
+            const u64 first = bx * (u64)(kMfmaWaves * LW) + (u64)(LW * wave + kMfmaBlocks * row);
+            u64 mnext = (first < n_lane && lk) ? colex_mask(first, lk, lm, a.binom, a.bk) : 0ull;
+#pragma unroll
+            for (int b = 0; b < kMfmaBlocks; ++b) {
+                const u64 idx = first + (u64)b;
+                const bool valid = idx < n_lane;
+                const u64 m = valid ? mnext : 0ull;
+                if (valid && lk && idx + 1 < n_lane) mnext = colex_next(mnext);
+                if (kb == 0) wlmk[kMfmaBlocks * row + b] = m;
+                u32 x[NKB];
+#pragma unroll
+                for (int w = 0; w < NKB; ++w) x[w] = 0u;
+                for (u64 mm = m; mm; mm &= mm - 1) {
+                    const int r = lbase + __builtin_ctzll(mm);
+#pragma unroll
+                    for (int w = 0; w < NKB; ++w) x[w] ^= rows_fast[r * XD + w];
+                }
+                x[NKB - 1] &= 0x0fffffffu;
+                int L = __popcll(m & lflags);
+                u32 xk = x[0];
+#pragma unroll
+                for (int w = 0; w < NKB; ++w) {
+                    L += __popc(x[w]);
+                    if (kb == w) xk = x[w];
+                }
+                u32 q[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) q[j] = 0x22222222u | (((xk >> j) & 0x11111111u) << 3);
+                if (kb == NKB - 1) {
+                    const u32 c = valid ? f4_lane_const(L) : 0x7777u;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) q[j] = (valid ? (q[j] & 0x0fffffffu) : 0u) | (((c >> (4 * j)) & 15u) << 28);
+                } else if (!valid) {
+                    q[0] = q[1] = q[2] = q[3] = 0u;
+                }
+                av[b] = i32x4{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+            }
+        }
+
+        // ---- this thread's stream run: elements s0 + EPT nch tid + [0, EPT nch)
+        const u64 ch0 = bg * (u64)a.G;
+        const int nch = (int)(min(ch0 + (u64)a.G, stream_chunks) - ch0);
+        const u64 s0 = ch0 * (u64)SL;
+        u64 sidx = s0 + (u64)(EPT * nch) * (u64)tid;
+        u64 smask = (sidx < n_stream && sk) ? colex_mask(sidx, sk, sm, a.binom, a.bk) : 0ull;
+        u32 sx[NKB];
+#pragma unroll
+        for (int w = 0; w < NKB; ++w) sx[w] = (fx && w < RD) ? fx[w] : 0u;
+        for (u64 mm = smask; mm; mm &= mm - 1) {
+            const int r = sbase + __builtin_ctzll(mm);
+#pragma unroll
+            for (int w = 0; w < NKB; ++w) sx[w] ^= rows_fast[r * XD + w];
+        }
+
+        for (int k = 0; k < nch; ++k) {
+            if (k != 0) {
+                bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                if (__syncthreads_or(bound <= floor)) break;   // also: the previous chunk is consumed
+            }
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {   // chunk positions EPT tid + e
+                const int pos = EPT * tid + e;
+                if (sidx < n_stream) {
+#pragma unroll
+                    for (int w = 0; w < NKB; ++w) {
+                        const u32 sw = w == NKB - 1 ? (sx[w] & 0x0fffffffu) : sx[w];
+                        u32 q[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) q[j] = ((sw >> j) & 0x11111111u) << 1;
+                        if (w == NKB - 1) {   // the fixed values (6, 6, 6, 1) of the lane-constant dimensions
+                            q[0] |= 7u << 28; q[1] |= 7u << 28; q[2] |= 7u << 28; q[3] |= 2u << 28;
+                        }
+                        sbytes[w * SL + pos] = i32x4{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+                    }
+                    soff[pos] = (float)min(foff + __popcll(smask & sflags), 127);
+                    smk[pos] = smask;
+                    if (sk && sidx + 1 < n_stream) {   // colex successor; XOR the rows that change
+                        const u64 nm = colex_next(smask);
+                        for (u64 mm = nm ^ smask; mm; mm &= mm - 1) {
+                            const int r = sbase + __builtin_ctzll(mm);
+#pragma unroll
+                            for (int w = 0; w < NKB; ++w) sx[w] ^= rows_fast[r * XD + w];
+                        }
+                        smask = nm;
+                    }
+                    ++sidx;
+                } else {   // past the list: zeros, and a threshold no product reaches
+#pragma unroll
+                    for (int w = 0; w < NKB; ++w) sbytes[w * SL + pos] = i32x4{0, 0, 0, 0};
+                    soff[pos] = 127.0f;
+                    smk[pos] = 0ull;
+                }
+            }
+            __syncthreads();
+
+            float thrf = (float)min(bound, 127);
+            // B operand and stream constant one group ahead
+            const i32x4* sb = sbytes + kb * SL + row;
+            constexpr int kGroups = SL / NR;
+            i32x4 bnext = sb[0];
+            float onext = soff[row];
+            for (int gi = 0; gi < kGroups; ++gi) {
+                const i32x4 bv = bnext;
+                const float thr = thrf - onext;   // this lane's column: L + sum < bound - off_stream
+                if (gi + 1 < kGroups) {
+                    bnext = sb[(gi + 1) * NR];
+                    onext = soff[(gi + 1) * NR + row];
+                }
+                float mn = 3.0e38f;
+                if constexpr (NV == 16) {
+                    auto red = [](const acc& c) {
+                        float m = fminf(fminf(c[0], c[1]), c[2]);
+#pragma unroll
+                        for (int v = 3; v < 15; v += 2) m = fminf(fminf(m, c[v]), c[v + 1]);
+                        return fminf(m, c[15]);
+                    };
+#pragma unroll
+                    for (int b0 = 0; b0 < kMfmaBlocks; b0 += 4) {
+                        const acc p0 = P::mm(av[b0 + 0], bv);
+                        const acc p1 = P::mm(av[b0 + 1], bv);
+                        const acc p2 = P::mm(av[b0 + 2], bv);
+                        const acc p3 = P::mm(av[b0 + 3], bv);
+                        mn = fminf(mn, fminf(fminf(red(p0), red(p1)), fminf(red(p2), red(p3))));
+                    }
+                } else {
+                    acc p[kMfmaBlocks];
+#pragma unroll
+                    for (int b = 0; b < kMfmaBlocks; ++b) p[b] = P::mm(av[b], bv);
+#pragma unroll
+                    for (int b = 0; b < kMfmaBlocks; ++b)
+                        mn = fminf(fminf(mn, fminf(p[b][0], p[b][1])), fminf(p[b][2], p[b][3]));
+                }
+                if (__builtin_expect(__ballot(mn < thr) == 0ull, 1)) continue;
+                // ---- exact tests of this group: the products are formed again, each lane
+                // marks its values below its threshold (bit NV b + v), then tests those
+                // (lane element, stream element) pairs; both elements are then real
+                ++medium_entries;
+                int mine = QSF_WEIGHT_NONE;
+                u64 cand[2] = {0ull, 0ull};
+                // an opaque copy of the B operand: the products are formed again here instead
+                // of all eight being kept live from the test above (which would spill)
+                i32x4 bx = bv;
+                asm volatile("" : "+v"(bx));
+#pragma unroll
+                for (int b = 0; b < kMfmaBlocks; ++b) {
+                    const acc p = P::mm(av[b], bx);
+                    float pm = p[0];
+#pragma unroll
+                    for (int v = 1; v < NV; ++v) pm = fminf(pm, p[v]);
+                    if (__ballot(pm < thr) == 0ull) continue;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const int q = NV * b + v;
+                        if (p[v] < thr) cand[q >> 6] |= 1ull << (q & 63);
+                    }
+                }
+                for (int part = 0; part < 2; ++part) {
+                    for (u64 cm = part ? cand[1] : cand[0]; cm; cm &= cm - 1) {
+                        const int q = 64 * part + __builtin_ctzll(cm);
+                        const int b = q / NV;
+                        const u32 ij = lay[lane * NV + (q % NV)];
+                        const u64 lmask = wlmk[kMfmaBlocks * (ij & 31) + b];
+                        const int pos = gi * NR + (int)(ij >> 8);
+                        const u64 sm_ = smk[pos];
+                        const int base = __popcll(lmask & lflags) + foff + __popcll(sm_ & sflags);
+                        int tk = base;
+                        {   // XD residual dwords of the pair's codeword
+                            u32 xw[XD];
+#pragma unroll
+                            for (int f = 0; f < XD; ++f) xw[f] = (fx && f < RD) ? fx[f] : 0u;
+                            for (u64 mm = lmask; mm; mm &= mm - 1) {
+                                const int r = lbase + __builtin_ctzll(mm);
+#pragma unroll
+                                for (int f = 0; f < XD; ++f) xw[f] ^= rows_fast[r * XD + f];
+                            }
+                            for (u64 mm = sm_; mm; mm &= mm - 1) {
+                                const int r = sbase + __builtin_ctzll(mm);
+#pragma unroll
+                                for (int f = 0; f < XD; ++f) xw[f] ^= rows_fast[r * XD + f];
+                            }
+#pragma unroll
+                            for (int f = 0; f < XD; ++f) tk += __popc(xw[f]);
+                        }
+                        if (tk >= bound || tk >= mine) continue;
+                        // exact weight and logical test of the pair
+                        ++slow_entries;
+                        int w = base;
+                        for (int f = 0; f < RD && w < bound && w < mine; ++f) {
+                            u32 x = fx ? fx[f] : 0u;
+                            for (u64 mm = lmask; mm; mm &= mm - 1) x ^= slow_tbl[(lbase + __builtin_ctzll(mm)) * RW + f];
+                            for (u64 mm = sm_; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
+                            w += __popc(x);
+                        }
+                        if (w >= bound || w >= mine) continue;
+                        bool logical = (SD == 0);
+                        for (int f = RD; f < RW && !logical; ++f) {
+                            u32 x = fx ? fx[f] : 0u;
+                            for (u64 mm = lmask; mm; mm &= mm - 1) x ^= slow_tbl[(lbase + __builtin_ctzll(mm)) * RW + f];
+                            for (u64 mm = sm_; mm; mm &= mm - 1) x ^= slow_tbl[(sbase + __builtin_ctzll(mm)) * RW + f];
+                            logical = x != 0;
+                        }
+                        if (logical) mine = w;
+                    }
+                }
+                const int wm = uni(wave_min(mine));
+                if (wm < bound) {
+                    bound = wm;
+                    if (lane == 0) atomicMin(J->best, wm);
+                    thrf = (float)min(bound, 127);
+                }
+                if (bound <= floor) break;
+            }
+        }
+    }
+    if (lane == 0 && slow_entries) atomicAdd(a.slow_hits, (unsigned long long)slow_entries);
+    if (lane == 0 && medium_entries) atomicAdd(a.slow_hits + 1, (unsigned long long)medium_entries);
+}
+
+// =====================================================================================
 // bz_rowsweep: full rows, lexicographic runs (small levels, K > 128, collect mode)
 // =====================================================================================
 
@@ -801,7 +1195,8 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
-    hipEvent_t gev[2 * 11] = {};         // start/stop per kernel-variant launch of a tiled run
+    static constexpr int kVariants = 14;  // kernel variants of a tiled run (vtab in run_tiled)
+    hipEvent_t gev[2 * kVariants] = {};   // start/stop per kernel-variant launch of a tiled run
     DevBuf<u64> gbinom;                  // (kMaxTiledK + 1) x (kMaxD + 1) saturating C(n, k) for the tiled kernel
     DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
     unsigned char* pin = nullptr;        // its pinned host staging
@@ -1250,7 +1645,9 @@ struct Engine {
     // available) absorbs the extra light codewords of structured codes (stabilizer
     // elements), which would otherwise reach the slow path. thr = bound - (flagged rows
     // that any d-subset must contain).
-    struct Variant { int fd, xd, rpl; };
+    // mm: 0 VALU hot test (bz_tiled<fd, xd, rpl>), 1 int8 products (bz_mfma<xd>), 2 / 3 fp4
+    // products of 32 x 32 / 16 x 16 codewords (bz_f4<xd, mm>); the matrix tests have fd = 0.
+    struct Variant { int fd, xd, rpl, mm; };
     static double binom_cdf_below(int bits, long thr) {   // P(Binomial(bits, 1/2) < thr)
         if (thr <= 0) return 0.0;
         if (thr > bits) return 1.0;
@@ -1258,17 +1655,22 @@ struct Engine {
         for (long k = 0; k < thr; ++k) { cdf += pmf; pmf = pmf * (bits - k) / (k + 1); }
         return cdf;
     }
-    // fd == 0 marks the int8-MFMA hot test (bz_mfma<xd>; rpl = 4 gives its 256-element lane tile)
+    // Matrix-core tests (fd == 0): rpl = lane elements per wave / 64.
     static Variant choose_variant(const Solve& s, const qsf_enum_plan& p, bool allow_mfma = true) {
-        static const Variant table[] = {{1, 1, 32}, {1, 2, 16}, {2, 2, 16}, {2, 4, 12}, {3, 4, 12}, {4, 4, 12},
-                                        {1, 2, 24}, {1, 2, 32}, {0, 1, 4},  {0, 2, 4},  {0, 4, 4}};
-        // the MFMA test compares its products with min(bound, 127)
+        static const Variant table[] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {2, 2, 16, 0}, {2, 4, 12, 0}, {3, 4, 12, 0},
+                                        {4, 4, 12, 0}, {1, 2, 24, 0}, {1, 2, 32, 0}, {0, 1, 4, 1},  {0, 2, 4, 1},
+                                        {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3}};
+        // the matrix tests compare their products with min(bound, 127); their padding
+        // elements stay above any bound <= 100
         allow_mfma = allow_mfma && p.current_best <= 100;
-        if (const char* env = std::getenv("QSA_TILED_VARIANT")) {   // tuning: "fd,xd,rpl"
-            int f = 0, x = 0, r = 0;
-            if (std::sscanf(env, "%d,%d,%d", &f, &x, &r) == 3)
+        if (const char* env = std::getenv("QSA_TILED_VARIANT")) {   // tuning: "fd,xd,rpl[,mm]"
+            int f = 0, x = 0, r = 0, m = -1;
+            const int got = std::sscanf(env, "%d,%d,%d,%d", &f, &x, &r, &m);
+            if (got >= 3)
                 for (const Variant& v : table)
-                    if (v.fd == f && v.xd == x && v.rpl == r && (x <= s.RD || x == 4) && (f > 0 || allow_mfma)) return v;
+                    if (v.fd == f && v.xd == x && v.rpl == r && (got == 3 || v.mm == m) && (x <= s.RD || x == 4) &&
+                        (v.mm == 0 || allow_mfma) && (v.mm < 2 || s.RD >= (v.mm == 2 ? 2 : 3)))
+                        return v;
         }
         if (const char* env = std::getenv("QSA_TILED_FD")) {   // tuning/testing override
             int f = std::atoi(env);
@@ -1277,15 +1679,29 @@ struct Engine {
             if (f == 3) return table[4];
             if (f == 4) return table[5];
         }
+        // The matrix tests run 4-wave workgroups over tiles of 1024 (512) lane elements x 512
+        // (256) stream elements: below ~2^33 codewords a level leaves most of the chip idle
+        // or pads heavily, and the VALU kernel's finer tiles are faster.
+        const char* mw = std::getenv("QSA_MATRIX_MIN_WORK");   // tuning/testing override
+        const u64 min_work = mw ? (u64)std::strtoull(mw, nullptr, 10) : (1ull << 33);
+        allow_mfma = allow_mfma && sat_mul(s.C(p.K, p.d), (u64)p.num_gamma) >= min_work;
         if (allow_mfma && !std::getenv("QSA_NO_MFMA")) {
-            // 30 residual bits + the flagged counts per codeword; a 32 x 256 group of codewords
-            // on fair-coin bits should reach the exact tests less than 2% of the time
+            // 30 residual bits + the flagged counts per codeword. Structured codes put far
+            // more light codewords on those bits than fair coins would (the gross code: ~100x
+            // at d = 9, 10), so the int8 test is taken only where a 32 x 256 group of
+            // codewords on fair-coin bits would reach the exact tests < 0.1% of the time.
             int min_off = p.d;
             for (int g = 0; g < p.num_gamma; ++g) min_off = std::min(min_off, std::max(0, p.d - (p.K - s.nflag[g])));
             const long thr = (long)p.current_best - min_off;
             const int res = std::max(1, s.maxres);
-            if (8192.0 * binom_cdf_below(std::min(30, res), thr) < 0.02)
+            if (8192.0 * binom_cdf_below(std::min(30, res), thr) < 0.001)
                 return s.RD >= 3 ? table[10] : s.RD == 2 ? table[9] : table[8];
+            // fp4 32x32x64 products: 60 test bits per 8192-codeword group (< 2%). The 16x16x128
+            // form (124 bits, table[13]) is not chosen automatically: its 256-element stream
+            // chunks pay the per-chunk bound reload and barrier four times as often, and on
+            // the quasi-cyclic evaluation codes it ran at half the VALU kernel's rate.
+            if (!std::getenv("QSA_NO_F4") && s.RD >= 2 && 8192.0 * binom_cdf_below(std::min(60, res), thr) < 0.02)
+                return s.RD >= 3 ? table[12] : table[11];
         }
         if (s.RD == 1) return table[0];
         int min_off = p.d;
@@ -1311,6 +1727,7 @@ struct Engine {
         const Solve& s;
         int g, RW, S;
         u64 lane_tile;
+        bool matrix;   // a matrix-core test: orient each product by its padded work
         std::vector<Rect> rv;
         std::vector<u32> fixed;
         static constexpr u64 kExpandBelow = 64;
@@ -1323,7 +1740,7 @@ struct Engine {
             auto padded = [&](u64 nl, u64 ns) {
                 return (double)((nl + lane_tile - 1) / lane_tile) * (double)lane_tile * (double)((ns + S - 1) / S) * (double)S;
             };
-            const bool x_lanes = lane_tile == (u64)kMfmaLane ? padded(nx, ny) <= padded(ny, nx) : nx >= ny;
+            const bool x_lanes = matrix ? padded(nx, ny) <= padded(ny, nx) : nx >= ny;
             r.lbase = x_lanes ? xb : yb; r.lm = x_lanes ? xm : ym; r.lk = x_lanes ? xk : yk;
             r.sbase = x_lanes ? yb : xb; r.sm = x_lanes ? ym : xm; r.sk = x_lanes ? yk : xk;
             r.n_lane = x_lanes ? nx : ny;
@@ -1403,13 +1820,18 @@ struct Engine {
     // one persistent launch per variant (tiles pulled from the group's counter), one
     // read-back of counters and bounds. res[i] = the bound job i ended with.
     void run_tiled(std::vector<TItem>& items, bool collect, int keep, u64 cap, std::vector<int>& res) {
-        constexpr int NV = 11;
-        static const Variant vtab[NV] = {{1, 1, 32}, {1, 2, 16}, {1, 2, 24}, {1, 2, 32}, {2, 2, 16}, {2, 4, 12},
-                                         {3, 4, 12}, {4, 4, 12}, {0, 1, 4},  {0, 2, 4},  {0, 4, 4}};
+        constexpr int NV = kVariants;
+        static const Variant vtab[NV] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {1, 2, 24, 0}, {1, 2, 32, 0}, {2, 2, 16, 0},
+                                         {2, 4, 12, 0}, {3, 4, 12, 0}, {4, 4, 12, 0}, {0, 1, 4, 1},  {0, 2, 4, 1},
+                                         {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3}};
         auto vidx = [&](const Variant& v) {
             for (int k = 0; k < NV; ++k)
-                if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl) return k;
+                if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl && vtab[k].mm == v.mm) return k;
             return 7;
+        };
+        auto stream_len = [](const Variant& v) { return v.mm == 0 ? kStreamMax : v.mm == 1 ? kMfmaStream : f4_stream(v.mm); };
+        auto lane_tile = [](const Variant& v) {
+            return v.mm == 0 ? 64ull * v.rpl : v.mm == 1 ? (u64)kMfmaLane : (u64)f4_lane_tile(v.mm);
         };
         const int ni = (int)items.size();
         std::vector<std::vector<Rect>> grect(NV);
@@ -1421,8 +1843,7 @@ struct Engine {
             const Solve& s = *it.s;
             const int K = it.p.K, d = it.p.d, h = s.h;
             const int vi = vidx(it.V);
-            RectBuilder B{s, 0, s.RW, it.V.fd == 0 ? kMfmaStream : kStreamMax,
-                          it.V.fd == 0 ? (u64)kMfmaLane : 64ull * it.V.rpl, {}, {}};
+            RectBuilder B{s, 0, s.RW, stream_len(it.V), lane_tile(it.V), it.V.mm > 0, {}, {}};
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
                 for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
@@ -1445,7 +1866,7 @@ struct Engine {
         for (int v = 0; v < NV; ++v) {
             u64 chunks = 0;
             for (const Rect& r : grect[v]) chunks += r.lane_blocks * r.stream_chunks;
-            const u64 resident = vtab[v].fd == 0 ? mfma_groups : waves;
+            const u64 resident = vtab[v].mm > 0 ? mfma_groups : waves;
             gG[v] = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * resident)));
             u64 tiles = 0;
             for (Rect& r : grect[v]) {
@@ -1509,7 +1930,7 @@ struct Engine {
             a.total_tiles = gtiles[v];
             a.binom = gbinom.p;
             a.bk = kMaxD + 1;
-            a.S = V.fd == 0 ? kMfmaStream : kStreamMax;
+            a.S = stream_len(V);
             a.G = gG[v];
             a.lds_full = glds[v];
             a.next_tile = cnt + 4 * v;
@@ -1519,8 +1940,8 @@ struct Engine {
             a.out_rows = cnt + 4 * v + 3;
             a.out = out.p;
             a.out_cap = cap;
-            const u64 grid = std::min<u64>(gtiles[v], V.fd == 0 ? mfma_groups : waves);
-            const dim3 gd((unsigned)grid), bd(V.fd == 0 ? 64 * kMfmaWaves : 64);
+            const u64 grid = std::min<u64>(gtiles[v], V.mm > 0 ? mfma_groups : waves);
+            const dim3 gd((unsigned)grid), bd(V.mm > 0 ? 64 * kMfmaWaves : 64);
             const size_t lds = (size_t)glds[v] * sizeof(u32);
             HIPCHK(hipEventRecord(gev[2 * v], stream));
             switch (v) {
@@ -1534,7 +1955,10 @@ struct Engine {
                 case 7: hipLaunchKernelGGL((bz_tiled<4, 4, 12>), gd, bd, lds, stream, a); break;
                 case 8: hipLaunchKernelGGL((bz_mfma<1>), gd, bd, lds, stream, a); break;
                 case 9: hipLaunchKernelGGL((bz_mfma<2>), gd, bd, lds, stream, a); break;
-                default: hipLaunchKernelGGL((bz_mfma<4>), gd, bd, lds, stream, a); break;
+                case 10: hipLaunchKernelGGL((bz_mfma<4>), gd, bd, lds, stream, a); break;
+                case 11: hipLaunchKernelGGL((bz_f4<2, 2>), gd, bd, lds, stream, a); break;
+                case 12: hipLaunchKernelGGL((bz_f4<4, 2>), gd, bd, lds, stream, a); break;
+                default: hipLaunchKernelGGL((bz_f4<4, 3>), gd, bd, lds, stream, a); break;
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(gev[2 * v + 1], stream));
@@ -1542,9 +1966,11 @@ struct Engine {
             stats.fd = V.fd;
             stats.xd = V.xd;
             stats.rpl = V.rpl;
-            // VALU lane-ops per codeword of the hot loop; the int8-MFMA test spends 0.5
-            // (v_max3 over its products) besides 32 int8 multiply-adds on the matrix cores
-            stats.ops_per_codeword = V.fd == 0 ? 0.5 : (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
+            stats.mfma = V.mm;
+            // VALU lane-ops per codeword of the hot loop; the matrix tests spend 0.5 (v_min3
+            // over their products) besides the products themselves (32 int8 multiply-adds per
+            // codeword, or 64 / 128 fp4 ones)
+            stats.ops_per_codeword = V.mm > 0 ? 0.5 : (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
         }
         HIPCHK(hipMemcpyAsync(pin, dev, o_jobs, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -1560,7 +1986,12 @@ struct Engine {
             ms_total += ms;
             stats.slow_hits += hc[4 * v + 1];
             stats.medium_hits += hc[4 * v + 2];
-            out_rows_last += hc[4 * v + 3];
+            if (vtab[v].mm >= 2) {   // the fp4 kernels count launches that met another output layout here
+                if (hc[4 * v + 3] != 0)
+                    throw HipError{3, "fp4 matrix products: output layout differs from the one the hot test assumes"};
+            } else {
+                out_rows_last += hc[4 * v + 3];
+            }
         }
         stats.kernel_ms = ms_total;
     }

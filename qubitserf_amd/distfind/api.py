@@ -139,7 +139,7 @@ def version() -> str:
 def last_trace():
     """Per-level records of the last solve on this thread, as dicts with the fields of
     DistfindLevel (include/distfind_capi.h): side, d, active, ngamma, K, inner, outer,
-    kernel, fd, xd, rpl, codewords, medium_hits, slow_hits, ms, kernel_ms, ops_per_codeword."""
+    kernel, fd, xd, rpl, mfma, codewords, medium_hits, slow_hits, ms, kernel_ms, ops_per_codeword."""
     L = _native.load()
     n = L.distfind_last_trace(None, 0)
     buf = (_native.DistfindLevel * max(n, 1))()
@@ -147,7 +147,7 @@ def last_trace():
     out = []
     for i in range(n):
         r = buf[i]
-        rec = {f: getattr(r, f) for f, _ in _native.DistfindLevel._fields_ if f != "pad"}
+        rec = {f: getattr(r, f) for f, _ in _native.DistfindLevel._fields_}
         rec["side"] = chr(rec["side"])
         out.append(rec)
     return out

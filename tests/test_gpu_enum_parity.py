@@ -191,13 +191,62 @@ def test_mfma_hot_test(hip, xd, monkeypatch):
         assert hip.last_stats()["fd"] > 0
 
 
-def test_mfma_chosen_on_large_levels(hip):
-    """Automatic choice: a large level with a good bound runs the MFMA test; results equal
-    the oracle's and the VALU kernel's."""
+def test_mfma_chosen_on_large_levels(hip, monkeypatch):
+    """Automatic choice: a level with a tight bound runs the int8 test; results equal the
+    oracle's. (The matrix tests are taken only on levels of >= 2^33 codewords; the size
+    rule is lowered here so that the oracle can check the level.)"""
+    monkeypatch.setenv("QSA_MATRIX_MIN_WORK", "0")
     rng = np.random.default_rng(3)
     gamma, check = rand_plan(rng, 80, 3, 1, 2, systematic=True)
     n = 192
-    want = O.enumerate_plan(gamma, n, 5, check=check, current_best=8)
-    got = hip.enumerate_level(gamma, n, 5, check=check, current_best=8)
-    assert hip.last_stats()["fd"] == 0 and got == want
+    want = O.enumerate_plan(gamma, n, 5, check=check, current_best=7)
+    got = hip.enumerate_level(gamma, n, 5, check=check, current_best=7)
+    st = hip.last_stats()
+    assert st["fd"] == 0 and st["mfma"] == 1 and got == want, st
+    monkeypatch.delenv("QSA_MATRIX_MIN_WORK")
+    got = hip.enumerate_level(gamma, n, 5, check=check, current_best=7)
+    assert hip.last_stats()["mfma"] == 0 and got == want   # C(80, 5) is below the size rule
 
+
+
+@pytest.mark.parametrize("mm,xd,stride", [(2, 2, 2), (2, 4, 3), (3, 4, 3)])
+def test_f4_hot_test(hip, mm, xd, stride, monkeypatch):
+    """The fp4 matrix-core hot tests (bz_f4: 32x32x64 products with 60 test bits, 16x16x128
+    with 124) on random and systematic generators, loose and tight bounds, against the
+    oracle. Systematic generators flag every row (the lane and stream constants are the
+    subset sizes); random ones flag few (the constants vary per element)."""
+    rpl = 4 if mm == 2 else 2
+    monkeypatch.setenv("QSA_TILED_VARIANT", f"0,{xd},{rpl},{mm}")
+    rng = np.random.default_rng(1300 + 10 * mm + xd)
+    kmax = 64 if stride == 2 else 120
+    with hip.kernel_policy("tiled"):
+        for i in range(12):
+            K = int(rng.integers(24, kmax + 1))
+            d = int(rng.integers(3, 6)) if K < 90 else 4
+            kcheck = int(rng.integers(0, 4))
+            best = [100, 70, 45, 30, 22, 14, 9, 60, 35, 25, 18, 5][i]
+            run_case(hip, rng, K, stride, d, 1 + i % 2, kcheck, best, systematic=bool(i % 3))
+            st = hip.last_stats()
+            assert st["fd"] == 0 and st["mfma"] == mm, st
+        # above 100 the VALU kernel takes over
+        run_case(hip, rng, 60, 3, 4, 1, 1, NONE, systematic=True)
+        assert hip.last_stats()["mfma"] == 0
+
+
+def test_f4_chosen_on_high_thresholds(hip, monkeypatch):
+    """Automatic choice: levels whose bound leaves too many light codewords on 30 test bits
+    run the fp4 32x32x64 test (60 bits), looser ones the VALU kernel; results equal the
+    oracle's. The size rule (>= 2^33 codewords) is lowered so that the oracle can check the
+    levels."""
+    monkeypatch.setenv("QSA_MATRIX_MIN_WORK", "0")
+    rng = np.random.default_rng(31)
+    seen = set()
+    for K, d, best in [(80, 5, 14), (72, 4, 16), (64, 4, 30), (90, 4, 40)]:
+        gamma, check = rand_plan(rng, K, 3, 1, 2, systematic=True)
+        n = 192
+        want = O.enumerate_plan(gamma, n, d, check=check, current_best=best)
+        got = hip.enumerate_level(gamma, n, d, check=check, current_best=best)
+        st = hip.last_stats()
+        assert got == want, (K, d, best, got, want, st)
+        seen.add(st["mfma"])
+    assert seen == {0, 2}, seen
