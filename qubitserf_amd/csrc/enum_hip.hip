@@ -931,13 +931,17 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                     bnext = sb[(gi + 1) * NR];
                     onext = soff[(gi + 1) * NR + row];
                 }
-                float mn = 3.0e38f;
+                // The minimum over the products' bit patterns: non-negative floats order like
+                // their bits as integers (v_min3_i32, no NaN canonicalisation); a negative value
+                // (only from a clamped lane constant) becomes a negative integer and sends the
+                // group to the exact tests, which compare in f32.
+                int mn = 0x7fffffff;
                 if constexpr (NV == 16) {
                     auto red = [](const acc& c) {
-                        float m = fminf(fminf(c[0], c[1]), c[2]);
+                        int m = min(min(__float_as_int(c[0]), __float_as_int(c[1])), __float_as_int(c[2]));
 #pragma unroll
-                        for (int v = 3; v < 15; v += 2) m = fminf(fminf(m, c[v]), c[v + 1]);
-                        return fminf(m, c[15]);
+                        for (int v = 3; v < 15; v += 2) m = min(min(m, __float_as_int(c[v])), __float_as_int(c[v + 1]));
+                        return min(m, __float_as_int(c[15]));
                     };
 #pragma unroll
                     for (int b0 = 0; b0 < kMfmaBlocks; b0 += 4) {
@@ -945,7 +949,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                         const acc p1 = P::mm(av[b0 + 1], bv);
                         const acc p2 = P::mm(av[b0 + 2], bv);
                         const acc p3 = P::mm(av[b0 + 3], bv);
-                        mn = fminf(mn, fminf(fminf(red(p0), red(p1)), fminf(red(p2), red(p3))));
+                        mn = min(mn, min(min(red(p0), red(p1)), min(red(p2), red(p3))));
                     }
                 } else {
                     acc p[kMfmaBlocks];
@@ -953,9 +957,11 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                     for (int b = 0; b < kMfmaBlocks; ++b) p[b] = P::mm(av[b], bv);
 #pragma unroll
                     for (int b = 0; b < kMfmaBlocks; ++b)
-                        mn = fminf(fminf(mn, fminf(p[b][0], p[b][1])), fminf(p[b][2], p[b][3]));
+                        mn = min(min(mn, min(__float_as_int(p[b][0]), __float_as_int(p[b][1]))),
+                                 min(__float_as_int(p[b][2]), __float_as_int(p[b][3])));
                 }
-                if (__builtin_expect(__ballot(mn < thr) == 0ull, 1)) continue;
+                // thr < 0 (a padded column) has a negative pattern below every non-negative one
+                if (__builtin_expect(__ballot(mn < __float_as_int(thr)) == 0ull, 1)) continue;
                 // ---- exact tests of this group: the products are formed again, each lane
                 // marks its values below its threshold (bit NV b + v), then tests those
                 // (lane element, stream element) pairs; both elements are then real
