@@ -15,6 +15,12 @@
 //   qsa_calib mfma16_and      the packed hot-test pattern: per step 8 16x16x64 products
 //                             (bias accumulator) AND-reduced, two codewords per entry;
 //                             prints codewords/s.
+//   qsa_calib f4              fp4 matrix-core rate: 8 independent chains of
+//                             v_mfma_scale_f32_32x32x64_f8f6f4 (fp4 operands, unit scales);
+//                             prints fp4 ops/s (2 per multiply-add).
+//   qsa_calib f4_max          the fp4 hot-test pattern: per step 8 such products (zero
+//                             accumulator) reduced with v_min3_i32 over their bit patterns;
+//                             prints products/s (1024 codewords each).
 //   An optional last argument sets the waves per CU of the mfma probes (default 12).
 #include <hip/hip_runtime.h>
 
@@ -131,6 +137,57 @@ __global__ __launch_bounds__(64) void mfma16_and(int* out, int iters) {
     out[blockIdx.x * 64 + l] = hit;
 }
 
+typedef int v8i_ __attribute__((ext_vector_type(8)));
+typedef float v16f_ __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ v16f_ f4mm(v8i_ a, v8i_ b, v16f_ c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, 127, 0, 127);
+}
+
+__global__ __launch_bounds__(64) void f4_chain(float* out, int iters) {
+    const int l = threadIdx.x;
+    v8i_ a[8], b = {l * 0x12345, l * 0x2233, l * 5, l * 7, 0, 0, 0, 0};
+    v16f_ c[8];
+    for (int k = 0; k < 8; ++k) {
+        a[k] = v8i_{(l + k) * 0x1111, (l ^ k) * 0x2222, l * k, l - k, 0, 0, 0, 0};
+        c[k] = v16f_{};
+    }
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c[k] = f4mm(a[k], b, c[k]);
+    float r = 0;
+    for (int k = 0; k < 8; ++k)
+        for (int v = 0; v < 16; ++v) r += c[k][v];
+    out[blockIdx.x * 64 + l] = r;
+}
+
+__global__ __launch_bounds__(64) void f4_max(int* out, int iters) {
+    const int l = threadIdx.x;
+    v8i_ a[8], b = {0x22222222, 0x22222222 ^ (l << 3), 0x2a2a2a2a, 0x22a222a2, 0, 0, 0, 0};
+    for (int k = 0; k < 8; ++k) a[k] = v8i_{0x22222222 ^ (k << 7), 0x2a2a2a2a, 0x22222222 ^ (l << 11), 0x22222222, 0, 0, 0, 0};
+    int hit = 0;
+    for (int i = 0; i < iters; ++i) {
+        b[0] ^= (i & 7) << 3;
+        int mn = 0x7fffffff;
+        const v16f_ z = {};
+#pragma unroll
+        for (int k0 = 0; k0 < 8; k0 += 4) {
+            const v16f_ p0 = f4mm(a[k0 + 0], b, z);
+            const v16f_ p1 = f4mm(a[k0 + 1], b, z);
+            const v16f_ p2 = f4mm(a[k0 + 2], b, z);
+            const v16f_ p3 = f4mm(a[k0 + 3], b, z);
+            auto red = [](const v16f_& c) {
+                int m = min(min(__float_as_int(c[0]), __float_as_int(c[1])), __float_as_int(c[2]));
+#pragma unroll
+                for (int v = 3; v < 15; v += 2) m = min(min(m, __float_as_int(c[v])), __float_as_int(c[v + 1]));
+                return min(m, __float_as_int(c[15]));
+            };
+            mn = min(mn, min(min(red(p0), red(p1)), min(red(p2), red(p3))));
+        }
+        hit += __ballot(mn < -1000) != 0ull;
+    }
+    out[blockIdx.x * 64 + l] = hit;
+}
+
 __global__ __launch_bounds__(256) void copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -184,6 +241,30 @@ int main(int argc, char** argv) {
     }
     const bool m32c = std::strcmp(argv[1], "mfma") == 0, m32r = std::strcmp(argv[1], "mfma_max") == 0;
     const bool m16c = std::strcmp(argv[1], "mfma16") == 0, m16r = std::strcmp(argv[1], "mfma16_and") == 0;
+    const bool f4c = std::strcmp(argv[1], "f4") == 0, f4r = std::strcmp(argv[1], "f4_max") == 0;
+    if (f4c || f4r) {
+        const int per_cu = argc >= 3 ? std::atoi(argv[2]) : 12;
+        const int blocks = 256 * per_cu, iters = 2000;
+        void* out;
+        CHECK(hipMalloc(&out, (size_t)blocks * 64 * sizeof(float)));
+        auto launch = [&](int it) {
+            if (f4c) hipLaunchKernelGGL(f4_chain, dim3(blocks), dim3(64), 0, 0, (float*)out, it);
+            else hipLaunchKernelGGL(f4_max, dim3(blocks), dim3(64), 0, 0, (int*)out, it);
+        };
+        launch(10);
+        CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(e0, 0));
+        launch(iters);
+        CHECK(hipEventRecord(e1, 0));
+        CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double mfmas = (double)blocks * iters * 8;
+        std::printf("{\"probe\": \"%s\", \"waves_per_cu\": %d, \"ms\": %.3f, \"fp4_ops_per_s\": %.4e, \"products_per_s\": %.4e}\n",
+                    argv[1], per_cu, ms, mfmas * 65536.0 * 2 / (ms * 1e-3), mfmas * 1024 / (ms * 1e-3));
+        CHECK(hipFree(out));
+        return 0;
+    }
     if (m32c || m32r || m16c || m16r) {
         const int per_cu = argc >= 3 ? std::atoi(argv[2]) : 12;
         const int blocks = 256 * per_cu, iters = (m32c || m16c) ? 4000 : 2000;

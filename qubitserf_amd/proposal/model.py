@@ -138,31 +138,33 @@ def pack_batch_bits(Hs, seeds=None):
     """pack_batch computed for the whole batch at once, with the two averaging operators
     built directly as bfloat16 bit patterns (uint16): an entry of Ac is 1 / deg(check) and
     of Av 1 / deg(column), so both come from one table of the patterns of 1 / k. Equal, bit
-    for bit, to pack_batch with bf16_bits applied to its float32 operators."""
+    for bit, to pack_batch with bf16_bits applied to its float32 operators (the degree sums
+    are small integers, exact in float32 in any summation order)."""
     from .devarray import bf16_bits
 
     B = len(Hs)
     m = max(h.shape[0] for h in Hs)
     n = max(h.shape[1] for h in Hs)
-    Hb = np.zeros((B, m, n), bool)
+    Hb = np.zeros((B, m, n), np.uint8)
     hms = np.zeros(B, np.int64)
     hns = np.zeros(B, np.int64)
     for b, H in enumerate(Hs):
         H = np.asarray(H)
-        Hb[b, : H.shape[0], : H.shape[1]] = (H.astype(np.uint8) & 1).astype(bool)
+        Hb[b, : H.shape[0], : H.shape[1]] = H.astype(np.uint8, copy=False) & 1
         hms[b], hns[b] = H.shape
     mask = (np.arange(n)[None, :] < hns[:, None]).astype(np.float32)
-    dc_raw = Hb.sum(2).astype(np.float32)                         # (B, m)
-    dv_raw = Hb.sum(1).astype(np.float32)                         # (B, n)
+    dc_raw = Hb.sum(2, dtype=np.int32).astype(np.float32)           # (B, m)
+    dv_raw = Hb.sum(1, dtype=np.int32).astype(np.float32)           # (B, n)
     inv = bf16_bits(np.float32(1.0) / np.arange(1, max(m, n) + 1, dtype=np.float32))
-    dc = np.maximum(dc_raw, 1).astype(np.int64)
-    dv = np.maximum(dv_raw, 1).astype(np.int64)
-    Ac = np.where(Hb, inv[dc - 1][:, :, None], np.uint16(0))
-    Av = np.where(Hb.transpose(0, 2, 1), inv[dv - 1][:, :, None], np.uint16(0))
+    pc = inv[np.maximum(dc_raw, 1).astype(np.int64) - 1]          # (B, m) pattern of 1 / deg_c
+    pv = inv[np.maximum(dv_raw, 1).astype(np.int64) - 1]          # (B, n) pattern of 1 / deg_v
+    # H is 0 / 1, so H x pattern is the pattern or 0: Ac = H / deg_c, Av = H^T / deg_v
+    Ac = Hb * pc[:, :, None]
+    Av = np.ascontiguousarray(Hb.transpose(0, 2, 1)) * pv[:, :, None]
     X = np.zeros((B, n, F), np.float32)
     X[:, :, 0] = mask
     X[:, :, 1] = dv_raw / np.maximum(np.float32(1.0), dv_raw.max(1, keepdims=True))
-    mean_dc = (Hb * dc_raw[:, :, None]).sum(1, dtype=np.float32) / np.maximum(dv_raw, np.float32(1.0))
+    mean_dc = np.einsum("bm,bmn->bn", dc_raw, Hb, dtype=np.float32) / np.maximum(dv_raw, np.float32(1.0))
     X[:, :, 2] = mean_dc / np.maximum(np.float32(1.0), dc_raw.max(1, keepdims=True))
     X[:, :, 3] = (hms / hns).astype(np.float32)[:, None] * mask
     for b in range(B):
