@@ -1630,9 +1630,28 @@ struct Engine {
             stats.codewords = sat_add(stats.codewords, cw);
             if (tileable(s, p) && policy != 1) {
                 item_of[j] = (int)items.size();
-                items.push_back(TItem{&s, p, floors ? floors[j] : -1, choose_variant(s, p)});
+                items.push_back(TItem{&s, p, floors ? floors[j] : -1, choose_variant(s, p, true, matrix_min_work() >> 6)});
             } else {
                 sweep.push_back(j);
+            }
+        }
+        {   // a matrix variant whose levels together stay under the single-level minimum
+            // goes back to the VALU kernel
+            std::vector<std::pair<Variant, u64>> tot;
+            for (const TItem& it : items) {
+                if (it.V.mm == 0) continue;
+                const u64 cw = sat_mul(it.s->C(it.p.K, it.p.d), (u64)it.p.num_gamma);
+                auto f = std::find_if(tot.begin(), tot.end(), [&](const std::pair<Variant, u64>& e) {
+                    return e.first.mm == it.V.mm && e.first.xd == it.V.xd && e.first.rpl == it.V.rpl;
+                });
+                if (f == tot.end()) tot.push_back({it.V, cw}); else f->second = sat_add(f->second, cw);
+            }
+            for (TItem& it : items) {
+                if (it.V.mm == 0) continue;
+                for (const auto& e : tot)
+                    if (e.first.mm == it.V.mm && e.first.xd == it.V.xd && e.first.rpl == it.V.rpl &&
+                        e.second < matrix_min_work())
+                        it.V = choose_variant(*it.s, it.p, false);
             }
         }
         double ms_total = 0;
@@ -1678,8 +1697,20 @@ struct Engine {
         for (long k = 0; k < thr; ++k) { cdf += pmf; pmf = pmf * (bits - k) / (k + 1); }
         return cdf;
     }
-    // Matrix-core tests (fd == 0): rpl = lane elements per wave / 64.
-    static Variant choose_variant(const Solve& s, const qsf_enum_plan& p, bool allow_mfma = true) {
+    // The matrix tests run 4-wave workgroups over tiles of 1024 (512) lane elements x 512
+    // (256) stream elements: a launch of less than ~2^33 codewords leaves most of the chip
+    // idle or pads heavily, and the VALU kernel's finer tiles are faster. A single level
+    // must hold that much work itself; in a batched run the levels of one variant share a
+    // launch, so levels of >= 2^27 codewords qualify when their variant's launch reaches
+    // 2^33 in total (enumerate_batch).
+    static u64 matrix_min_work() {
+        const char* mw = std::getenv("QSA_MATRIX_MIN_WORK");   // tuning/testing override
+        return mw ? (u64)std::strtoull(mw, nullptr, 10) : (1ull << 33);
+    }
+    // Matrix-core tests (fd == 0): rpl = lane elements per wave / 64. min_work: smallest
+    // level (codewords) a matrix test may take (default matrix_min_work()).
+    static Variant choose_variant(const Solve& s, const qsf_enum_plan& p, bool allow_mfma = true,
+                                  u64 min_work = ~0ull) {
         static const Variant table[] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {2, 2, 16, 0}, {2, 4, 12, 0}, {3, 4, 12, 0},
                                         {4, 4, 12, 0}, {1, 2, 24, 0}, {1, 2, 32, 0}, {0, 1, 4, 1},  {0, 2, 4, 1},
                                         {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3}};
@@ -1702,11 +1733,7 @@ struct Engine {
             if (f == 3) return table[4];
             if (f == 4) return table[5];
         }
-        // The matrix tests run 4-wave workgroups over tiles of 1024 (512) lane elements x 512
-        // (256) stream elements: below ~2^33 codewords a level leaves most of the chip idle
-        // or pads heavily, and the VALU kernel's finer tiles are faster.
-        const char* mw = std::getenv("QSA_MATRIX_MIN_WORK");   // tuning/testing override
-        const u64 min_work = mw ? (u64)std::strtoull(mw, nullptr, 10) : (1ull << 33);
+        if (min_work == ~0ull) min_work = matrix_min_work();
         allow_mfma = allow_mfma && sat_mul(s.C(p.K, p.d), (u64)p.num_gamma) >= min_work;
         if (allow_mfma && !std::getenv("QSA_NO_MFMA")) {
             // 30 residual bits + the flagged counts per codeword. Structured codes put more
