@@ -22,6 +22,8 @@ Prints ONE JSON line. Fields beyond the common contract:
                       test)
   cpu_baseline        the reference's own CPU enumerator (oracle/_ref) on one level of the
                       same workload, on this host's cores
+  config1             time-to-certify of the [64,32] random code of seed 0 on the device and
+                      by the reference's CPU bz on this host (median of repeated solves)
   batched_learned     plain (one code per call, natural information sets) vs batched +
                       learned (one call, trained scorer's proposals) on 256 random codes
                       (n 64..160, rate 0.3..0.6) and 16 quasi-cyclic codes (L 20..40), under
@@ -83,6 +85,42 @@ def cpu_baseline(threads: int):
         "sample": f"gross Z side, weight level d={d} of the full-rank information set "
                   f"(C(78,{d}) = {cw:.3e} codewords, bound 12), {dt:.2f} s, result {got}",
     }
+
+
+def config1_gpu(reps=20):
+    """Time-to-certify on the device of the reference's CPU-runnable case: the [64, 32]
+    random code of seed 0 (32 x 64 Bernoulli(1/2) parity check, full rank)."""
+    from qubitserf_amd import codes
+    from qubitserf_amd import distfind as df
+
+    H = codes.config1_parity(0)
+    df.classical_distance(H, backend="gpu")
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d = df.classical_distance(H, backend="gpu")
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return {"code": "[64,32] random, seed 0", "distance": d, "gpu_time_to_certify_s": ts[len(ts) // 2]}
+
+
+def config1_cpu(threads, reps=5):
+    """The reference's CPU bz (oracle/_ref, threads=all given) on the same code; the oracle
+    port when the reference library was not built."""
+    from qubitserf_amd import codes
+    from tests import oracle as O
+
+    H = codes.config1_parity(0)
+    kind = "reference" if O.ref_available() else "port"
+    fn = (lambda: O.ref_classical_distance(H, threads)) if kind == "reference" else (lambda: O.classical_distance(H, threads))
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return {"cpu_kind": kind, "cpu_threads": threads, "cpu_distance": int(r.distance), "cpu_time_to_certify_s": ts[len(ts) // 2]}
 
 
 def _certified(rs):
@@ -252,9 +290,13 @@ def main():
     }
     if not args.no_extras:
         line["batched_learned"] = batched_learned(lambda m: print(m, file=sys.stderr, flush=True))
+        line["config1"] = config1_gpu()
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(min(threads, 16))
+        if "config1" in line:
+            line["config1"].update(config1_cpu(min(threads, 16)))
+            line["config1"]["distances_agree"] = line["config1"]["distance"] == line["config1"]["cpu_distance"]
     print(json.dumps(line), flush=True)
 
 
