@@ -153,6 +153,14 @@ void qsf_hip_last_elim_stats(qsf_hip_elim_stats* out);
  * each. Returns 0; 1 bad arguments; 3 HIP error; 4 a weight split too large for the device
  * table (more than 2^27 parts on the smaller side). */
 int qsf_hip_mitm_weight(const uint64_t* syn, const uint64_t* log, int n, int nl, int d, int count_all, int* hit);
+
+/* The symplectic form (replaces has_codeword_of_weight_sym, cpp/distfind/mitm.cpp:131-237):
+ * *hit = 1 iff some Pauli operator touching exactly d qubits has a zero syndrome and a
+ * nonzero detector fingerprint. syn / log hold 2q columns in [z | x] order as above (a
+ * qubit's Z reads column j, X column q + j, Y both). Qubits [0, ql) form the left half and
+ * [ql, q) the right, at most 64 each; a side of weight k has C(m, k) 3^k parts. Returns as
+ * qsf_hip_mitm_weight. */
+int qsf_hip_mitm_weight_sym(const uint64_t* syn, const uint64_t* log, int q, int ql, int d, int* hit);
 const char* qsf_hip_mitm_last_error(void);
 
 /* Statistics of the last enumerate/collect call on this thread's library instance:

@@ -38,13 +38,12 @@ void fill(DistfindResult* out, const BzResult& r) {
     std::strncpy(out->backend, "hip", sizeof(out->backend) - 1);
 }
 
-// Shared argument policy: method "bz" (and "mitm" for CSS and classical codes); backend
-// "gpu"/"auto" only, device present.
+// Shared argument policy: methods "bz" and "mitm" (classical, CSS and non-CSS codes);
+// backend "gpu"/"auto" only, device present.
 int check_args(const char* method, const char* backend, bool noncss = false) {
     std::string m = method ? method : "bz";
     std::string b = backend ? backend : "auto";
     if (m == "cc" && noncss) { g_err = "connected cluster has no non-CSS form"; return RC_CC_NONCSS; }
-    if (m == "mitm" && noncss) { g_err = "symplectic meet-in-the-middle is not provided by the gfx950 engine"; return RC_METHOD; }
     if (m != "bz" && m != "mitm") { g_err = "method '" + m + "' is not provided by the gfx950 engine (it runs 'bz' and 'mitm')"; return RC_METHOD; }
     if (b != "gpu" && b != "auto") { g_err = "backend '" + b + "' is not provided by the gfx950 engine (use 'gpu')"; return RC_BACKEND; }
     if (!qsf_hip_available()) { g_err = std::string("no usable gfx950 device: ") + qsf_hip_last_error(); return RC_BACKEND; }
@@ -134,6 +133,10 @@ int distfind_stabilizer_distance(const uint8_t* S, int rows, int cols, const cha
             return css_solve(Hx, Hz, which, verbose, out, is_mitm(method));
         }
         BzOptions o = options(verbose);
+        if (is_mitm(method)) {   // symplectic meet in the middle (capi.cpp:212-218 of the reference)
+            fill(out, mitm_distance(symplectic_plain(m), o));
+            return RC_OK;
+        }
         if (o.verbose)
             std::fprintf(stderr, "distfind: non-CSS BZ via the (a|b)->(a|b|a^b) isometry; bounds are Hamming "
                                  "weights of the length-%d code (twice the symplectic weight).\n", 3 * (cols / 2));

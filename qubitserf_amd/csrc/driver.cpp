@@ -703,18 +703,24 @@ namespace {
 struct MitmSide {
     bool has = false, resolved = false, confirmed = false, count_all = false;
     int n = 0, nl = 0, target = 0, seed_upper = 0, found = -1, level = 0;
+    int qubits = 0;   // > 0: symplectic problem; weights and halves count qubits
     const char* tag = "";
     std::vector<u64> syn, log;
 
     void setup(const Problem& p, const char* t) {
         tag = t;
         n = p.n;
+        qubits = p.qubits;
         const int K = p.gen.rows;
         has = !(K == 0 || (!p.every_nonzero && p.detector.rows == 0));
         if (!has) { resolved = confirmed = true; return; }
         const BitMat H = kernel_basis(p.gen);   // the code's parity checks
-        if (H.rows > 63 || p.detector.rows > 64 || n > 128 || n - n / 2 > 64)
-            throw EngineError{4, "mitm on the device needs at most 63 parity checks, 64 detector rows and n <= 128"};
+        const int coords = qubits > 0 ? qubits : n;
+        if (H.rows > 63 || p.detector.rows > 64 || coords > 128 || coords - coords / 2 > 64)
+            throw EngineError{4, qubits > 0 ? "symplectic mitm on the device needs at most 63 parity checks, 64 detector "
+                                              "rows and 128 qubits"
+                                            : "mitm on the device needs at most 63 parity checks, 64 detector rows and "
+                                              "n <= 128"};
         count_all = p.every_nonzero;
         syn.assign(n, 0);
         log.assign(n, 0);
@@ -725,15 +731,16 @@ struct MitmSide {
             for (int c = 0; c < p.detector.rows; ++c)
                 for (int j = 0; j < n; ++j)
                     if (p.detector.bit(c, j)) log[j] |= 1ull << c;
-        nl = n / 2;
+        nl = coords / 2;
         seed_upper = p.seed_upper;
-        target = seed_upper > 0 ? std::min(seed_upper, n) : n;
+        target = seed_upper > 0 ? std::min(seed_upper, coords) : coords;
     }
 
     bool step(int d) {
         ++level;
         int hit = 0;
-        const int rc = qsf_hip_mitm_weight(syn.data(), log.data(), n, nl, d, count_all ? 1 : 0, &hit);
+        const int rc = qubits > 0 ? qsf_hip_mitm_weight_sym(syn.data(), log.data(), qubits, nl, d, &hit)
+                                  : qsf_hip_mitm_weight(syn.data(), log.data(), n, nl, d, count_all ? 1 : 0, &hit);
         if (rc != 0) throw EngineError{rc, qsf_hip_mitm_last_error()};
         if (hit) found = d;
         return hit != 0;

@@ -87,6 +87,44 @@ __device__ void for_parts(const Side& s, const u64* B, F f) {
     }
 }
 
+// Symplectic parts (cpp/distfind/mitm.cpp:131-237): a coordinate is a qubit and the cost
+// is the number of qubits touched. A part of weight k is k qubits of the side, each carrying
+// Z, X or Y; its syndrome (fingerprint) is the XOR over those qubits of the Pauli's column:
+// the z column, the x column, or both (precomputed per qubit). Part i = r 3^k + c: r ranks
+// the qubit subset in colex order, the base-3 digits of c choose the Paulis.
+struct SymSide {
+    const u64* syn3;    // per qubit of this side: syndromes of Z, X, Y
+    const u64* log3;    // per qubit: fingerprints of Z, X, Y
+    int m, k;           // k of m qubits
+    u64 p3;             // 3^k
+    u64 count;          // C(m, k) 3^k
+};
+
+template <class F>
+__device__ void for_parts(const SymSide& s, const u64* B, F f) {
+    const u64 first = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * kRun;
+    if (first >= s.count) return;
+    const u64 last = min(s.count, first + kRun);
+    const u64 r = first / s.p3;
+    u64 c = first - r * s.p3;
+    u64 mask = s.k ? unrank(r, s.k, s.m, B) : 0ull;
+    for (u64 i = first; i < last; ++i) {
+        u64 syn = 0, log = 0, cc = c;
+        for (u64 mm = mask; mm; mm &= mm - 1) {
+            const int q = __builtin_ctzll(mm);
+            const int pauli = (int)(cc % 3);
+            cc /= 3;
+            syn ^= s.syn3[3 * q + pauli];
+            log ^= s.log3[3 * q + pauli];
+        }
+        f(syn, log);
+        if (++c == s.p3) {   // every Pauli choice of this subset done: the next subset
+            c = 0;
+            if (s.k && i + 1 < last) mask = next_subset(mask);
+        }
+    }
+}
+
 __global__ void clear_table(u64* keys, u64* lmin, u64* lmax, u64 cap) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (u64)gridDim.x * blockDim.x) {
         keys[i] = kEmpty;
@@ -95,7 +133,8 @@ __global__ void clear_table(u64* keys, u64* lmin, u64* lmax, u64 cap) {
     }
 }
 
-__global__ void insert_parts(Side s, const u64* B, u64* keys, unsigned long long* lmin, unsigned long long* lmax,
+template <class S>
+__global__ void insert_parts(S s, const u64* B, u64* keys, unsigned long long* lmin, unsigned long long* lmax,
                              u64 cap) {
     for_parts(s, B, [&](u64 syn, u64 log) {
         u64 h = mix(syn) & (cap - 1);
@@ -112,7 +151,8 @@ __global__ void insert_parts(Side s, const u64* B, u64* keys, unsigned long long
     });
 }
 
-__global__ void probe_parts(Side s, const u64* B, const u64* keys, const u64* lmin, const u64* lmax, u64 cap,
+template <class S>
+__global__ void probe_parts(S s, const u64* B, const u64* keys, const u64* lmin, const u64* lmax, u64 cap,
                             int count_all, int* hit) {
     for_parts(s, B, [&](u64 syn, u64 log) {
         u64 h = mix(syn) & (cap - 1);
@@ -234,10 +274,84 @@ int qsf_hip_mitm_weight(const uint64_t* syn, const uint64_t* log, int n, int nl,
         (void)hipEventRecord(e0, nullptr);
         hipLaunchKernelGGL(clear_table, dim3((unsigned)std::min<u64>(4096, (cap + 255) / 256)), dim3(256), 0, nullptr,
                            M.keys, M.lmin, M.lmax, cap);
-        hipLaunchKernelGGL(insert_parts, dim3(blocks_for(ins.count)), dim3(256), 0, nullptr, ins, M.binom, M.keys,
-                           (unsigned long long*)M.lmin, (unsigned long long*)M.lmax, cap);
-        hipLaunchKernelGGL(probe_parts, dim3(blocks_for(prb.count)), dim3(256), 0, nullptr, prb, M.binom, M.keys,
-                           M.lmin, M.lmax, cap, count_all, M.hit);
+        hipLaunchKernelGGL(insert_parts<Side>, dim3(blocks_for(ins.count)), dim3(256), 0, nullptr, ins, M.binom,
+                           M.keys, (unsigned long long*)M.lmin, (unsigned long long*)M.lmax, cap);
+        hipLaunchKernelGGL(probe_parts<Side>, dim3(blocks_for(prb.count)), dim3(256), 0, nullptr, prb, M.binom,
+                           M.keys, M.lmin, M.lmax, cap, count_all, M.hit);
+        (void)hipEventRecord(e1, nullptr);
+        if ((e = hipGetLastError()) != hipSuccess) return fail("launch");
+        if ((e = hipMemcpy(hit_out, M.hit, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return fail("read-back");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        total += ms;
+    }
+    M.ms = total;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+int qsf_hip_mitm_weight_sym(const uint64_t* syn, const uint64_t* log, int q, int ql, int d, int* hit_out) {
+    Mitm& M = mitm();
+    std::lock_guard<std::mutex> lk(M.mu);
+    if (!syn || !log || !hit_out || q < 1 || ql < 0 || ql > q || q - ql > 64 || ql > 64 || d < 0 || d > q) {
+        M.err = "qsf_hip_mitm_weight_sym: bad arguments (each half must have at most 64 qubits)";
+        return 1;
+    }
+    *hit_out = 0;
+    hipError_t e = hipSuccess;
+    M.ms = 0;
+    auto fail = [&](const char* what) {
+        M.err = std::string("qsf_hip_mitm_weight_sym: ") + what + ": " + hipGetErrorString(e);
+        return 3;
+    };
+    hipEvent_t e0, e1;
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) return fail("event");
+    if ((e = M.ensure(0, 3 * q)) != hipSuccess) return fail("allocation");
+    // per qubit j: Z = column j, X = column q + j, Y = both
+    std::vector<u64> host(6 * (size_t)q);
+    for (int j = 0; j < q; ++j) {
+        host[3 * j] = syn[j];
+        host[3 * j + 1] = syn[q + j];
+        host[3 * j + 2] = syn[j] ^ syn[q + j];
+        host[3 * q + 3 * j] = log[j];
+        host[3 * q + 3 * j + 1] = log[q + j];
+        host[3 * q + 3 * j + 2] = log[j] ^ log[q + j];
+    }
+    if ((e = hipMemcpy(M.cols, host.data(), host.size() * 8, hipMemcpyHostToDevice)) != hipSuccess) return fail("copy");
+    if ((e = hipMemset(M.hit, 0, sizeof(int))) != hipSuccess) return fail("memset");
+    auto parts = [](int m, int k, u64& p3) {   // C(m, k) 3^k, saturating; p3 = 3^k
+        p3 = 1;
+        for (int i = 0; i < k; ++i) p3 = p3 > (~0ull) / 3 ? ~0ull : p3 * 3;
+        const u64 c = Mitm::C(m, k);
+        return (c != 0 && p3 > (~0ull) / c) ? ~0ull : c * p3;
+    };
+    const int qr = q - ql;
+    float total = 0;
+    for (int wl = 0; wl <= d && !*hit_out; ++wl) {
+        const int wr = d - wl;
+        if (wl > ql || wr > qr) continue;
+        u64 pl = 0, pr = 0;
+        const u64 cl = parts(ql, wl, pl), cr = parts(qr, wr, pr);
+        SymSide L{M.cols, M.cols + 3 * q, ql, wl, pl, cl};
+        SymSide R{M.cols + 3 * ql, M.cols + 3 * q + 3 * ql, qr, wr, pr, cr};
+        const bool left_small = cl <= cr;
+        const SymSide& ins = left_small ? L : R;
+        const SymSide& prb = left_small ? R : L;
+        if (ins.count > (1ull << 27) || prb.count > (1ull << 40)) {
+            M.err = "qsf_hip_mitm_weight_sym: weight split too large for the device table";
+            return 4;
+        }
+        u64 cap = 1;
+        while (cap < 2 * ins.count) cap <<= 1;
+        if ((e = M.ensure(cap, 3 * q)) != hipSuccess) return fail("table allocation");
+        (void)hipEventRecord(e0, nullptr);
+        hipLaunchKernelGGL(clear_table, dim3((unsigned)std::min<u64>(4096, (cap + 255) / 256)), dim3(256), 0, nullptr,
+                           M.keys, M.lmin, M.lmax, cap);
+        hipLaunchKernelGGL(insert_parts<SymSide>, dim3(blocks_for(ins.count)), dim3(256), 0, nullptr, ins, M.binom,
+                           M.keys, (unsigned long long*)M.lmin, (unsigned long long*)M.lmax, cap);
+        hipLaunchKernelGGL(probe_parts<SymSide>, dim3(blocks_for(prb.count)), dim3(256), 0, nullptr, prb, M.binom,
+                           M.keys, M.lmin, M.lmax, cap, 0, M.hit);
         (void)hipEventRecord(e1, nullptr);
         if ((e = hipGetLastError()) != hipSuccess) return fail("launch");
         if ((e = hipMemcpy(hit_out, M.hit, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return fail("read-back");

@@ -181,34 +181,44 @@ void split_css(const BitMat& S, BitMat& Hx, BitMat& Hz) {
     }
 }
 
-Problem symplectic_isometric(const BitMat& S) {
+Problem symplectic_plain(const BitMat& S) {
     const int q = S.cols / 2;
     BitMat sw = swap_halves(S);
-    BitMat gen = kernel_basis(sw);                 // centralizer of S
-    BitMat det = swap_halves(quotient(S, sw));     // pre-swapped so <det, c> is a dot
+    Problem p;
+    p.n = 2 * q;
+    p.qubits = q;
+    p.gen = kernel_basis(sw);                      // centralizer of S
+    p.detector = swap_halves(quotient(S, sw));     // pre-swapped so <det, c> is a dot
     int best = 0;
-    for (int i = 0; i < gen.rows; ++i) {
-        if (in_rowspan(S, gen.row(i))) continue;   // a stabilizer, not a logical
+    for (int i = 0; i < p.gen.rows; ++i) {
+        if (in_rowspan(S, p.gen.row(i))) continue; // a stabilizer, not a logical
         int w = 0;
-        for (int j = 0; j < q; ++j) w += (gen.bit(i, j) || gen.bit(i, q + j)) ? 1 : 0;
+        for (int j = 0; j < q; ++j) w += (p.gen.bit(i, j) || p.gen.bit(i, q + j)) ? 1 : 0;
         if (best == 0 || w < best) best = w;
     }
+    p.seed_upper = best;
+    return p;
+}
+
+Problem symplectic_isometric(const BitMat& S) {
+    const Problem s = symplectic_plain(S);
+    const int q = s.qubits;
     Problem e;
     e.n = 3 * q;
     e.even = true;                  // every image (a|b|a^b) has even weight
-    e.seed_upper = best > 0 ? 2 * best : 0;
-    e.gen = BitMat(gen.rows, e.n);
-    for (int i = 0; i < gen.rows; ++i)
+    e.seed_upper = 2 * s.seed_upper;
+    e.gen = BitMat(s.gen.rows, e.n);
+    for (int i = 0; i < s.gen.rows; ++i)
         for (int j = 0; j < q; ++j) {
-            bool a = gen.bit(i, j), b = gen.bit(i, q + j);
+            bool a = s.gen.bit(i, j), b = s.gen.bit(i, q + j);
             if (a) e.gen.set(i, j);
             if (b) e.gen.set(i, q + j);
             if (a != b) e.gen.set(i, 2 * q + j);
         }
-    e.detector = BitMat(det.rows, e.n);
-    for (int i = 0; i < det.rows; ++i)
+    e.detector = BitMat(s.detector.rows, e.n);
+    for (int i = 0; i < s.detector.rows; ++i)
         for (int j = 0; j < 2 * q; ++j)
-            if (det.bit(i, j)) e.detector.set(i, j);
+            if (s.detector.bit(i, j)) e.detector.set(i, j);
     return e;
 }
 

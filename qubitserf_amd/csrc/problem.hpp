@@ -17,6 +17,8 @@ struct Problem {
     bool every_nonzero = false;  // classical: every nonzero codeword is a logical
     bool even = false;      // every codeword has even weight
     int seed_upper = 0;     // a known logical's weight (0 = none)
+    int qubits = 0;         // > 0: symplectic problem, n = 2 qubits columns [z | x] and the
+                            // weight of a vector is the number of qubits it touches
 };
 
 // CSS side 'Z' searches ker(Hx) modulo rowspan(Hz); side 'X' the converse
@@ -29,6 +31,10 @@ Problem classical(const BitMat& H);
 // (a|b) -> (a|b|a^b) to a Hamming problem of length 3q whose distance is twice the
 // symplectic one (cpp/distfind/stab.cpp:98-132, :147-178).
 Problem symplectic_isometric(const BitMat& S);
+// The same problem in symplectic form (cpp/distfind/stab.cpp:98-132): gen = the
+// centralizer of S over 2q columns, detectors pre-swapped so that <d, c> is a plain dot
+// product, qubits = q, seed_upper = the lightest logical row's symplectic weight.
+Problem symplectic_plain(const BitMat& S);
 bool is_css_symplectic(const BitMat& S);
 void split_css(const BitMat& S, BitMat& Hx, BitMat& Hz);
 

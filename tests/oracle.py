@@ -260,11 +260,11 @@ def ref_classical_distance(H, threads=0) -> RefResult:
     return r
 
 
-def ref_stabilizer_distance(S, which="M", threads=0) -> RefResult:
+def ref_stabilizer_distance(S, which="M", threads=0, method="bz") -> RefResult:
     a = _u8(S)
     r = RefResult()
-    assert ref().distfind_stabilizer_distance(_p(a), a.shape[0], a.shape[1], b"bz", which.encode()[:1], b"cpu",
-                                              threads, 0, ctypes.byref(r)) == 0
+    assert ref().distfind_stabilizer_distance(_p(a), a.shape[0], a.shape[1], method.encode(), which.encode()[:1],
+                                              b"cpu", threads, 0, ctypes.byref(r)) == 0
     return r
 
 

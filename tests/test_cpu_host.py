@@ -86,9 +86,11 @@ def test_c_abi_error_codes_without_device():
     assert L.distfind_classical_distance(None, 3, 7, b"bz", b"gpu", 0, 0, ctypes.byref(out)) == 1
     assert L.distfind_classical_distance(p, 3, 7, b"cc", b"gpu", 0, 0, ctypes.byref(out)) == 4
     assert L.distfind_classical_distance(p, 3, 7, b"mitm", b"gpu", 0, 0, ctypes.byref(out)) == 5   # no device here
-    S6 = np.array([[1, 0, 0, 0, 1, 0], [0, 1, 0, 0, 0, 1]], np.uint8)   # non-CSS: symplectic mitm is not provided
+    S6 = np.array([[1, 0, 0, 0, 1, 0], [0, 1, 0, 0, 0, 1]], np.uint8)   # non-CSS: symplectic mitm, no device here
     assert L.distfind_stabilizer_distance(S6.ctypes.data_as(_native.u8p), 2, 6, b"mitm", b"M", b"gpu", 0, 0,
-                                          ctypes.byref(out)) == 4
+                                          ctypes.byref(out)) == 5
+    assert L.distfind_stabilizer_distance(S6.ctypes.data_as(_native.u8p), 2, 6, b"cc", b"M", b"gpu", 0, 0,
+                                          ctypes.byref(out)) == 3   # connected cluster has no non-CSS form
     assert L.distfind_classical_distance(p, 3, 7, b"bz", b"cpu", 0, 0, ctypes.byref(out)) == 5
     S = np.zeros((2, 5), np.uint8)
     assert L.distfind_stabilizer_distance(S.ctypes.data_as(_native.u8p), 2, 5, b"bz", b"M", b"gpu", 0, 0,

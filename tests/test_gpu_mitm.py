@@ -68,3 +68,66 @@ def test_limits_are_errors(hip):
     H = codes.random_full_rank(np.random.default_rng(1), 70, 140)   # 70 parity checks
     with pytest.raises(ValueError, match="parity checks"):
         df.classical_distance(H, method="mitm", backend="gpu")
+
+
+# ---- non-CSS stabilizer codes: the symplectic meet in the middle -----------------------
+
+STAB = [c for c in GOLD["stabilizer"]]
+
+
+@pytest.mark.parametrize("case", STAB, ids=[c["name"] for c in STAB])
+def test_symplectic_golden(hip, case):
+    """Symplectic MITM against the reference's certified distances of its non-CSS fixtures."""
+    from qubitserf_amd import distfind as df
+    S = dec(case["S"])
+    r = df.api.stabilizer_distance_raw(S, "mitm", "M", "gpu")
+    assert r.backend.decode() == "hip"
+    assert (r.distance, bool(r.proven)) == (case["ref"]["distance"], bool(case["ref"]["proven"]))
+    if O.ref_available():   # the reference's own symplectic meet in the middle agrees
+        assert O.ref_stabilizer_distance(S, method="mitm").distance == r.distance
+
+
+def _css_symplectic(Hx, Hz):
+    """[z | x] stabilizer matrix of a CSS code: Z-type rows of Hz, X-type rows of Hx."""
+    q = Hx.shape[1]
+    return np.vstack([np.hstack([Hz, np.zeros_like(Hz)]), np.hstack([np.zeros_like(Hx), Hx])]).astype(np.uint8)
+
+
+def _local_cliffords(S, rng):
+    """Each qubit gets a random single-qubit Clifford (H: swap its z and x columns; S:
+    z += x, X -> Y). Every operator keeps the set of qubits it touches, so distances are
+    unchanged, while the code stops being CSS."""
+    S = S.copy()
+    q = S.shape[1] // 2
+    for j in range(q):
+        for op in rng.choice(["H", "S"], size=int(rng.integers(1, 4))):
+            if op == "H":
+                S[:, [j, q + j]] = S[:, [q + j, j]]
+            else:
+                S[:, j] ^= S[:, q + j]
+    return S
+
+
+@pytest.mark.parametrize("name,seed", [("toric3", 0), ("steane", 1), ("shor", 2), ("surface3", 3), ("toric3", 4)])
+def test_symplectic_local_clifford(hip, name, seed):
+    """Symplectic MITM on CSS codes scrambled by local Cliffords: the known distance, equal to
+    the device BZ (via the (a|b)->(a|b|a^b) isometry) and the oracle's BZ."""
+    from qubitserf_amd import distfind as df
+    Hx, Hz = {"toric3": lambda: codes.toric(3), "steane": codes.steane, "shor": codes.shor,
+              "surface3": lambda: codes.surface(3)}[name]()
+    want = 3
+    S = _local_cliffords(_css_symplectic(Hx, Hz), np.random.default_rng(seed))
+    got = df.stabilizer_distance(S, method="mitm", backend="gpu")
+    assert got == want
+    assert df.stabilizer_distance(S, method="bz", backend="gpu") == want
+    assert O.stabilizer_distance(S).distance == want
+    if O.ref_available():
+        assert O.ref_stabilizer_distance(S, method="mitm").distance == want
+
+
+def test_symplectic_five_qubit(hip):
+    from qubitserf_amd import distfind as df
+    S = np.array([[0, 1, 1, 0, 0, 1, 0, 0, 1, 0], [0, 0, 1, 1, 0, 0, 1, 0, 0, 1],
+                  [0, 0, 0, 1, 1, 1, 0, 1, 0, 0], [1, 0, 0, 0, 1, 0, 1, 0, 1, 0]], np.uint8)
+    r = df.api.stabilizer_distance_raw(S, "mitm", "M", "gpu")
+    assert (r.distance, bool(r.proven), r.lower_bound) == (3, True, 3)
