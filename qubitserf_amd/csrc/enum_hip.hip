@@ -13,6 +13,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <atomic>
 #include <string>
@@ -21,6 +22,7 @@
 #include "../../include/qsf_hip.h"
 #include "elim.hpp"
 #include "kernels.hpp"
+#include "masks.hpp"
 
 namespace qsa {
 
@@ -30,38 +32,6 @@ namespace qsa {
 
 __device__ __forceinline__ u64 bin(const u64* B, int bk, int n, int k) {
     return (k < 0 || n < 0 || k > n) ? 0ull : B[n * bk + k];
-}
-
-// Colex unrank: the idx-th k-subset of [0, m) in colex order (= increasing order of the
-// subset's bitmask), as a bitmask. Element c_i (i = k..1, decreasing) is the largest c
-// with C(c, i) <= remaining index (combinatorial number system).
-__device__ __forceinline__ u64 colex_mask(u64 idx, int k, int m, const u64* B, int bk) {
-    u64 mask = 0;
-    int cap = m;
-    for (int i = k; i >= 1; --i) {
-        int lo = i - 1, hi = cap - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (B[mid * bk + i] <= idx) lo = mid; else hi = mid - 1;
-        }
-        mask |= 1ull << lo;
-        idx -= (lo >= i) ? B[lo * bk + i] : 0ull;
-        cap = lo;
-    }
-    return mask;
-}
-
-// Next larger integer with the same popcount (colex successor); x != 0.
-__device__ __forceinline__ u64 colex_next(u64 x) {
-    u64 c = x & (~x + 1ull);
-    u64 r = x + c;
-    return (((r ^ x) >> 2) >> __builtin_ctzll(c)) | r;
-}
-
-// Bits [base, base + m) of the 128-bit mask (lo, hi), shifted down to bit 0 (m <= 64).
-__device__ __forceinline__ u64 range_bits(u64 lo, u64 hi, int base, int m) {
-    u64 v = base == 0 ? lo : base < 64 ? (lo >> base) | (hi << (64 - base)) : hi >> (base - 64);
-    return m >= 64 ? v : v & ((1ull << m) - 1ull);
 }
 
 // Values every lane already agrees on, moved to scalar registers so that loops and branches
@@ -91,12 +61,15 @@ struct SElem {
 
 // FD residual dwords are tested in the hot loop; XD >= FD dwords are held per codeword and
 // re-tested (the "medium" test) before a stream step drops into the exact slow path.
-template <int FD, int XD, int RPL>
+// W: words per list subset mask (1: lists of up to 64 rows, K <= 128; 2: up to 128 rows,
+// K <= 256). The masks enter only where list elements are made or re-made, not the hot loop.
+template <int FD, int XD, int RPL, int W>
 __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
-    __shared__ u32 rows_fast[kMaxTiledK * XD];
+    using Mask = typename std::conditional<W == 2, M2, u64>::type;
+    __shared__ u32 rows_fast[(W == 2 ? kMaxTiledK : kMaxMatrixK) * XD];
     __shared__ SElem<XD> stream[kStreamMax];
     __shared__ u32 sfull[kMaxRW];
-    __shared__ u64 run_start[64];            // colex mask of the first stream element of each lane's run
+    __shared__ Mask run_start[64];           // colex mask of the first stream element of each lane's run
     extern __shared__ u32 rows_full[];       // whole residual+syndrome table when it fits (dynamic size)
 
     const int lane = threadIdx.x;
@@ -137,10 +110,10 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
             }
         }
         struct {
-            u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream, f_lo, f_hi;
+            u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream;
             int fixed;
         } R = {uni64(Rr.tile_base), uni64(Rr.stream_groups), uni64(Rr.stream_chunks), uni64(Rr.n_lane),
-               uni64(Rr.n_stream), uni64(Rr.f_lo), uni64(Rr.f_hi), uni(Rr.fixed)};
+               uni64(Rr.n_stream), uni(Rr.fixed)};
         const u64 local = t - R.tile_base;
         const u64 bx = local / R.stream_groups;
         const u64 bg = local - bx * R.stream_groups;
@@ -148,9 +121,8 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
 
         const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
         const int sk = uni(Rr.sk), sbase = uni(Rr.sbase), sm = uni(Rr.sm);
-        const u64 flo = J->flags[2 * g], fhi = J->flags[2 * g + 1];
-        const u64 lflags = range_bits(flo, fhi, lbase, lm);
-        const u64 sflags = range_bits(flo, fhi, sbase, sm);
+        const Mask lflags = range_mask<Mask>(J->flags + 4 * g, lbase, lm);
+        const Mask sflags = range_mask<Mask>(J->flags + 4 * g, sbase, sm);
         const u32* fx = R.fixed >= 0 ? J->fixed + (size_t)R.fixed * (RW + 1) : nullptr;
         u32 fw[XD];
 #pragma unroll
@@ -173,23 +145,22 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         u32 xw[RPL][XD];
         int xoff[RPL];
         const u64 b0 = bx * (u64)(64 * RPL) + (u64)lane * RPL;
-        const u64 m0 = (b0 < R.n_lane) ? colex_mask(b0, lk, lm, a.binom, a.bk) : 0ull;
+        const Mask m0 = (b0 < R.n_lane) ? colex_unrank<Mask>(b0, lk, lm, a.binom, a.bk) : Mask{};
         {
-            u64 m = m0;
+            Mask m = m0;
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
                 const bool valid = b0 + k < R.n_lane;
 #pragma unroll
                 for (int f = 0; f < XD; ++f) xw[k][f] = 0;
                 if (valid) {
-                    u64 mm = m;
-                    while (mm) {
-                        int b = __builtin_ctzll(mm);
-                        mm &= mm - 1;
+                    Mask mm = m;
+                    while (nonzero(mm)) {
+                        const int b = take_low(mm);
 #pragma unroll
                         for (int f = 0; f < XD; ++f) xw[k][f] ^= rows_fast[(lbase + b) * XD + f];
                     }
-                    xoff[k] = __popcll(m & lflags);
+                    xoff[k] = pop_and(m, lflags);
                     if (b0 + k + 1 < R.n_lane && lk > 0) m = colex_next(m);
                 } else {
                     xoff[k] = kBigOffset;
@@ -209,20 +180,19 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 const int i0 = lane * per;
                 const int i1 = min(len, i0 + per);
                 if (i0 < i1) {
-                    u64 m = colex_mask(c0 + (u64)i0, sk, sm, a.binom, a.bk);
+                    Mask m = colex_unrank<Mask>(c0 + (u64)i0, sk, sm, a.binom, a.bk);
                     run_start[lane] = m;
                     for (int i = i0; i < i1; ++i) {
                         SElem<XD> e;
 #pragma unroll
                         for (int f = 0; f < XD; ++f) e.w[f] = fw[f];
-                        u64 mm = m;
-                        while (mm) {
-                            int b = __builtin_ctzll(mm);
-                            mm &= mm - 1;
+                        Mask mm = m;
+                        while (nonzero(mm)) {
+                            const int b = take_low(mm);
 #pragma unroll
                             for (int f = 0; f < XD; ++f) e.w[f] ^= rows_fast[(sbase + b) * XD + f];
                         }
-                        e.off = foff + __popcll(m & sflags);
+                        e.off = foff + pop_and(m, sflags);
                         stream[i] = e;
                         if (i + 1 < i1) m = colex_next(m);
                     }
@@ -261,16 +231,12 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                     // ---- slow path: exact weight and logical test of the light candidates
                     ++slow_entries;
                     const int owner = s / per;
-                    u64 smask = run_start[owner];
+                    Mask smask = run_start[owner];
                     for (int j = owner * per; j < s; ++j) smask = colex_next(smask);
                     if (lane < RW) {
                         u32 v = fx ? fx[lane] : 0u;
-                        u64 mm = smask;
-                        while (mm) {
-                            int b = __builtin_ctzll(mm);
-                            mm &= mm - 1;
-                            v ^= slow_tbl[(sbase + b) * RW + lane];
-                        }
+                        Mask mm = smask;
+                        while (nonzero(mm)) v ^= slow_tbl[(sbase + take_low(mm)) * RW + lane];
                         sfull[lane] = v;
                     }
                     __syncthreads();
@@ -278,17 +244,13 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                     while (cand) {
                         const int k = __builtin_ctz(cand);
                         cand &= cand - 1;
-                        u64 lmask = m0;
+                        Mask lmask = m0;
                         for (int j = 0; j < k; ++j) lmask = colex_next(lmask);
-                        int w = __popcll(lmask & lflags) + eoff;
+                        int w = pop_and(lmask, lflags) + eoff;
                         for (int f = 0; f < RD && w < bound; ++f) {
                             u32 v = sfull[f];
-                            u64 mm = lmask;
-                            while (mm) {
-                                int b = __builtin_ctzll(mm);
-                                mm &= mm - 1;
-                                v ^= slow_tbl[(lbase + b) * RW + f];
-                            }
+                            Mask mm = lmask;
+                            while (nonzero(mm)) v ^= slow_tbl[(lbase + take_low(mm)) * RW + f];
                             w += __popc(v);
                         }
                         if (a.collect) {
@@ -299,10 +261,11 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                                     const u64* G = J->gamma + (size_t)g * K * stride;
                                     for (int wd = 0; wd < stride; ++wd) {
                                         u64 v = 0;
-                                        for (u64 mm = R.f_lo; mm; mm &= mm - 1) v ^= G[(size_t)__builtin_ctzll(mm) * stride + wd];
-                                        for (u64 mm = R.f_hi; mm; mm &= mm - 1) v ^= G[(size_t)(64 + __builtin_ctzll(mm)) * stride + wd];
-                                        for (u64 mm = smask; mm; mm &= mm - 1) v ^= G[(size_t)(sbase + __builtin_ctzll(mm)) * stride + wd];
-                                        for (u64 mm = lmask; mm; mm &= mm - 1) v ^= G[(size_t)(lbase + __builtin_ctzll(mm)) * stride + wd];
+                                        for (int q = 0; q < 4; ++q)
+                                            for (u64 mm = Rr.fmask[q]; mm; mm &= mm - 1)
+                                                v ^= G[(size_t)(64 * q + __builtin_ctzll(mm)) * stride + wd];
+                                        for (Mask mm = smask; nonzero(mm);) v ^= G[(size_t)(sbase + take_low(mm)) * stride + wd];
+                                        for (Mask mm = lmask; nonzero(mm);) v ^= G[(size_t)(lbase + take_low(mm)) * stride + wd];
                                         a.out[slot * stride + wd] = v;
                                     }
                                 }
@@ -313,12 +276,8 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                             bool logical = (SD == 0);
                             for (int f = 0; f < SD && !logical; ++f) {
                                 u32 v = sfull[RD + f];
-                                u64 mm = lmask;
-                                while (mm) {
-                                    int b = __builtin_ctzll(mm);
-                                    mm &= mm - 1;
-                                    v ^= slow_tbl[(lbase + b) * RW + RD + f];
-                                }
+                                Mask mm = lmask;
+                                while (nonzero(mm)) v ^= slow_tbl[(lbase + take_low(mm)) * RW + RD + f];
                                 logical = v != 0;
                             }
                             if (logical) mine = w;
@@ -388,7 +347,7 @@ template <int XD>
 __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
     typedef int v4i __attribute__((ext_vector_type(4)));
     typedef int v16i __attribute__((ext_vector_type(16)));
-    __shared__ u32 rows_fast[kMaxTiledK * XD];
+    __shared__ u32 rows_fast[kMaxMatrixK * XD];
     __shared__ v4i sbytes[2 * kMfmaStream];                  // half h of chunk position e at [h * kMfmaStream + e]
     __shared__ u64 smk[kMfmaStream];                          // the chunk's stream-element subsets
     __shared__ u64 lmk[kMfmaWaves * 256];                     // each wave's lane-element subsets
@@ -450,9 +409,8 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         const int g = uni(Rr.g);
         const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
         const int sk = uni(Rr.sk), sbase = uni(Rr.sbase), sm = uni(Rr.sm);
-        const u64 flo = J->flags[2 * g], fhi = J->flags[2 * g + 1];
-        const u64 lflags = range_bits(flo, fhi, lbase, lm);
-        const u64 sflags = range_bits(flo, fhi, sbase, sm);
+        const u64 lflags = range_mask<u64>(J->flags + 4 * g, lbase, lm);
+        const u64 sflags = range_mask<u64>(J->flags + 4 * g, sbase, sm);
         const u32* fx = fixed >= 0 ? J->fixed + (size_t)fixed * (RW + 1) : nullptr;
         const u32 fw0 = fx ? fx[0] : 0u;
         const int foff = fx ? (int)fx[RW] : 0;
@@ -726,7 +684,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     constexpr int SL = f4_stream(MM);             // stream chunk length
     constexpr int EPT = SL / (64 * kMfmaWaves);   // stream elements each thread makes per chunk
     static_assert(XD >= NKB, "the held residual dwords must cover the test bits");
-    __shared__ u32 rows_fast[kMaxTiledK * XD];
+    __shared__ u32 rows_fast[kMaxMatrixK * XD];
     __shared__ i32x4 sbytes[NKB * SL];            // block kb of chunk position e at [kb * SL + e]
     __shared__ float soff[SL];                    // off_stream of each chunk position (127: padding)
     __shared__ u64 smk[SL];                       // the chunk's stream-element subsets
@@ -802,9 +760,8 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         const int g = uni(Rr.g);
         const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
         const int sk = uni(Rr.sk), sbase = uni(Rr.sbase), sm = uni(Rr.sm);
-        const u64 flo = J->flags[2 * g], fhi = J->flags[2 * g + 1];
-        const u64 lflags = range_bits(flo, fhi, lbase, lm);
-        const u64 sflags = range_bits(flo, fhi, sbase, sm);
+        const u64 lflags = range_mask<u64>(J->flags + 4 * g, lbase, lm);
+        const u64 sflags = range_mask<u64>(J->flags + 4 * g, sbase, sm);
         const u32* fx = fixed >= 0 ? J->fixed + (size_t)fixed * (RW + 1) : nullptr;
         const int foff = fx ? (int)fx[RW] : 0;
         const u32* gtbl = J->tbl + (size_t)g * K * RW;
@@ -1201,7 +1158,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
-    static constexpr int kVariants = 14;  // kernel variants of a tiled run (vtab in run_tiled)
+    static constexpr int kVariants = 22;  // kernel variants of a tiled run (vtab in run_tiled)
     hipEvent_t gev[2 * kVariants] = {};   // start/stop per kernel-variant launch of a tiled run
     DevBuf<u64> gbinom;                  // (kMaxTiledK + 1) x (kMaxD + 1) saturating C(n, k) for the tiled kernel
     DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
@@ -1390,7 +1347,7 @@ struct Engine {
         s.SD = (p.kcheck + 31) / 32;
         s.RW = s.RD + s.SD;
         std::vector<u32> tbl((size_t)s.ngt * K * s.RW, 0);
-        std::vector<u64> fl((size_t)s.ngt * 2, 0);
+        std::vector<u64> fl((size_t)s.ngt * 4, 0);
         for (int g = 0; g < s.ngt; ++g) {
             const u64* G = p.gamma + (size_t)g * K * st;
             for (int r = 0; r < K; ++r) {
@@ -1406,8 +1363,8 @@ struct Engine {
                     for (int w = 0; w < st; ++w) acc ^= ch[w] & row[w];
                     if (__builtin_popcountll(acc) & 1) out[s.RD + (c >> 5)] |= 1u << (c & 31);
                 }
-                // two mask words per generator cover rows 0..127, the tiled kernel's K range
-                if (flagged[g][r] && r < kMaxTiledK) fl[2 * g + (r >> 6)] |= 1ull << (r & 63);
+                // four mask words per generator cover rows 0..255, the tiled kernels' K range
+                if (flagged[g][r] && r < kMaxTiledK) fl[4 * g + (r >> 6)] |= 1ull << (r & 63);
             }
         }
         // one staging vector, one allocation, one copy
@@ -1689,7 +1646,11 @@ struct Engine {
     // that any d-subset must contain).
     // mm: 0 VALU hot test (bz_tiled<fd, xd, rpl>), 1 int8 products (bz_mfma<xd>), 2 / 3 fp4
     // products of 32 x 32 / 16 x 16 codewords (bz_f4<xd, mm>); the matrix tests have fd = 0.
-    struct Variant { int fd, xd, rpl, mm; };
+    // w: words per list subset mask (2 for K > 128, VALU kernel only).
+    struct Variant {
+        int fd, xd, rpl, mm;
+        int w = 1;
+    };
     static double binom_cdf_below(int bits, long thr) {   // P(Binomial(bits, 1/2) < thr)
         if (thr <= 0) return 0.0;
         if (thr > bits) return 1.0;
@@ -1711,6 +1672,12 @@ struct Engine {
     // level (codewords) a matrix test may take (default matrix_min_work()).
     static Variant choose_variant(const Solve& s, const qsf_enum_plan& p, bool allow_mfma = true,
                                   u64 min_work = ~0ull) {
+        if (p.K <= kMaxMatrixK) return choose_variant_k128(s, p, allow_mfma, min_work);
+        Variant v = choose_variant_k128(s, p, false, min_work);   // two-word masks: VALU kernel only
+        v.w = 2;
+        return v;
+    }
+    static Variant choose_variant_k128(const Solve& s, const qsf_enum_plan& p, bool allow_mfma, u64 min_work) {
         static const Variant table[] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {2, 2, 16, 0}, {2, 4, 12, 0}, {3, 4, 12, 0},
                                         {4, 4, 12, 0}, {1, 2, 24, 0}, {1, 2, 32, 0}, {0, 1, 4, 1},  {0, 2, 4, 1},
                                         {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3}};
@@ -1784,7 +1751,11 @@ struct Engine {
         std::vector<u32> fixed;
         static constexpr u64 kExpandBelow = 64;
 
-        void emit(u64 f_lo, u64 f_hi, int xb, int xm, int xk, int yb, int ym, int yk, u64 nx, u64 ny) {
+        struct RowSet {   // fixed rows, rows 0..255
+            u64 w[4];
+        };
+
+        void emit(const RowSet& f, int xb, int xm, int xk, int yb, int ym, int yk, u64 nx, u64 ny) {
             Rect r{};
             r.g = g;
             // the longer list on the lanes; with the MFMA test's 1024-element lane tiles, the
@@ -1800,26 +1771,24 @@ struct Engine {
             r.lane_blocks = (r.n_lane + lane_tile - 1) / lane_tile;
             r.stream_chunks = (r.n_stream + S - 1) / S;
             r.fixed = -1;
-            r.f_lo = f_lo;
-            r.f_hi = f_hi;
-            if (f_lo | f_hi) {
+            for (int q = 0; q < 4; ++q) r.fmask[q] = f.w[q];
+            if (f.w[0] | f.w[1] | f.w[2] | f.w[3]) {
                 r.fixed = (int)(fixed.size() / (RW + 1));
                 const size_t at = fixed.size();
                 fixed.resize(at + RW + 1, 0);
                 int off = 0;
                 for (int row = 0; row < s.K; ++row) {
-                    const bool in = row < 64 ? (f_lo >> row) & 1 : (f_hi >> (row - 64)) & 1;
-                    if (!in) continue;
+                    if (!((f.w[row >> 6] >> (row & 63)) & 1)) continue;
                     const u32* src = s.htbl.data() + ((size_t)g * s.K + row) * RW;
                     for (int w = 0; w < RW; ++w) fixed[at + w] ^= src[w];
-                    off += (int)((s.hflags[2 * g + (row >> 6)] >> (row & 63)) & 1);
+                    off += (int)((s.hflags[4 * g + (row >> 6)] >> (row & 63)) & 1);
                 }
                 fixed[at + RW] = (u32)off;
             }
             rv.push_back(r);
         }
 
-        void split(u64 f_lo, u64 f_hi, int xb, int xm, int xk, int yb, int ym, int yk, int depth) {
+        void split(const RowSet& f, int xb, int xm, int xk, int yb, int ym, int yk, int depth) {
             const u64 nx = s.C(xm, xk), ny = s.C(ym, yk);
             if (nx == 0 || ny == 0) return;
             const bool x_small = nx <= ny;
@@ -1830,20 +1799,22 @@ struct Engine {
             // while every rectangle costs host time, upload and a longer tile search
             const int max_depth = matrix ? 1 : 3;
             if (ns >= kExpandBelow || bk == 0 || bm < 2 || depth >= max_depth || sat_mul(nx, ny) < (1ull << 22)) {
-                emit(f_lo, f_hi, xb, xm, xk, yb, ym, yk, nx, ny);
+                emit(f, xb, xm, xk, yb, ym, yk, nx, ny);
                 return;
             }
-            const int sb = x_small ? xb : yb, sk = x_small ? xk : yk;   // small side, enumerated
+            // small side, enumerated: fewer than 64 subsets, so its range has fewer than 64
+            // rows (or the subset is the whole range, of sk <= 32 rows)
+            const int sb = x_small ? xb : yb, sk = x_small ? xk : yk;
             const int mid = bm / 2;
-            u64 m = sk == 0 ? 0 : (sk >= 64 ? ~0ull : ((1ull << sk) - 1));
+            u64 m = sk == 0 ? 0 : ((1ull << sk) - 1);
             for (u64 e = 0; e < ns; ++e) {
-                u64 lo = f_lo, hi = f_hi;
+                RowSet fe = f;
                 for (u64 mm = m; mm; mm &= mm - 1) {
                     const int row = sb + __builtin_ctzll(mm);
-                    if (row < 64) lo |= 1ull << row; else hi |= 1ull << (row - 64);
+                    fe.w[row >> 6] |= 1ull << (row & 63);
                 }
                 for (int j = std::max(0, bk - (bm - mid)); j <= std::min(bk, mid); ++j)
-                    split(lo, hi, bb, mid, j, bb + mid, bm - mid, bk - j, depth + 1);
+                    split(fe, bb, mid, j, bb + mid, bm - mid, bk - j, depth + 1);
                 if (e + 1 < ns) {   // next sk-subset of [sb, sb + sm) in colex order
                     const u64 c = m & (~m + 1ull), r = m + c;
                     m = (((r ^ m) >> 2) >> __builtin_ctzll(c)) | r;
@@ -1877,13 +1848,17 @@ struct Engine {
     // read-back of counters and bounds. res[i] = the bound job i ended with.
     void run_tiled(std::vector<TItem>& items, bool collect, int keep, u64 cap, std::vector<int>& res) {
         constexpr int NV = kVariants;
-        static const Variant vtab[NV] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {1, 2, 24, 0}, {1, 2, 32, 0}, {2, 2, 16, 0},
-                                         {2, 4, 12, 0}, {3, 4, 12, 0}, {4, 4, 12, 0}, {0, 1, 4, 1},  {0, 2, 4, 1},
-                                         {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3}};
+        static const Variant vtab[NV] = {
+            {1, 1, 32, 0},    {1, 2, 16, 0},    {1, 2, 24, 0},    {1, 2, 32, 0},    {2, 2, 16, 0},    {2, 4, 12, 0},
+            {3, 4, 12, 0},    {4, 4, 12, 0},    {0, 1, 4, 1},     {0, 2, 4, 1},     {0, 4, 4, 1},     {0, 2, 4, 2},
+            {0, 4, 4, 2},     {0, 4, 2, 3},     {1, 1, 32, 0, 2}, {1, 2, 16, 0, 2}, {1, 2, 24, 0, 2}, {1, 2, 32, 0, 2},
+            {2, 2, 16, 0, 2}, {2, 4, 12, 0, 2}, {3, 4, 12, 0, 2}, {4, 4, 12, 0, 2}};
         auto vidx = [&](const Variant& v) {
             for (int k = 0; k < NV; ++k)
-                if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl && vtab[k].mm == v.mm) return k;
-            return 7;
+                if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl && vtab[k].mm == v.mm &&
+                    vtab[k].w == v.w)
+                    return k;
+            return v.w == 2 ? 21 : 7;
         };
         auto stream_len = [](const Variant& v) { return v.mm == 0 ? kStreamMax : v.mm == 1 ? kMfmaStream : f4_stream(v.mm); };
         auto lane_tile = [](const Variant& v) {
@@ -1903,7 +1878,7 @@ struct Engine {
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
                 for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
-                    B.split(0, 0, 0, h, ka, h, K - h, d - ka, 0);
+                    B.split(RectBuilder::RowSet{}, 0, h, ka, h, K - h, d - ka, 0);
             }
             for (Rect& r : B.rv) r.job = i;
             grect[vi].insert(grect[vi].end(), B.rv.begin(), B.rv.end());
@@ -2001,20 +1976,28 @@ struct Engine {
             const size_t lds = (size_t)glds[v] * sizeof(u32);
             HIPCHK(hipEventRecord(gev[2 * v], stream));
             switch (v) {
-                case 0: hipLaunchKernelGGL((bz_tiled<1, 1, 32>), gd, bd, lds, stream, a); break;
-                case 1: hipLaunchKernelGGL((bz_tiled<1, 2, 16>), gd, bd, lds, stream, a); break;
-                case 2: hipLaunchKernelGGL((bz_tiled<1, 2, 24>), gd, bd, lds, stream, a); break;
-                case 3: hipLaunchKernelGGL((bz_tiled<1, 2, 32>), gd, bd, lds, stream, a); break;
-                case 4: hipLaunchKernelGGL((bz_tiled<2, 2, 16>), gd, bd, lds, stream, a); break;
-                case 5: hipLaunchKernelGGL((bz_tiled<2, 4, 12>), gd, bd, lds, stream, a); break;
-                case 6: hipLaunchKernelGGL((bz_tiled<3, 4, 12>), gd, bd, lds, stream, a); break;
-                case 7: hipLaunchKernelGGL((bz_tiled<4, 4, 12>), gd, bd, lds, stream, a); break;
+                case 0: hipLaunchKernelGGL((bz_tiled<1, 1, 32, 1>), gd, bd, lds, stream, a); break;
+                case 1: hipLaunchKernelGGL((bz_tiled<1, 2, 16, 1>), gd, bd, lds, stream, a); break;
+                case 2: hipLaunchKernelGGL((bz_tiled<1, 2, 24, 1>), gd, bd, lds, stream, a); break;
+                case 3: hipLaunchKernelGGL((bz_tiled<1, 2, 32, 1>), gd, bd, lds, stream, a); break;
+                case 4: hipLaunchKernelGGL((bz_tiled<2, 2, 16, 1>), gd, bd, lds, stream, a); break;
+                case 5: hipLaunchKernelGGL((bz_tiled<2, 4, 12, 1>), gd, bd, lds, stream, a); break;
+                case 6: hipLaunchKernelGGL((bz_tiled<3, 4, 12, 1>), gd, bd, lds, stream, a); break;
+                case 7: hipLaunchKernelGGL((bz_tiled<4, 4, 12, 1>), gd, bd, lds, stream, a); break;
                 case 8: hipLaunchKernelGGL((bz_mfma<1>), gd, bd, lds, stream, a); break;
                 case 9: hipLaunchKernelGGL((bz_mfma<2>), gd, bd, lds, stream, a); break;
                 case 10: hipLaunchKernelGGL((bz_mfma<4>), gd, bd, lds, stream, a); break;
                 case 11: hipLaunchKernelGGL((bz_f4<2, 2>), gd, bd, lds, stream, a); break;
                 case 12: hipLaunchKernelGGL((bz_f4<4, 2>), gd, bd, lds, stream, a); break;
-                default: hipLaunchKernelGGL((bz_f4<4, 3>), gd, bd, lds, stream, a); break;
+                case 13: hipLaunchKernelGGL((bz_f4<4, 3>), gd, bd, lds, stream, a); break;
+                case 14: hipLaunchKernelGGL((bz_tiled<1, 1, 32, 2>), gd, bd, lds, stream, a); break;
+                case 15: hipLaunchKernelGGL((bz_tiled<1, 2, 16, 2>), gd, bd, lds, stream, a); break;
+                case 16: hipLaunchKernelGGL((bz_tiled<1, 2, 24, 2>), gd, bd, lds, stream, a); break;
+                case 17: hipLaunchKernelGGL((bz_tiled<1, 2, 32, 2>), gd, bd, lds, stream, a); break;
+                case 18: hipLaunchKernelGGL((bz_tiled<2, 2, 16, 2>), gd, bd, lds, stream, a); break;
+                case 19: hipLaunchKernelGGL((bz_tiled<2, 4, 12, 2>), gd, bd, lds, stream, a); break;
+                case 20: hipLaunchKernelGGL((bz_tiled<3, 4, 12, 2>), gd, bd, lds, stream, a); break;
+                default: hipLaunchKernelGGL((bz_tiled<4, 4, 12, 2>), gd, bd, lds, stream, a); break;
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(gev[2 * v + 1], stream));

@@ -13,7 +13,12 @@
 //    v_min3: no loads, no divergence, no per-candidate unranking.
 //  * bz_rowsweep — the plain kernel: every thread owns a run of lexicographic combinations
 //    and XORs full rows (the reference's two-level sweep, enum_core.hpp:109-194). Used for
-//    small levels, K > 128, and collect mode.
+//    small levels and K > 256.
+//
+// Row masks. A list's subsets are bit masks over its row range: one 64-bit word for ranges
+// of up to 64 rows (K <= 128: both halves fit), two words (W = 2) for up to 128 rows, which
+// the VALU kernel uses for 128 < K <= 256. Fixed row sets and the flagged rows of a
+// generator are 4-word masks over rows 0..255.
 //
 // Codeword representation in the tiled kernel ("residual form"). For generator g, a
 // column that holds a single 1, at row i, is a unit column of row i; one unit column per
@@ -32,7 +37,8 @@ namespace qsa {
 using u64 = std::uint64_t;
 using u32 = std::uint32_t;
 
-constexpr int kMaxTiledK = 128;     // both halves must fit a 64-bit subset mask
+constexpr int kMaxTiledK = 256;     // VALU kernel: two-word masks over each half when K > 128
+constexpr int kMaxMatrixK = 128;    // matrix-core kernels: one-word masks over each half
 constexpr int kMaxD = 64;           // binomial table columns
 constexpr int kStreamMax = 512;     // LDS stream chunk capacity (elements)
 constexpr int kMfmaBlocks = 8;      // bz_mfma: 32-element lane blocks per wave
@@ -48,16 +54,16 @@ constexpr int kLdsFullRows = 1024;  // dwords: the whole K x RW table is kept in
 constexpr int kBigOffset = (1 << 30) + (1 << 29);  // masks lanes past the end of a list
 
 // One complete product  F + subsets(L, lk) x subsets(Sr, sk)  of one generator: L and Sr
-// are disjoint row ranges of at most 64 rows, F a fixed set of rows outside both (its
-// codeword is XORed into every stream element). The register tile holds subsets of L,
-// the stream subsets of Sr.
+// are disjoint row ranges of at most 64 rows (128 in the two-word kernels), F a fixed set
+// of rows outside both (its codeword is XORed into every stream element). The register
+// tile holds subsets of L, the stream subsets of Sr.
 struct Rect {
     int job;               // index into TiledArgs::jobs
     int g;
     int lbase, lm, lk;     // lane list: lk-subsets of rows [lbase, lbase + lm)
     int sbase, sm, sk;     // stream list: sk-subsets of rows [sbase, sbase + sm)
     int fixed;             // index into TiledArgs::fixed, or -1 for no fixed rows
-    u64 f_lo, f_hi;        // the fixed rows as a 128-bit row mask
+    u64 fmask[4];          // the fixed rows as a 256-bit row mask
     u64 n_lane, n_stream;  // list lengths C(lm, lk), C(sm, sk)
     u64 lane_blocks;       // ceil(n_lane / (64 * RPL))
     u64 stream_chunks;     // ceil(n_stream / S)
@@ -68,7 +74,7 @@ struct Rect {
 // One problem's level inside a tiled launch (a launch may hold many problems' levels).
 struct TJob {
     const u32* tbl;          // [g][row][RW]: RD residual dwords, then SD syndrome dwords
-    const u64* flags;        // [g][2]: flagged rows 0..63 and 64..127 as bit masks
+    const u64* flags;        // [g][4]: flagged rows 0..255 as a 256-bit mask
     const u32* fixed;        // [f][RW + 1]: residual+syndrome of a fixed row set, then its flagged count
     const u64* gamma;        // [g][row][stride] full rows (collect mode)
     int* best;               // running bound; starts at current_best

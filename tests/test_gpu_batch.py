@@ -51,6 +51,21 @@ def test_batch_equals_single_and_oracle(hip):
     assert st["rounds"] <= max(2 * g.levels for g in got)
 
 
+def test_batch_mixed_widths(hip):
+    """One batch holding codes with K <= 128 and K > 128 (the two tiled mask widths share
+    the rounds): each result equals the single-code one and the oracle's."""
+    from qubitserf_amd import distfind as df
+    rng = np.random.default_rng(45)
+    probs = [("classical", codes.random_rate_code(rng, 150, 220, 0.82, 0.9)) for _ in range(4)]
+    probs += [("classical", codes.random_rate_code(rng, 60, 100)) for _ in range(4)]
+    got = df.batch_distance(probs)
+    for p, g in zip(probs, got):
+        s = _single(p)
+        o = _oracle(p)
+        assert (g.distance, g.lower_bound, g.proven, g.levels) == (s.distance, s.lower_bound, s.proven, s.levels)
+        assert (g.distance, g.proven) == (o.distance, o.proven), p[1].shape
+
+
 def test_batch_repeatable(hip):
     from qubitserf_amd import distfind as df
     probs = _mixed_problems()[:8]

@@ -79,6 +79,19 @@ def test_random_classical_sweep(hip):
         assert (raw.distance, raw.proven) == (o.distance, o.proven), H.shape
 
 
+def test_high_rate_classical_wide(hip):
+    """Codes with 128 < K <= 186 (the tiled kernel with two-word list masks): distance and
+    proven flag equal the oracle's."""
+    from qubitserf_amd import distfind as df
+
+    rng = np.random.default_rng(44)
+    for _ in range(6):
+        H = codes.random_rate_code(rng, 150, 220, 0.82, 0.9)
+        raw = df.api.classical_distance_raw(H, "bz", "gpu")
+        o = O.classical_distance(H)
+        assert (raw.distance, raw.proven) == (o.distance, o.proven), H.shape
+
+
 @pytest.mark.parametrize("m", [7, 9, 10])
 def test_qrm_large_n(hip, m):
     from qubitserf_amd import distfind as df

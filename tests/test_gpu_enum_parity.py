@@ -172,6 +172,35 @@ def test_tiled_large_levels_with_groups(hip):
         assert st["kernel"] == 2
 
 
+@pytest.mark.parametrize("fd", [1, 2, 3, 4])
+def test_tiled_wide_rows(hip, fd, monkeypatch):
+    """K > 128: the VALU kernel with two-word list masks (each half of the rows up to 128
+    rows), every fast-test width, random and systematic generators, against the oracle."""
+    monkeypatch.setenv("QSA_TILED_FD", str(fd))
+    rng = np.random.default_rng(1700 + fd)
+    cases = [(129, 5, 3), (150, 5, 3), (176, 5, 3), (200, 6, 3), (256, 6, 2), (140, 5, 4)]
+    with hip.kernel_policy("tiled"):
+        for i, (K, stride, d) in enumerate(cases):
+            best = [NONE, 60, 40, 90, NONE, 30][i]
+            run_case(hip, rng, K, stride, d, 1 + i % 2, i % 3, best, systematic=i % 3 != 1)
+            st = hip.last_stats()
+            assert st["kernel"] == 2 and st["fd"] == fd, st
+
+
+def test_tiled_wide_collect(hip):
+    """K > 128 in collect mode: the multiset of emitted codewords equals the oracle's, an
+    exact-cover check of the products over rows 0..255 (fixed row sets included)."""
+    rng = np.random.default_rng(1800)
+    with hip.kernel_policy("tiled"):
+        for K, stride, d in [(150, 4, 3), (200, 5, 2), (240, 5, 3)]:
+            gamma, _ = rand_plan(rng, K, stride, 1, 0, systematic=True)
+            n = stride * 64
+            low = O.enumerate_plan(gamma, n, d)
+            rows = _check_collect(hip, gamma, n, d, low + 8)
+            assert rows > 0
+            assert hip.last_stats()["kernel"] == 2
+
+
 @pytest.mark.parametrize("xd", [1, 2, 4])
 def test_mfma_hot_test(hip, xd, monkeypatch):
     """The int8-MFMA hot test (bz_mfma) with each medium-test width, on random and
