@@ -11,6 +11,11 @@
 //                             (2 per multiply-add).
 //   qsa_calib mfma_max        the int8 hot-test pattern: per step 8 products (zero
 //                             accumulator) reduced with v_max3; prints products/s.
+//   qsa_calib mfma_issue      the hot test's products alone: per step 8 independent
+//                             products (zero accumulator), one value of each consumed;
+//                             prints products/s (the matrix cores' rate for this shape).
+//   qsa_calib mfma_valu       8 such products plus 64 v_min3 that do not read them:
+//                             shows how far VALU work overlaps the products on a SIMD.
 //   qsa_calib mfma16          as mfma with v_mfma_i32_16x16x64_i8.
 //   qsa_calib mfma16_and      the packed hot-test pattern: per step 8 16x16x64 products
 //                             (bias accumulator) AND-reduced, two codewords per entry;
@@ -97,6 +102,34 @@ __global__ __launch_bounds__(64) void mfma_max(int* out, int iters) {
             mx = max(mx, max(max(red(p0), red(p1)), max(red(p2), red(p3))));
         }
         hit += __ballot(mx > (1 << 29)) != 0ull;
+    }
+    out[blockIdx.x * 64 + l] = hit;
+}
+
+// V = 0: products only; V = 1: products plus 64 independent v_min3 per step
+template <int V>
+__global__ __launch_bounds__(64) void mfma_issue(int* out, int iters) {
+    const int l = threadIdx.x;
+    v4i a[8], b = {l, l * 3, l * 5, l * 7};
+    for (int k = 0; k < 8; ++k) a[k] = v4i{l + k, l ^ k, l * k, l - k};
+    v16i w;
+    for (int v = 0; v < 16; ++v) w[v] = l * v + 7 * v;
+    int hit = 0;
+    const v16i z = {};
+    for (int i = 0; i < iters; ++i) {
+        b[0] += 1;
+        int mn = 1 << 30;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const v16i p = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[k], b, z, 0, 0, 0);
+            mn = min(mn, p[k]);
+            if (V == 1) {
+                asm volatile("" : "+v"(w));   // fresh values: the min3 chain is not hoisted
+#pragma unroll
+                for (int v = 0; v < 16; v += 2) mn = min(min(mn, w[v]), w[v + 1]);
+            }
+        }
+        hit += __ballot(mn < -(1 << 29)) != 0ull;
     }
     out[blockIdx.x * 64 + l] = hit;
 }
@@ -240,6 +273,7 @@ int main(int argc, char** argv) {
         return 0;
     }
     const bool m32c = std::strcmp(argv[1], "mfma") == 0, m32r = std::strcmp(argv[1], "mfma_max") == 0;
+    const bool m32i = std::strcmp(argv[1], "mfma_issue") == 0, m32v = std::strcmp(argv[1], "mfma_valu") == 0;
     const bool m16c = std::strcmp(argv[1], "mfma16") == 0, m16r = std::strcmp(argv[1], "mfma16_and") == 0;
     const bool f4c = std::strcmp(argv[1], "f4") == 0, f4r = std::strcmp(argv[1], "f4_max") == 0;
     if (f4c || f4r) {
@@ -265,7 +299,7 @@ int main(int argc, char** argv) {
         CHECK(hipFree(out));
         return 0;
     }
-    if (m32c || m32r || m16c || m16r) {
+    if (m32c || m32r || m16c || m16r || m32i || m32v) {
         const int per_cu = argc >= 3 ? std::atoi(argv[2]) : 12;
         const int blocks = 256 * per_cu, iters = (m32c || m16c) ? 4000 : 2000;
         int* out;
@@ -273,6 +307,8 @@ int main(int argc, char** argv) {
         auto launch = [&](int it) {
             if (m32c) hipLaunchKernelGGL(mfma_chain, dim3(blocks), dim3(64), 0, 0, out, it);
             else if (m32r) hipLaunchKernelGGL(mfma_max, dim3(blocks), dim3(64), 0, 0, out, it);
+            else if (m32i) hipLaunchKernelGGL(mfma_issue<0>, dim3(blocks), dim3(64), 0, 0, out, it);
+            else if (m32v) hipLaunchKernelGGL(mfma_issue<1>, dim3(blocks), dim3(64), 0, 0, out, it);
             else if (m16c) hipLaunchKernelGGL(mfma16_chain, dim3(blocks), dim3(64), 0, 0, out, it);
             else hipLaunchKernelGGL(mfma16_and, dim3(blocks), dim3(64), 0, 0, out, it);
         };
@@ -290,7 +326,7 @@ int main(int argc, char** argv) {
             std::printf("{\"probe\": \"%s\", \"waves_per_cu\": %d, \"int8_ops\": %.6e, \"ms\": %.3f, \"tops\": %.1f}\n", argv[1],
                         per_cu, mfmas * ops_each, ms, mfmas * ops_each / (ms * 1e-3) / 1e12);
         else {
-            const double cw = mfmas * (m32r ? 1024.0 : 512.0);
+            const double cw = mfmas * (m32r || m32i || m32v ? 1024.0 : 512.0);
             std::printf("{\"probe\": \"%s\", \"waves_per_cu\": %d, \"products\": %.6e, \"ms\": %.3f, \"products_per_s\": %.4e}\n",
                         argv[1], per_cu, cw, ms, cw / (ms * 1e-3));
         }
