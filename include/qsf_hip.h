@@ -166,7 +166,8 @@ const char* qsf_hip_mitm_last_error(void);
 /* Statistics of the last enumerate/collect call on this thread's library instance:
  * codewords examined (C(K,d) x num_gamma, the reference's `work`, bz.cpp:151-154), device
  * time in milliseconds of the enumeration kernel, and which kernel ran
- * (0 = none, 1 = row-sweep kernel, 2 = tiled split kernel). */
+ * (0 = none, 1 = row-sweep kernel, 2 = tiled split kernel, 3 = small-level row sweep:
+ * levels under 2^16 codewords, many per launch). */
 typedef struct {
     uint64_t codewords;
     double kernel_ms;

@@ -55,11 +55,16 @@ struct SetRequest {
     const std::vector<std::vector<int>>* parts = nullptr;
     bool ranks_only = false;   // out.gamma stays empty
     Family out;
+    // the random information-set bound in the same device call (device_sets with trials > 0)
+    const u64* check = nullptr;
+    int kcheck = 0, every_nonzero = 0, init = kNoWeight;
+    int best = kNoWeight;      // out
 };
 
 // All requests in one device call (a request whose peel outgrows the first capacity guess
-// is redone with room for every set).
-void device_sets(std::vector<SetRequest*>& reqs) {
+// is redone with room for every set). trials > 0: the first call also computes every
+// request's random information-set bound (check, kcheck, every_nonzero, init -> best).
+void device_sets(std::vector<SetRequest*>& reqs, int trials = 0, u64 seed = 0) {
     std::vector<SetRequest*> todo = reqs;
     std::vector<int> caps(reqs.size());
     std::unordered_map<SetRequest*, int> cap;
@@ -77,7 +82,12 @@ void device_sets(std::vector<SetRequest*>& reqs) {
             q.gen = g.data.data();
             q.K = g.rows;
             q.n = g.cols;
-            q.init = kNoWeight;
+            q.init = trials > 0 ? r.init : kNoWeight;
+            if (trials > 0) {
+                q.check = r.check;
+                q.kcheck = r.kcheck;
+                q.every_nonzero = r.every_nonzero;
+            }
             q.order = r.order ? r.order->data() : nullptr;
             if (r.parts) {
                 off[k].push_back(0);
@@ -96,12 +106,14 @@ void device_sets(std::vector<SetRequest*>& reqs) {
             q.set_cap = c;
             q.ranks = ranks[k].data();
         }
-        const int rc = qsf_hip_eliminate(jobs.data(), (int)m, 0, 0, QSF_ELIM_SETS);
+        const int rc = qsf_hip_eliminate(jobs.data(), (int)m, trials, seed,
+                                         trials > 0 ? QSF_ELIM_SETS | QSF_ELIM_BOUND : QSF_ELIM_SETS);
         if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
         std::vector<SetRequest*> again;
         for (size_t k = 0; k < m; ++k) {
             SetRequest& r = *todo[k];
             const qsf_elim_job& q = jobs[k];
+            if (trials > 0) r.best = q.best;
             if (q.nsets > q.set_cap) {
                 cap[todo[k]] = q.nsets;
                 again.push_back(todo[k]);
@@ -121,6 +133,7 @@ void device_sets(std::vector<SetRequest*>& reqs) {
             }
         }
         todo.swap(again);
+        trials = 0;   // a redone peel needs no second bound
     }
 }
 
@@ -286,10 +299,18 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
     }
     if (live.empty()) return;
     const BzOptions& o = *live[0]->opt;
+    // without proposal candidates the bound rides along with the peels (one device call);
+    // with them it has a call of its own, so that the candidates' peels run no trials
+    bool fuse = true;
+    for (size_t i = 0; i < sides.size() && i < proposals.size(); ++i)
+        if (sides[i]->has && proposals[i] && proposals[i]->candidates > 0 && (int)proposals[i]->scores.size() == sides[i]->n)
+            fuse = false;
     auto ta = clk::now();
-    const int rc = qsf_hip_eliminate(jobs.data(), (int)jobs.size(), o.is_trials, o.seed, QSF_ELIM_BOUND);
-    if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
-    for (size_t k = 0; k < live.size(); ++k) live[k]->inner = jobs[k].best;
+    if (!fuse) {
+        const int rc = qsf_hip_eliminate(jobs.data(), (int)jobs.size(), o.is_trials, o.seed, QSF_ELIM_BOUND);
+        if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
+        for (size_t k = 0; k < live.size(); ++k) live[k]->inner = jobs[k].best;
+    }
     auto tb = clk::now();
     const double t_bound = std::chrono::duration<double, std::milli>(tb - ta).count();
 
@@ -301,8 +322,13 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
     std::vector<SetRequest*> reqs;
     for (size_t i = 0; i < ns; ++i) {
         if (!sides[i]->has) continue;
-        greedy[i].gen = &sides[i]->prob->gen;
+        Side& S = *sides[i];
+        greedy[i].gen = &S.prob->gen;
         greedy[i].order = orders.empty() ? nullptr : orders[i];
+        greedy[i].check = S.check.data();
+        greedy[i].kcheck = S.kcheck;
+        greedy[i].every_nonzero = S.prob->every_nonzero ? 1 : 0;
+        greedy[i].init = S.inner;
         reqs.push_back(&greedy[i]);
         const Proposal* pr = proposals.empty() ? nullptr : proposals[i];
         if (!pr || pr->candidates <= 0 || (int)pr->scores.size() != sides[i]->n) continue;
@@ -315,7 +341,10 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
             reqs.push_back(&cand_req[i][t]);
         }
     }
-    device_sets(reqs);
+    device_sets(reqs, fuse ? o.is_trials : 0, o.seed);
+    if (fuse && o.is_trials > 0)
+        for (size_t i = 0; i < ns; ++i)
+            if (sides[i]->has) sides[i]->inner = greedy[i].best;
     auto tm = clk::now();
     // candidates were peeled for their ranks only; the winner's generators are peeled again
     std::vector<SetRequest*> winners;

@@ -1084,6 +1084,104 @@ __global__ __launch_bounds__(256) void bz_rowsweep(SweepArgs a) {
 }
 
 // =====================================================================================
+// bz_sweep_small: the small levels (< 2^16 codewords) of many problems in one launch
+// =====================================================================================
+//
+// The row-sweep kernel above gives each thread a run of at least 64 combinations, which
+// suits large levels; a level of a few thousand combinations then runs on a few hundred
+// threads, each waiting on one global load after another (the d = 3 level of a [64, 32]
+// code, two generators: 83 us for 9920 combinations each). Here each workgroup stages its
+// generator's rows and the binomials of the unrank in LDS when they fit, each thread takes
+// a run of at most 32 combinations, and every small level of a batched round shares one
+// launch and one read-back. Same two-level sweep and the same result as bz_rowsweep.
+__global__ __launch_bounds__(256) void bz_sweep_small(SweepJobsArgs a) {
+    __shared__ u64 srows[kSweepRowsLds];
+    __shared__ u64 sbin[kSweepBinLds];
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = a.njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.jobs[mid].block_base <= b) lo = mid; else hi = mid - 1;
+    }
+    const SJob& J = a.jobs[lo];
+    const int K = J.K, D = J.d, M = D - 1, st = J.stride, bpg = J.blocks_per_g;
+    const int g = (int)((b - J.block_base) / (unsigned)bpg);
+    const int lb = (int)((b - J.block_base) - (u64)g * bpg);
+    const u64* G = J.gamma + (size_t)g * K * st;
+    const bool rows_lds = K * st <= kSweepRowsLds, bin_lds = K * D <= kSweepBinLds;
+    if (rows_lds)
+        for (int i = threadIdx.x; i < K * st; i += 256) srows[i] = G[i];
+    if (bin_lds)
+        for (int i = threadIdx.x; i < K * D; i += 256) sbin[i] = J.binom[(size_t)(i / D) * J.bk + (i % D)];
+    __syncthreads();
+    const u64* R = rows_lds ? srows : G;
+    auto binom = [&](int n, int k) -> u64 {   // n in [0, K), k in [0, D)
+        if (k > n) return 0ull;
+        return bin_lds ? sbin[n * D + k] : J.binom[(size_t)n * J.bk + k];
+    };
+
+    const u64 start = ((u64)lb * 256 + threadIdx.x) * (u64)J.chunk;
+    int mine = QSF_WEIGHT_NONE;
+    if (start < J.total) {
+        const u64 end = min(J.total, start + (u64)J.chunk);
+        int pos[kMaxD];
+        u64 base[16], cw[16];
+        {   // lexicographic unrank of `start`
+            u64 r = start;
+            int x = 0;
+            for (int i = 0; i < D; ++i)
+                for (;;) {
+                    const u64 cnt = binom(K - 1 - x, D - 1 - i);
+                    if (r < cnt) { pos[i] = x++; break; }
+                    r -= cnt;
+                    ++x;
+                }
+        }
+        for (int w = 0; w < st; ++w) base[w] = 0;
+        for (int i = 0; i < M; ++i)
+            for (int w = 0; w < st; ++w) base[w] ^= R[(size_t)pos[i] * st + w];
+        int bound = __hip_atomic_load(J.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int last = pos[D - 1];
+        u64 done = 0;
+        const u64 need = end - start;
+        while (done < need) {
+            int top = K;
+            if ((u64)(K - last) > need - done) top = last + (int)(need - done);
+            const int first = last;
+            for (; last < top; ++last) {
+                int wt = 0;
+                for (int w = 0; w < st; ++w) {
+                    cw[w] = base[w] ^ R[(size_t)last * st + w];
+                    wt += __popcll(cw[w]);
+                }
+                if (wt >= bound) continue;
+                bool logical = J.kcheck == 0;
+                for (int c = 0; c < J.kcheck && !logical; ++c) {
+                    u64 acc = 0;
+                    for (int w = 0; w < st; ++w) acc ^= J.check[(size_t)c * st + w] & cw[w];
+                    logical = __popcll(acc) & 1;
+                }
+                if (logical) { bound = wt; mine = wt; }
+            }
+            done += (u64)(top - first);
+            if (done >= need) break;
+            int j = M - 1;
+            while (j >= 0 && pos[j] == K - D + j) --j;
+            if (j < 0) break;
+            for (int t = j; t < M; ++t)
+                for (int w = 0; w < st; ++w) base[w] ^= R[(size_t)pos[t] * st + w];
+            ++pos[j];
+            for (int t = j + 1; t < M; ++t) pos[t] = pos[t - 1] + 1;
+            for (int t = j; t < M; ++t)
+                for (int w = 0; w < st; ++w) base[w] ^= R[(size_t)pos[t] * st + w];
+            last = pos[M - 1] + 1;
+        }
+    }
+    const int wm = wave_min(mine);
+    if ((threadIdx.x & 63) == 0 && wm < QSF_WEIGHT_NONE) atomicMin(J.best, wm);
+}
+
+// =====================================================================================
 // host side
 // =====================================================================================
 
@@ -1401,7 +1499,9 @@ struct Engine {
     DevBuf<int> el_int;        // bounds, set counts, ranks, partitions
     DevBuf<u64> el_scratch;
     DevBuf<ElimJob> el_jobs;
-    std::vector<ElimJob> el_hjobs;
+    unsigned char* el_pin = nullptr;   // pinned staging of an elimination call: [u64 area | int area | jobs]
+    size_t el_pin_cap = 0;
+    hipEvent_t el_b0 = nullptr, el_b1 = nullptr;   // random-IS kernels' start/stop
     qsf_hip_elim_stats el_stats{};
 
     int eliminate(qsf_elim_job* jobs, int njobs, int trials, u64 seed, int what) {
@@ -1428,14 +1528,26 @@ struct Engine {
         const int batch = !want_bound ? 0
                           : (int)std::max<size_t>(1, std::min<size_t>((size_t)trials, scratch_trial ? (1ull << 27) / scratch_trial : (size_t)trials));
         const size_t nscratch = std::max(scratch_sets, scratch_trial * (size_t)batch);
-        std::vector<u64> hu(nu, 0);
-        std::vector<int> hi(ni, 0);
+        // staging in pinned memory: the copies are asynchronous DMA, not runtime-staged
+        const size_t o_hi = align16(nu * 8), o_hj = align16(o_hi + ni * 4);
+        const size_t pin_bytes = o_hj + (size_t)njobs * sizeof(ElimJob) + 16;
+        if (pin_bytes > el_pin_cap) {
+            if (el_pin) (void)hipHostFree(el_pin);
+            el_pin = nullptr;
+            el_pin_cap = 0;
+            HIPCHK(hipHostMalloc((void**)&el_pin, pin_bytes));
+            el_pin_cap = pin_bytes;
+        }
+        std::memset(el_pin, 0, o_hj);
+        u64* hu = reinterpret_cast<u64*>(el_pin);
+        int* hi = reinterpret_cast<int*>(el_pin + o_hi);
+        ElimJob* hj = reinterpret_cast<ElimJob*>(el_pin + o_hj);
         for (int j = 0; j < njobs; ++j) {
             const qsf_elim_job& q = jobs[j];
             const int st = (q.n + 63) / 64;
-            std::memcpy(hu.data() + u_off[j], q.gen, (size_t)q.K * st * 8);
-            if (q.kcheck > 0) std::memcpy(hu.data() + u_off[j] + (size_t)q.K * st, q.check, (size_t)q.kcheck * st * 8);
-            int* h = hi.data() + i_off[j];
+            std::memcpy(hu + u_off[j], q.gen, (size_t)q.K * st * 8);
+            if (q.kcheck > 0) std::memcpy(hu + u_off[j] + (size_t)q.K * st, q.check, (size_t)q.kcheck * st * 8);
+            int* h = hi + i_off[j];
             h[0] = q.init;
             int* po = h + 2 + (want_sets ? q.set_cap : 0);
             if (q.nparts > 0) {
@@ -1449,13 +1561,13 @@ struct Engine {
         el_int.reserve(std::max<size_t>(ni, 1));
         if (nscratch) el_scratch.reserve(nscratch);
         el_jobs.reserve(njobs);
-        el_hjobs.assign(njobs, ElimJob{});
+        for (int j = 0; j < njobs; ++j) hj[j] = ElimJob{};
         size_t sc = 0;
         size_t lds_sets = 0, lds_trials = 0;
         for (int j = 0; j < njobs; ++j) {
             const qsf_elim_job& q = jobs[j];
             const int st = (q.n + 63) / 64;
-            ElimJob& e = el_hjobs[j];
+            ElimJob& e = hj[j];
             u64* base = el_u64.p + u_off[j];
             int* ib = el_int.p + i_off[j];
             e.gen = base;
@@ -1482,40 +1594,37 @@ struct Engine {
         for (int j = 0; j < njobs; ++j) {   // inputs only: generator and detector rows
             const qsf_elim_job& q = jobs[j];
             const size_t w = (size_t)q.K * ((q.n + 63) / 64) + (size_t)std::max(0, q.kcheck) * ((q.n + 63) / 64);
-            if (w) HIPCHK(hipMemcpyAsync(el_u64.p + u_off[j], hu.data() + u_off[j], w * 8, hipMemcpyHostToDevice, stream));
+            if (w) HIPCHK(hipMemcpyAsync(el_u64.p + u_off[j], hu + u_off[j], w * 8, hipMemcpyHostToDevice, stream));
         }
-        HIPCHK(hipMemcpyAsync(el_int.p, hi.data(), ni * 4, hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(el_jobs.p, el_hjobs.data(), njobs * sizeof(ElimJob), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(el_int.p, hi, ni * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(el_jobs.p, hj, njobs * sizeof(ElimJob), hipMemcpyHostToDevice, stream));
         if (want_sets) {
             HIPCHK(hipEventRecord(ev0, stream));
             HIPCHK(launch_info_sets(el_jobs.p, njobs, lds_sets, stream));
             HIPCHK(hipEventRecord(ev1, stream));
         }
-        hipEvent_t b0 = nullptr, b1 = nullptr;
         if (want_bound) {
-            HIPCHK(hipEventCreate(&b0));
-            HIPCHK(hipEventCreate(&b1));
-            HIPCHK(hipEventRecord(b0, stream));
+            if (!el_b0) HIPCHK(hipEventCreate(&el_b0));
+            if (!el_b1) HIPCHK(hipEventCreate(&el_b1));
+            HIPCHK(hipEventRecord(el_b0, stream));
             for (int t0 = 0; t0 < trials; t0 += batch)
                 HIPCHK(launch_is_trials(el_jobs.p, njobs, lds_trials, seed, t0, std::min(batch, trials - t0), stream));
-            HIPCHK(hipEventRecord(b1, stream));
+            HIPCHK(hipEventRecord(el_b1, stream));
         }
-        if (want_sets) HIPCHK(hipMemcpyAsync(hu.data(), el_u64.p, nu * 8, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(hi.data(), el_int.p, ni * 4, hipMemcpyDeviceToHost, stream));
+        if (want_sets) HIPCHK(hipMemcpyAsync(hu, el_u64.p, nu * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(hi, el_int.p, ni * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         float ms = 0;
         if (want_sets) { HIPCHK(hipEventElapsedTime(&ms, ev0, ev1)); el_stats.sets_ms = ms; }
         if (want_bound) {
-            HIPCHK(hipEventElapsedTime(&ms, b0, b1));
+            HIPCHK(hipEventElapsedTime(&ms, el_b0, el_b1));
             el_stats.bound_ms = ms;
             el_stats.trials = trials;
-            (void)hipEventDestroy(b0);
-            (void)hipEventDestroy(b1);
         }
         for (int j = 0; j < njobs; ++j) {
             qsf_elim_job& q = jobs[j];
             const int st = (q.n + 63) / 64;
-            const int* h = hi.data() + i_off[j];
+            const int* h = hi + i_off[j];
             q.best = want_bound ? std::min(q.init, h[0]) : q.init;
             if (!want_sets) continue;
             q.nsets = h[1];
@@ -1526,7 +1635,7 @@ struct Engine {
             const int keep = std::min(q.nsets, q.set_cap);
             std::memcpy(q.ranks, h + 2, (size_t)keep * 4);
             if (!q.sets) continue;
-            std::memcpy(q.sets, hu.data() + u_off[j] + (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st,
+            std::memcpy(q.sets, hu + u_off[j] + (size_t)q.K * st + (size_t)std::max(0, q.kcheck) * st,
                         (size_t)keep * q.K * st * 8);
             el_stats.sets_bytes += (u64)keep * q.K * st * 8;   // systematic generators written
         }
@@ -1553,6 +1662,13 @@ struct Engine {
             *best_out = std::min(res[0], p.current_best);
             return 0;
         }
+        if (small_sweep(p, stats.codewords)) {
+            std::vector<SItem> items{SItem{&s, &p}};
+            std::vector<int> res;
+            run_sweep_small(items, res);
+            *best_out = std::min(res[0], p.current_best);
+            return 0;
+        }
         best.reserve(1);
         counter.reserve(4);
         HIPCHK(hipMemcpyAsync(best.p, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
@@ -1575,7 +1691,8 @@ struct Engine {
         cache_cap = std::max(cache_cap, (size_t)n + 6);
         std::vector<Solve*> solves = solves_for(plans, n);   // cache_cap > n: none evicted
         std::vector<TItem> items;
-        std::vector<int> item_of(n, -1), sweep;
+        std::vector<SItem> small;
+        std::vector<int> item_of(n, -1), small_of(n, -1), sweep;
         for (int j = 0; j < n; ++j) {
             const qsf_enum_plan& p = plans[j];
             Solve& s = *solves[j];
@@ -1588,6 +1705,9 @@ struct Engine {
             if (tileable(s, p) && policy != 1) {
                 item_of[j] = (int)items.size();
                 items.push_back(TItem{&s, p, floors ? floors[j] : -1, choose_variant(s, p, true, matrix_min_work() >> 6)});
+            } else if (small_sweep(p, cw)) {
+                small_of[j] = (int)small.size();
+                small.push_back(SItem{&s, &plans[j]});
             } else {
                 sweep.push_back(j);
             }
@@ -1619,6 +1739,13 @@ struct Engine {
             for (int j = 0; j < n; ++j)
                 if (item_of[j] >= 0) best_out[j] = std::min(res[item_of[j]], plans[j].current_best);
         }
+        if (!small.empty()) {
+            std::vector<int> res;
+            run_sweep_small(small, res);
+            ms_total += stats.kernel_ms;
+            for (int j = 0; j < n; ++j)
+                if (small_of[j] >= 0) best_out[j] = std::min(res[small_of[j]], plans[j].current_best);
+        }
         best.reserve(1);
         counter.reserve(4);
         for (int j : sweep) {
@@ -1634,8 +1761,68 @@ struct Engine {
             best_out[j] = std::min(result, p.current_best);
         }
         stats.kernel_ms = ms_total;
-        stats.kernel = items.empty() ? (sweep.empty() ? 0 : 1) : 2;
+        stats.kernel = !items.empty() ? 2 : !sweep.empty() ? 1 : !small.empty() ? 3 : 0;
         return 0;
+    }
+
+    // ---- small levels -----------------------------------------------------------------
+    // Levels under 2^16 codewords (min mode, automatic policy) go to bz_sweep_small; the
+    // row-sweep policy keeps them on bz_rowsweep, the second implementation of the tests.
+    bool small_sweep(const qsf_enum_plan& p, u64 codewords) const {
+        return policy == 0 && codewords < (1ull << 16) && p.d >= 1 && p.d <= kMaxD && p.stride <= 16;
+    }
+
+    struct SItem {
+        Solve* s;
+        const qsf_enum_plan* p;
+    };
+
+    // One bz_sweep_small launch for the levels of `items`: one upload of
+    // [items x int bounds | items x SJob], one read-back of the bounds.
+    void run_sweep_small(const std::vector<SItem>& items, std::vector<int>& res) {
+        const int ni = (int)items.size();
+        const size_t o_jobs = align16((size_t)ni * sizeof(int));
+        const size_t total = align16(o_jobs + (size_t)ni * sizeof(SJob));
+        pin_reserve(total);
+        blk.reserve(total);
+        unsigned char* dev = blk.p;
+        int* hb = reinterpret_cast<int*>(pin);
+        SJob* hj = reinterpret_cast<SJob*>(pin + o_jobs);
+        u64 blocks = 0;
+        for (int i = 0; i < ni; ++i) {
+            const Solve& s = *items[i].s;
+            const qsf_enum_plan& p = *items[i].p;
+            hb[i] = p.current_best;
+            SJob J{};
+            J.gamma = s.gamma.p;
+            J.check = s.check.p;
+            J.binom = s.dbinom.p;
+            J.best = reinterpret_cast<int*>(dev) + i;
+            J.total = s.C(p.K, p.d);
+            J.K = p.K; J.d = p.d; J.stride = p.stride; J.kcheck = p.kcheck; J.ngamma = p.num_gamma; J.bk = s.bk;
+            // runs of <= 32 combinations, >= 1024 threads per generator when the level has them
+            J.chunk = (int)std::max<u64>(1, std::min<u64>(32, (J.total + 1023) / 1024));
+            const u64 threads = (J.total + (u64)J.chunk - 1) / (u64)J.chunk;
+            J.blocks_per_g = (int)((threads + 255) / 256);
+            J.block_base = blocks;
+            blocks += (u64)J.blocks_per_g * (u64)p.num_gamma;
+            hj[i] = J;
+        }
+        HIPCHK(hipMemcpyAsync(dev, pin, total, hipMemcpyHostToDevice, stream));
+        SweepJobsArgs a{reinterpret_cast<const SJob*>(dev + o_jobs), ni};
+        stats.kernel = 3;
+        HIPCHK(hipEventRecord(ev0, stream));
+        if (blocks > 0) {
+            hipLaunchKernelGGL(bz_sweep_small, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(ev1, stream));
+        HIPCHK(hipMemcpyAsync(pin, dev, (size_t)ni * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        res.assign(hb, hb + ni);
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.kernel_ms = ms;
     }
 
     // Kernel variant: FD residual dwords tested in the hot loop, XD held for the medium

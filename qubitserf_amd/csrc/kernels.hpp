@@ -117,4 +117,27 @@ struct SweepArgs {
     u64 out_cap;             // rows
 };
 
+// bz_sweep_small: many small levels (min mode) in one launch. Workgroup b serves one
+// (level, generator) pair: the pair's job is the last one with block_base <= b, the
+// generator (b - block_base) / blocks_per_g.
+struct SJob {
+    const u64* gamma;        // [g][row][stride] full rows
+    const u64* check;        // [c][stride]
+    const u64* binom;        // the solve's C(n, k) at binom[n * bk + k]
+    int* best;               // running bound; starts at current_best
+    u64 total;               // C(K, d)
+    u64 block_base;          // first workgroup of this job
+    int K, d, stride, kcheck, ngamma, bk;
+    int blocks_per_g;        // workgroups per active generator
+    int chunk;               // lexicographic combinations per thread
+};
+
+struct SweepJobsArgs {
+    const SJob* jobs;        // sorted by block_base
+    int njobs;
+};
+
+constexpr int kSweepRowsLds = 2048;   // u64: a generator's rows are staged in LDS up to this size
+constexpr int kSweepBinLds = 1024;    // u64: binomials C(n < K, k < d) staged in LDS up to this count
+
 }  // namespace qsa

@@ -172,6 +172,19 @@ def test_tiled_large_levels_with_groups(hip):
         assert st["kernel"] == 2
 
 
+def test_small_level_sweep(hip):
+    """Levels under 2^16 codewords in the automatic policy run the small-level sweep (rows
+    and binomials staged in LDS when they fit, read from global memory when not: K x stride
+    > 2048 words, K x d > 1024): results equal the oracle's, loose and tight bounds."""
+    rng = np.random.default_rng(2100)
+    cases = [(32, 1, 3, 2, 0), (12, 2, 6, 3, 2), (64, 2, 2, 1, 1), (200, 4, 2, 2, 0), (300, 16, 1, 1, 3),
+             (40, 16, 2, 2, 2), (20, 1, 19, 1, 0), (66, 3, 64, 1, 1), (150, 3, 2, 3, 2)]
+    for i, (K, stride, d, ng, kcheck) in enumerate(cases):
+        for best in (NONE, stride * 64 // 3, 3):
+            run_case(hip, rng, K, stride, d, ng, kcheck, best, systematic=bool(i % 2))
+            assert hip.last_stats()["kernel"] == 3, (K, d)
+
+
 @pytest.mark.parametrize("fd", [1, 2, 3, 4])
 def test_tiled_wide_rows(hip, fd, monkeypatch):
     """K > 128: the VALU kernel with two-word list masks (each half of the rows up to 128
