@@ -127,13 +127,15 @@ def _certified(rs):
     return sum(1 for r in rs if r.proven)
 
 
-def batched_learned(log):
+def batched_learned(log, c4_cap=int(1e11)):
     """Plain vs batched+learned time-to-certify on the batched (256 random codes) and learned
     (quasi-cyclic evaluation codes) workloads, the elimination phase's HBM rate, and the
     serving loop. Plain = one code per call with the natural information sets; batched +
     learned = one call for the whole set with the trained scorer's proposals (scorer
     inference included). Every code carries the same work cap in both forms; a code over
-    the cap is reported unproven (the full sets need ~1e19 codewords)."""
+    the cap is reported unproven (the full sets need ~1e19 codewords). c4_cap: the
+    quasi-cyclic codes' cap (1e11 by default; at 1e13 more of them certify and the run
+    takes ~12 s)."""
     import numpy as np
     from qubitserf_amd import codes, engine, serving
     from qubitserf_amd import distfind as df
@@ -171,7 +173,7 @@ def batched_learned(log):
 
     c3 = codes.config3_batch(0, 256)
     out["batched_c3"] = compare("batched", c3, int(1e10), 8)
-    out["learned_c4"] = compare("learned", codes.config4_eval(0, 16, 20, 40), int(1e11), 16)
+    out["learned_c4"] = compare("learned", codes.config4_eval(0, 16, 20, 40), int(c4_cap), 16)
     # elimination phase: information sets of all 256 codes in one launch
     probs = [dict(gen=engine.pack_rows(codes.gf2_kernel(H)), n=H.shape[1], set_cap=8) for H in c3]
     engine.eliminate(probs, what=engine.ELIM_SETS)
@@ -193,6 +195,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched / learned / serving figures")
+    ap.add_argument("--c4-cap", type=float, default=1e11, help="work cap per quasi-cyclic code (learned figure)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -289,7 +292,7 @@ def main():
         },
     }
     if not args.no_extras:
-        line["batched_learned"] = batched_learned(lambda m: print(m, file=sys.stderr, flush=True))
+        line["batched_learned"] = batched_learned(lambda m: print(m, file=sys.stderr, flush=True), int(args.c4_cap))
         line["config1"] = config1_gpu()
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)

@@ -172,6 +172,19 @@ def test_tiled_large_levels_with_groups(hip):
         assert st["kernel"] == 2
 
 
+@pytest.mark.parametrize("K,stride,d,kern", [(128, 3, 3, 2), (129, 4, 3, 2), (256, 5, 3, 2), (257, 5, 3, 1),
+                                              (34, 1, 32, 2), (36, 1, 33, 1)])
+def test_kernel_boundaries(hip, K, stride, d, kern):
+    """Either side of the kernels' limits, tiled policy, against the oracle: K = 128 / 129
+    (one- and two-word list masks), K = 256 / 257 (tiled kernel / row sweep), d = 32 / 33
+    (tiled kernel / row sweep)."""
+    rng = np.random.default_rng(K * 7 + d)
+    with hip.kernel_policy("tiled"):
+        for best in (NONE, 30, 8):
+            run_case(hip, rng, K, stride, d, 2, 1, best, systematic=True)
+            assert hip.last_stats()["kernel"] == kern, (K, d, hip.last_stats())
+
+
 def test_small_level_sweep(hip):
     """Levels under 2^16 codewords in the automatic policy run the small-level sweep (rows
     and binomials staged in LDS when they fit, read from global memory when not: K x stride
