@@ -6,8 +6,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/codeaut_capi.h"
@@ -224,43 +226,73 @@ thread_local BatchStats g_batch;
 namespace {
 
 // Batch items -> problems (problem build on the host, as the single entry points do).
+// One batch item's problem (kernel bases, detectors, proposal, order); rc != 0 with err set
+// for a malformed item.
+int build_problem(const DistfindBatchItem* items, int i, BatchProblem& bp, std::string& err) {
+    const DistfindBatchItem& it = items[i];
+    if (!it.A || (it.kind == 1 && !it.B)) { err = "null matrix in batch item " + std::to_string(i); return RC_NULL; }
+    if (it.kind == 0) {
+        bp.a = classical(BitMat::from_bytes(it.A, it.ar, it.ac));
+    } else if (it.kind == 1) {
+        if (it.ac != it.bc) { err = "Hx and Hz column counts differ in batch item " + std::to_string(i); return RC_NULL; }
+        BitMat Hx = BitMat::from_bytes(it.A, it.ar, it.ac), Hz = BitMat::from_bytes(it.B, it.br, it.bc);
+        const char w = it.which == 'z' ? 'Z' : it.which == 'x' ? 'X' : it.which;
+        if (w == 'Z' || w == 'X') {
+            bp.a = css_side(Hx, Hz, w);
+        } else {
+            bp.a = css_side(Hx, Hz, 'Z');
+            bp.b = css_side(Hx, Hz, 'X');
+            bp.pair = true;
+        }
+    } else {
+        err = "batch item kind must be 0 (classical) or 1 (CSS)";
+        return RC_METHOD;
+    }
+    bp.work_cap = it.work_cap > 0 ? (u64)it.work_cap : 0;
+    if (it.scores && it.candidates > 0 && !bp.pair) {
+        bp.proposal.scores.assign(it.scores, it.scores + it.ac);
+        bp.proposal.candidates = std::min(it.candidates, 1024);
+        bp.proposal.seed = it.proposal_seed;
+    }
+    if (it.order && !bp.pair) {
+        const int n = it.ac;
+        bp.order.assign(it.order, it.order + n);
+        std::vector<char> seen(n, 0);
+        for (int c : bp.order) {
+            if (c < 0 || c >= n || seen[c]) { err = "batch item order is not a permutation"; return RC_NULL; }
+            seen[c] = 1;
+        }
+    }
+    return RC_OK;
+}
+
+// All items' problems, on up to 16 host threads for large batches; the first malformed
+// item (by index) sets the error.
 int build_problems(const DistfindBatchItem* items, int count, std::vector<BatchProblem>& probs) {
     probs.assign(count, BatchProblem{});
+    std::vector<int> rcs(count, RC_OK);
+    std::vector<std::string> errs(count);
+    std::vector<std::exception_ptr> exc(count);
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int i; (i = next++) < count;) {
+            try {
+                rcs[i] = build_problem(items, i, probs[i], errs[i]);
+            } catch (...) {
+                exc[i] = std::current_exception();
+            }
+        }
+    };
+    const int workers = count >= 32 ? (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())) : 1;
+    std::vector<std::thread> pool;
+    for (int w = 1; w < workers; ++w) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
     for (int i = 0; i < count; ++i) {
-        const DistfindBatchItem& it = items[i];
-        if (!it.A || (it.kind == 1 && !it.B)) { g_err = "null matrix in batch item " + std::to_string(i); return RC_NULL; }
-        BatchProblem& bp = probs[i];
-        if (it.kind == 0) {
-            bp.a = classical(BitMat::from_bytes(it.A, it.ar, it.ac));
-        } else if (it.kind == 1) {
-            if (it.ac != it.bc) { g_err = "Hx and Hz column counts differ in batch item " + std::to_string(i); return RC_NULL; }
-            BitMat Hx = BitMat::from_bytes(it.A, it.ar, it.ac), Hz = BitMat::from_bytes(it.B, it.br, it.bc);
-            const char w = it.which == 'z' ? 'Z' : it.which == 'x' ? 'X' : it.which;
-            if (w == 'Z' || w == 'X') {
-                bp.a = css_side(Hx, Hz, w);
-            } else {
-                bp.a = css_side(Hx, Hz, 'Z');
-                bp.b = css_side(Hx, Hz, 'X');
-                bp.pair = true;
-            }
-        } else {
-            g_err = "batch item kind must be 0 (classical) or 1 (CSS)";
-            return RC_METHOD;
-        }
-        bp.work_cap = it.work_cap > 0 ? (u64)it.work_cap : 0;
-        if (it.scores && it.candidates > 0 && !bp.pair) {
-            bp.proposal.scores.assign(it.scores, it.scores + it.ac);
-            bp.proposal.candidates = std::min(it.candidates, 1024);
-            bp.proposal.seed = it.proposal_seed;
-        }
-        if (it.order && !bp.pair) {
-            const int n = it.ac;
-            bp.order.assign(it.order, it.order + n);
-            std::vector<char> seen(n, 0);
-            for (int c : bp.order) {
-                if (c < 0 || c >= n || seen[c]) { g_err = "batch item order is not a permutation"; return RC_NULL; }
-                seen[c] = 1;
-            }
+        if (exc[i]) std::rethrow_exception(exc[i]);
+        if (rcs[i] != RC_OK) {
+            g_err = errs[i];
+            return rcs[i];
         }
     }
     return RC_OK;

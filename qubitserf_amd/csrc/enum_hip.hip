@@ -1591,10 +1591,25 @@ struct Engine {
             lds_trials = std::max(lds_trials, is_trials_lds_bytes(e));
             if (want_sets) el_stats.sets_bytes += (u64)q.K * st * 8;   // generator read once
         }
-        for (int j = 0; j < njobs; ++j) {   // inputs only: generator and detector rows
-            const qsf_elim_job& q = jobs[j];
-            const size_t w = (size_t)q.K * ((q.n + 63) / 64) + (size_t)std::max(0, q.kcheck) * ((q.n + 63) / 64);
-            if (w) HIPCHK(hipMemcpyAsync(el_u64.p + u_off[j], hu + u_off[j], w * 8, hipMemcpyHostToDevice, stream));
+        {   // inputs: generator and detector rows; neighbouring jobs' ranges in one copy when
+            // the output area between them is small (it is overwritten on the device)
+            constexpr size_t kGap = 32768;   // u64
+            size_t lo = 0, hi_end = 0;
+            bool open = false;
+            for (int j = 0; j < njobs; ++j) {
+                const qsf_elim_job& q = jobs[j];
+                const size_t w = (size_t)q.K * ((q.n + 63) / 64) + (size_t)std::max(0, q.kcheck) * ((q.n + 63) / 64);
+                if (!w) continue;
+                if (open && u_off[j] - hi_end <= kGap) {
+                    hi_end = u_off[j] + w;
+                    continue;
+                }
+                if (open) HIPCHK(hipMemcpyAsync(el_u64.p + lo, hu + lo, (hi_end - lo) * 8, hipMemcpyHostToDevice, stream));
+                lo = u_off[j];
+                hi_end = u_off[j] + w;
+                open = true;
+            }
+            if (open) HIPCHK(hipMemcpyAsync(el_u64.p + lo, hu + lo, (hi_end - lo) * 8, hipMemcpyHostToDevice, stream));
         }
         HIPCHK(hipMemcpyAsync(el_int.p, hi, ni * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(el_jobs.p, hj, njobs * sizeof(ElimJob), hipMemcpyHostToDevice, stream));
