@@ -1317,15 +1317,12 @@ struct Engine {
             }
         }
         auto s = std::make_unique<Solve>();
-        build(*s, p);
+        build_host(*s, p);
+        evict(cache_cap > 0 ? cache_cap - 1 : 0);   // room for the new solve, which may take a leaver's arena
+        std::vector<Solve*> one{s.get()};
+        upload_all(one);
         cache.push_front(std::move(s));
-        while (cache.size() > cache_cap) cache.pop_back();
         return *cache.front();
-    }
-
-    void build(Solve& s, const qsf_enum_plan& p) {
-        build_host(s, p);
-        upload(s);
     }
 
     // Solves for many plans: the missing ones' host tables are built on several threads,
@@ -1355,30 +1352,37 @@ struct Engine {
             fresh.emplace_back(out[j], &p);
             if (p.buffers_key != 0) made[p.buffers_key] = out[j];
         }
-        const int workers = (int)std::min<size_t>(fresh.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
-        if (workers > 1) {
-            std::atomic<size_t> next{0};
-            std::vector<std::thread> pool;
-            std::exception_ptr fail;
-            std::mutex fail_mu;
-            for (int w = 0; w < workers; ++w)
-                pool.emplace_back([&] {
-                    for (size_t i; (i = next++) < fresh.size();) {
-                        try {
-                            build_host(*fresh[i].first, *fresh[i].second);
-                        } catch (...) {
-                            std::lock_guard<std::mutex> lk(fail_mu);
-                            fail = std::current_exception();
+        std::vector<Solve*> up;
+        for (auto& f : fresh) up.push_back(f.first);
+        try {   // a solve whose build fails leaves the cache
+            const int workers = (int)std::min<size_t>(fresh.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+            if (workers > 1) {
+                std::atomic<size_t> next{0};
+                std::vector<std::thread> pool;
+                std::exception_ptr fail;
+                std::mutex fail_mu;
+                for (int w = 0; w < workers; ++w)
+                    pool.emplace_back([&] {
+                        for (size_t i; (i = next++) < fresh.size();) {
+                            try {
+                                build_host(*fresh[i].first, *fresh[i].second);
+                            } catch (...) {
+                                std::lock_guard<std::mutex> lk(fail_mu);
+                                fail = std::current_exception();
+                            }
                         }
-                    }
-                });
-            for (auto& t : pool) t.join();
-            if (fail) std::rethrow_exception(fail);
-        } else {
-            for (auto& f : fresh) build_host(*f.first, *f.second);
+                    });
+                for (auto& t : pool) t.join();
+                if (fail) std::rethrow_exception(fail);
+            } else {
+                for (auto& f : fresh) build_host(*f.first, *f.second);
+            }
+            evict(cache_cap);   // older solves leave first (the new ones are at the front), so
+            upload_all(up);     // that the new ones can take their arenas
+        } catch (...) {
+            for (Solve* f : up) cache.remove_if([&](const std::unique_ptr<Solve>& u) { return u.get() == f; });
+            throw;
         }
-        for (auto& f : fresh) upload(*f.first);
-        while (cache.size() > cache_cap) cache.pop_back();
         return out;
     }
 
@@ -1468,7 +1472,7 @@ struct Engine {
         // one staging vector, one allocation, one copy
         const size_t n_gamma = (size_t)s.ngt * K * st, n_check = (size_t)std::max(1, p.kcheck) * st;
         const size_t n_binom = s.binom.size(), n_flags = fl.size(), n_tbl = (tbl.size() + 1) / 2;
-        std::vector<u64>& stage = s.stage;   // kept with the solve: the async copy reads it
+        std::vector<u64>& stage = s.stage;   // host image of the device arena
         stage.assign(n_gamma + n_check + n_binom + n_flags + n_tbl, 0);
         u64* at = stage.data();
         std::memcpy(at, p.gamma, n_gamma * 8);
@@ -1481,17 +1485,70 @@ struct Engine {
         s.hflags = std::move(fl);
     }
 
-    void upload(Solve& s) {
-        const std::vector<u64>& stage = s.stage;
-        const size_t n_gamma = s.n_gamma, n_check = s.n_check, n_binom = s.n_binom, n_flags = s.n_flags;
-        s.arena.reserve(stage.size());
-        u64* d = s.arena.p;
-        s.gamma.p = d;
-        s.check.p = d + n_gamma;
-        s.dbinom.p = d + n_gamma + n_check;
-        s.flags.p = d + n_gamma + n_check + n_binom;
-        s.tbl.p = reinterpret_cast<u32*>(d + n_gamma + n_check + n_binom + n_flags);
-        HIPCHK(hipMemcpyAsync(d, stage.data(), stage.size() * 8, hipMemcpyHostToDevice, stream));
+    // Solves past the cache capacity leave; their device arenas go to a spare list that new
+    // solves take from (a batch of new codes would otherwise pay one hipMalloc and one
+    // hipFree per code). Eviction happens between calls, after the stream has drained.
+    std::vector<std::pair<u64*, size_t>> spare_arenas;
+    void evict(size_t keep) {
+        while (cache.size() > keep) {
+            Solve& s = *cache.back();
+            if (s.arena.p && spare_arenas.size() < 1024) {
+                spare_arenas.push_back({s.arena.p, s.arena.n});
+                s.arena.p = nullptr;
+                s.arena.n = 0;
+            }
+            cache.pop_back();
+        }
+    }
+    void take_spare(DevBuf<u64>& a, size_t need) {
+        if (a.n >= need) return;
+        int pick = -1;
+        for (int k = 0; k < (int)spare_arenas.size(); ++k)
+            if (spare_arenas[k].second >= need && (pick < 0 || spare_arenas[k].second < spare_arenas[pick].second)) pick = k;
+        if (pick < 0) return;
+        if (a.p) (void)hipFree(a.p);
+        a.p = spare_arenas[pick].first;
+        a.n = spare_arenas[pick].second;
+        spare_arenas[pick] = spare_arenas.back();
+        spare_arenas.pop_back();
+    }
+
+    // Device arenas of new solves, filled from one pinned staging buffer (asynchronous DMA;
+    // the buffer is reused once the previous uploads have completed).
+    u64* up_pin = nullptr;
+    size_t up_pin_cap = 0;   // u64
+    hipEvent_t up_done = nullptr;
+    void upload_all(const std::vector<Solve*>& ss) {
+        if (ss.empty()) return;
+        size_t total = 0;
+        for (const Solve* s : ss) total += s->stage.size();
+        if (up_done) HIPCHK(hipEventSynchronize(up_done));
+        else HIPCHK(hipEventCreateWithFlags(&up_done, hipEventDisableTiming));
+        if (total > up_pin_cap) {
+            if (up_pin) (void)hipHostFree(up_pin);
+            up_pin = nullptr;
+            up_pin_cap = 0;
+            HIPCHK(hipHostMalloc((void**)&up_pin, std::max<size_t>(total, 1) * 8));
+            up_pin_cap = total;
+        }
+        size_t at = 0;
+        for (Solve* sp : ss) {
+            Solve& s = *sp;
+            const std::vector<u64>& stage = s.stage;
+            const size_t n_gamma = s.n_gamma, n_check = s.n_check, n_binom = s.n_binom, n_flags = s.n_flags;
+            take_spare(s.arena, stage.size());
+            s.arena.reserve(stage.size());
+            u64* d = s.arena.p;
+            s.gamma.p = d;
+            s.check.p = d + n_gamma;
+            s.dbinom.p = d + n_gamma + n_check;
+            s.flags.p = d + n_gamma + n_check + n_binom;
+            s.tbl.p = reinterpret_cast<u32*>(d + n_gamma + n_check + n_binom + n_flags);
+            std::memcpy(up_pin + at, stage.data(), stage.size() * 8);
+            HIPCHK(hipMemcpyAsync(d, up_pin + at, stage.size() * 8, hipMemcpyHostToDevice, stream));
+            at += stage.size();
+        }
+        HIPCHK(hipEventRecord(up_done, stream));
     }
 
     // ---- elimination: information sets and random-IS bounds, many problems per launch --
@@ -2071,11 +2128,14 @@ struct Engine {
         std::vector<u32> hfixed;
         std::vector<size_t> fixed_off(ni, 0);
         std::vector<int> glds(NV, 0);
-        for (int i = 0; i < ni; ++i) {
-            TItem& it = items[i];
+        // each item's decomposition on its own (host threads for large batches), then
+        // concatenated in item order
+        std::vector<std::vector<Rect>> irect(ni);
+        std::vector<std::vector<u32>> ifixed(ni);
+        auto decompose = [&](int i) {
+            const TItem& it = items[i];
             const Solve& s = *it.s;
             const int K = it.p.K, d = it.p.d, h = s.h;
-            const int vi = vidx(it.V);
             RectBuilder B{s, 0, s.RW, stream_len(it.V), lane_tile(it.V), it.V.mm > 0, {}, {}};
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
@@ -2083,10 +2143,39 @@ struct Engine {
                     B.split(RectBuilder::RowSet{}, 0, h, ka, h, K - h, d - ka, 0);
             }
             for (Rect& r : B.rv) r.job = i;
-            grect[vi].insert(grect[vi].end(), B.rv.begin(), B.rv.end());
+            irect[i] = std::move(B.rv);
+            ifixed[i] = std::move(B.fixed);
+        };
+        const int workers = ni >= 16 ? (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())) : 1;
+        if (workers > 1) {
+            std::atomic<int> next{0};
+            std::vector<std::thread> pool;
+            std::exception_ptr fail;
+            std::mutex fail_mu;
+            for (int w = 0; w < workers; ++w)
+                pool.emplace_back([&] {
+                    for (int i; (i = next++) < ni;) {
+                        try {
+                            decompose(i);
+                        } catch (...) {
+                            std::lock_guard<std::mutex> lk(fail_mu);
+                            fail = std::current_exception();
+                        }
+                    }
+                });
+            for (auto& t : pool) t.join();
+            if (fail) std::rethrow_exception(fail);
+        } else {
+            for (int i = 0; i < ni; ++i) decompose(i);
+        }
+        for (int i = 0; i < ni; ++i) {
+            const TItem& it = items[i];
+            const Solve& s = *it.s;
+            const int vi = vidx(it.V);
+            grect[vi].insert(grect[vi].end(), irect[i].begin(), irect[i].end());
             fixed_off[i] = hfixed.size();
-            hfixed.insert(hfixed.end(), B.fixed.begin(), B.fixed.end());
-            if (K * s.RW <= kLdsFullRows) glds[vi] = std::max(glds[vi], K * s.RW);
+            hfixed.insert(hfixed.end(), ifixed[i].begin(), ifixed[i].end());
+            if (it.p.K * s.RW <= kLdsFullRows) glds[vi] = std::max(glds[vi], it.p.K * s.RW);
         }
         // tiles: a lane block x G stream chunks; G large enough to amortise the register
         // tile, small enough to leave >= 8 tiles per resident workgroup for load balance
