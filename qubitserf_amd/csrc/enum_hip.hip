@@ -1978,7 +1978,9 @@ struct Engine {
             // form (124 bits, table[13]) is not chosen automatically: its 256-element stream
             // chunks pay the per-chunk bound reload and barrier four times as often, and on
             // the quasi-cyclic evaluation codes it ran at half the VALU kernel's rate.
-            if (!std::getenv("QSA_NO_F4") && s.RD >= 2 && 8192.0 * binom_cdf_below(std::min(60, res), thr) < 0.02)
+            const char* f4_rate = std::getenv("QSA_F4_RATE");   // tuning: the fp4 rule's bound (default 2%)
+            const double f4_max = f4_rate ? std::atof(f4_rate) : 0.02;
+            if (!std::getenv("QSA_NO_F4") && s.RD >= 2 && 8192.0 * binom_cdf_below(std::min(60, res), thr) < f4_max)
                 return s.RD >= 3 ? table[12] : table[11];
         }
         if (s.RD == 1) return table[0];
