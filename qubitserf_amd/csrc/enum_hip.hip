@@ -517,25 +517,40 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
                     for (int v = 3; v < 15; v += 2) m = min(min(m, c[v]), c[v + 1]);
                     return min(m, c[15]);
                 };
-                int mn = 1 << 30;
+                int mh[2];   // minimum over blocks 0..3 and 4..7, kept for the exact path
 #pragma unroll
                 for (int b0 = 0; b0 < kMfmaBlocks; b0 += 4) {
                     const v16i p0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 0], bv, zero, 0, 0, 0);
                     const v16i p1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 1], bv, zero, 0, 0, 0);
                     const v16i p2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 2], bv, zero, 0, 0, 0);
                     const v16i p3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b0 + 3], bv, zero, 0, 0, 0);
-                    mn = min(mn, min(min(red(p0), red(p1)), min(red(p2), red(p3))));
+                    // one v_min3 chain over the half's 64 values (32 instructions)
+                    int m = min(min(p0[0], p0[1]), p0[2]);
+#pragma unroll
+                    for (int v = 3; v < 15; v += 2) m = min(min(m, p0[v]), p0[v + 1]);
+                    m = min(min(m, p0[15]), p1[0]);
+#pragma unroll
+                    for (int v = 1; v < 15; v += 2) m = min(min(m, p1[v]), p1[v + 1]);
+                    m = min(min(m, p1[15]), p2[0]);
+#pragma unroll
+                    for (int v = 1; v < 15; v += 2) m = min(min(m, p2[v]), p2[v + 1]);
+                    m = min(min(m, p2[15]), p3[0]);
+#pragma unroll
+                    for (int v = 1; v < 15; v += 2) m = min(min(m, p3[v]), p3[v + 1]);
+                    mh[b0 / 4] = min(m, p3[15]);
                 }
-                if (__builtin_expect(__ballot(mn < thrd) == 0ull, 1)) continue;
-                // ---- exact tests of this group: the products are formed again, each lane
-                // marks its values below the threshold (bit 16 (b % 4) + v of cand[b / 4]),
-                // then tests those (lane element, stream element) pairs; both elements are
-                // then real (padding gives products >= 127)
+                if (__builtin_expect(__ballot(min(mh[0], mh[1]) < thrd) == 0ull, 1)) continue;
+                // ---- exact tests of this group: the products of the half (four blocks) whose
+                // minimum is below the threshold are formed again, each lane marks its values
+                // below it (bit 16 (b % 4) + v of cand[b / 4]), then tests those (lane
+                // element, stream element) pairs; both elements are then real (padding gives
+                // products >= 127)
                 ++medium_entries;
                 int mine = QSF_WEIGHT_NONE;
                 u64 cand[2] = {0ull, 0ull};
 #pragma unroll
                 for (int b = 0; b < kMfmaBlocks; ++b) {
+                    if (__ballot(mh[b / 4] < thrd) == 0ull) continue;
                     const v16i p = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[b], bv, zero, 0, 0, 0);
                     if (__ballot(red(p) < thrd) == 0ull) continue;
                     u64 bits = 0;
