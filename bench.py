@@ -146,10 +146,12 @@ def batched_learned(log, c4_cap=int(1e11)):
     out = {"scorer": str(wfile.relative_to(ROOT)) if scorer else None}
 
     def compare(name, Hs, cap, cands):
-        # untimed warm-up of each path on a few codes (first-call allocations, pools)
-        df.batch_distance([("classical", H) for H in Hs[:4]], work_cap=cap)
-        df.batch_distance(choose_orders(Hs[:4], scorer, candidates=cands), work_cap=cap)
-        df.batch_distance(choose_orders(Hs[:4], None, candidates=cands), work_cap=cap)
+        # untimed warm-up of each form at its full size, so that every timed run is in the
+        # steady state (pinned staging, device pools and solve arenas already grown)
+        for H in Hs:
+            df.batch_distance([("classical", H)], work_cap=cap)
+        df.batch_distance(choose_orders(Hs, scorer, candidates=cands), work_cap=cap)
+        df.batch_distance(choose_orders(Hs, None, candidates=cands), work_cap=cap)
         t0 = time.perf_counter()
         plain = [df.batch_distance([("classical", H)], work_cap=cap)[0] for H in Hs]
         t_plain = time.perf_counter() - t0

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <cmath>
 #include <cstdio>
+#include <exception>
 #include <memory>
 #include <thread>
 #include <unordered_map>
@@ -320,6 +321,32 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
     std::vector<std::vector<std::vector<int>>> cand(ns);
     std::vector<std::vector<SetRequest>> cand_req(ns);
     std::vector<SetRequest*> reqs;
+    {   // candidate orders of every side with a proposal, on host threads (sorts of n keys)
+        std::vector<size_t> with;
+        for (size_t i = 0; i < ns; ++i) {
+            const Proposal* pr = proposals.empty() ? nullptr : proposals[i];
+            if (sides[i]->has && pr && pr->candidates > 0 && (int)pr->scores.size() == sides[i]->n) with.push_back(i);
+        }
+        std::atomic<size_t> next{0};
+        std::vector<std::exception_ptr> fail(with.size());
+        auto work = [&] {
+            for (size_t k; (k = next++) < with.size();) {
+                try {
+                    const Proposal* pr = proposals[with[k]];
+                    cand[with[k]] = proposal_orders(pr->scores, pr->candidates, pr->seed);
+                } catch (...) {
+                    fail[k] = std::current_exception();
+                }
+            }
+        };
+        const size_t workers = with.size() >= 16 ? std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())) : 1;
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < workers; ++w) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+        for (auto& e : fail)
+            if (e) std::rethrow_exception(e);
+    }
     for (size_t i = 0; i < ns; ++i) {
         if (!sides[i]->has) continue;
         Side& S = *sides[i];
@@ -332,7 +359,6 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
         reqs.push_back(&greedy[i]);
         const Proposal* pr = proposals.empty() ? nullptr : proposals[i];
         if (!pr || pr->candidates <= 0 || (int)pr->scores.size() != sides[i]->n) continue;
-        cand[i] = proposal_orders(pr->scores, pr->candidates, pr->seed);
         cand_req[i].resize(cand[i].size());
         for (size_t t = 0; t < cand[i].size(); ++t) {
             cand_req[i][t].gen = &sides[i]->prob->gen;

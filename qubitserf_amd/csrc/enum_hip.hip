@@ -2141,6 +2141,30 @@ struct Engine {
             return v.mm == 0 ? 64ull * v.rpl : v.mm == 1 ? (u64)kMfmaLane : (u64)f4_lane_tile(v.mm);
         };
         const int ni = (int)items.size();
+        // stream chunk length per VALU variant: halved (down to 64) while the launch would
+        // have fewer than 4 tiles per resident wave, so that a small or medium level (one
+        // code per call) spreads over the chip instead of a few hundred waves; the [64..160]
+        // random codes one per call: 241 -> 203 ms of device time. Tuning override
+        // QSA_TILED_S: a fixed length.
+        std::vector<int> vS(NV);
+        for (int v = 0; v < NV; ++v) vS[v] = stream_len(vtab[v]);
+        {
+            const char* env = std::getenv("QSA_TILED_S");
+            std::vector<double> vcw(NV, 0.0);
+            for (const TItem& it : items)
+                vcw[vidx(it.V)] += (double)sat_mul(it.s->C(it.p.K, it.p.d), (u64)it.p.num_gamma);
+            const double resident = (double)std::max(1, cus) * 16;
+            for (int v = 0; v < NV; ++v) {
+                if (vtab[v].mm != 0) continue;
+                if (env) {
+                    vS[v] = std::max(64, std::min(kStreamMax, std::atoi(env)));
+                } else {
+                    int S = kStreamMax;
+                    while (S > 64 && vcw[v] / (64.0 * vtab[v].rpl * S) < 4 * resident) S /= 2;
+                    vS[v] = S;
+                }
+            }
+        }
         std::vector<std::vector<Rect>> grect(NV);
         std::vector<u32> hfixed;
         std::vector<size_t> fixed_off(ni, 0);
@@ -2153,7 +2177,7 @@ struct Engine {
             const TItem& it = items[i];
             const Solve& s = *it.s;
             const int K = it.p.K, d = it.p.d, h = s.h;
-            RectBuilder B{s, 0, s.RW, stream_len(it.V), lane_tile(it.V), it.V.mm > 0, {}, {}};
+            RectBuilder B{s, 0, s.RW, vS[vidx(it.V)], lane_tile(it.V), it.V.mm > 0, {}, {}};
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
                 for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
@@ -2269,7 +2293,7 @@ struct Engine {
             a.total_tiles = gtiles[v];
             a.binom = gbinom.p;
             a.bk = kMaxD + 1;
-            a.S = stream_len(V);
+            a.S = vS[v];
             a.G = gG[v];
             a.lds_full = glds[v];
             a.next_tile = cnt + 4 * v;
