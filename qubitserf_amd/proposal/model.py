@@ -160,7 +160,7 @@ def pack_batch_bits(Hs, seeds=None):
     pv = inv[np.maximum(dv_raw, 1).astype(np.int64) - 1]          # (B, n) pattern of 1 / deg_v
     # H is 0 / 1, so H x pattern is the pattern or 0: Ac = H / deg_c, Av = H^T / deg_v
     Ac = Hb * pc[:, :, None]
-    Av = np.ascontiguousarray(Hb.transpose(0, 2, 1)) * pv[:, :, None]
+    Av = np.multiply(Hb.transpose(0, 2, 1), pv[:, :, None], dtype=np.uint16)   # strided read, one pass
     X = np.zeros((B, n, F), np.float32)
     X[:, :, 0] = mask
     X[:, :, 1] = dv_raw / np.maximum(np.float32(1.0), dv_raw.max(1, keepdims=True))
