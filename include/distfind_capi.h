@@ -2,12 +2,12 @@
  * (python/qubitserf/distfind/_native.py:39-107), with the reference's signatures and
  * struct layouts (cpp/distfind/capi.h:12-111).
  *
- * What this engine answers: methods "bz" and (CSS and classical codes) "mitm" with
- * backend "gpu" or "auto", on the device. Return codes: 0 ok; 1 null pointer (capi.cpp:35,
- * :71); 2 odd column count for a symplectic matrix (capi.cpp:171); 3 "cc" on a non-CSS code
- * (capi.cpp:214); and, new here, 4 method not provided by this engine ("cc", symplectic
- * "mitm", "mitm" beyond 63 parity checks / 64 detectors / n = 128; the operator-weight and
- * subsystem entry points), 5 backend not provided ("cpu", or no gfx950 device), 6 device
+ * What this engine answers: methods "bz" and "mitm" (classical, CSS and non-CSS codes)
+ * with backend "gpu" or "auto", on the device. Return codes: 0 ok; 1 null pointer
+ * (capi.cpp:35, :71); 2 odd column count for a symplectic matrix (capi.cpp:171); 3 "cc" on
+ * a non-CSS code (capi.cpp:214); and, new here, 4 method not provided by this engine
+ * ("cc", "mitm" beyond 63 parity checks / 64 detectors / n = 128 (128 qubits for a
+ * non-CSS code); the operator-weight and subsystem entry points), 5 backend not provided ("cpu", or no gfx950 device), 6 device
  * error (distfind_last_error() has the text). There is no silent
  * CPU fallback: `DistfindResult.backend` is always "hip" on success.
  */

@@ -187,7 +187,7 @@ typedef struct {
 void qsf_hip_last_stats(qsf_hip_stats* out);
 
 /* Kernel choice for later calls (process-wide): 0 automatic (default), 1 always the
- * row-sweep kernel, 2 the tiled kernel wherever it applies (K <= 128, d <= 32). For tests
+ * row-sweep kernel, 2 the tiled kernels wherever they apply (K <= 256, d <= 32). For tests
  * and tuning; results never depend on it. Returns the previous policy. */
 int qsf_hip_set_kernel_policy(int policy);
 
