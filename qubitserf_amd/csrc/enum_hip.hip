@@ -2142,10 +2142,11 @@ struct Engine {
         };
         const int ni = (int)items.size();
         // stream chunk length per VALU variant: halved (down to 64) while the launch would
-        // have fewer than 4 tiles per resident wave, so that a small or medium level (one
-        // code per call) spreads over the chip instead of a few hundred waves; the [64..160]
-        // random codes one per call: 241 -> 203 ms of device time. Tuning override
-        // QSA_TILED_S: a fixed length.
+        // have fewer tiles than half the resident waves, so that a small or medium level
+        // (one code per call) spreads over the chip instead of a few hundred waves; the
+        // [64..160] random codes one per call: 241 -> 187 ms of device time (a target of 4
+        // tiles per wave gave 200 ms and slowed the gross d = 7 level by 15%). Tuning
+        // overrides: QSA_TILED_S (a fixed length), QSA_TILED_S_TILES (the target).
         std::vector<int> vS(NV);
         for (int v = 0; v < NV; ++v) vS[v] = stream_len(vtab[v]);
         {
@@ -2154,13 +2155,15 @@ struct Engine {
             for (const TItem& it : items)
                 vcw[vidx(it.V)] += (double)sat_mul(it.s->C(it.p.K, it.p.d), (u64)it.p.num_gamma);
             const double resident = (double)std::max(1, cus) * 16;
+            const char* per = std::getenv("QSA_TILED_S_TILES");   // tuning: tiles per resident wave
+            const double want = per ? std::atof(per) : 0.5;
             for (int v = 0; v < NV; ++v) {
                 if (vtab[v].mm != 0) continue;
                 if (env) {
                     vS[v] = std::max(64, std::min(kStreamMax, std::atoi(env)));
                 } else {
                     int S = kStreamMax;
-                    while (S > 64 && vcw[v] / (64.0 * vtab[v].rpl * S) < 4 * resident) S /= 2;
+                    while (S > 64 && vcw[v] / (64.0 * vtab[v].rpl * S) < want * resident) S /= 2;
                     vS[v] = S;
                 }
             }
