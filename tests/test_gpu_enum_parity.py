@@ -198,6 +198,20 @@ def test_small_level_sweep(hip):
             assert hip.last_stats()["kernel"] == 3, (K, d)
 
 
+@pytest.mark.parametrize("S", [64, 128, 256, 512])
+def test_tiled_stream_lengths(hip, S, monkeypatch):
+    """Every stream chunk length the launch sizing can pick (the automatic choice halves it
+    from 512 for launches with few tiles), forced, against the oracle: one- and two-word
+    masks, random and systematic generators, loose and tight bounds."""
+    monkeypatch.setenv("QSA_TILED_S", str(S))
+    rng = np.random.default_rng(2300 + S)
+    with hip.kernel_policy("tiled"):
+        for i, (K, stride, d) in enumerate([(60, 2, 4), (96, 3, 4), (140, 4, 3), (40, 1, 5)]):
+            best = [NONE, 25, 40, 12][i]
+            run_case(hip, rng, K, stride, d, 1 + i % 2, i % 3, best, systematic=i % 2 == 0)
+            assert hip.last_stats()["kernel"] == 2
+
+
 @pytest.mark.parametrize("fd", [1, 2, 3, 4])
 def test_tiled_wide_rows(hip, fd, monkeypatch):
     """K > 128: the VALU kernel with two-word list masks (each half of the rows up to 128
