@@ -466,6 +466,10 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         const int nch = (int)(min(ch0 + (u64)a.G, stream_chunks) - ch0);
         const u64 s0 = ch0 * (u64)kMfmaStream;
         u64 sidx = s0 + (u64)(2 * nch) * (u64)tid;
+        // threads whose run holds a real element; every chunk position from 2 t_real on is
+        // padding (products >= 127), so the tile's groups past it are not tested
+        const u64 t_real = min((u64)(64 * kMfmaWaves), (n_stream - s0 + (u64)(2 * nch) - 1) / (u64)(2 * nch));
+        const int gmax = uni((int)min((u64)(kMfmaStream / 32), (2 * t_real + 31) / 32));
         u64 smask = (sidx < n_stream && sk) ? colex_mask(sidx, sk, sm, a.binom, a.bk) : 0ull;
         u32 sx0 = fw0;
         for (u64 mm = smask; mm; mm &= mm - 1) sx0 ^= rows_fast[(sbase + __builtin_ctzll(mm)) * XD];
@@ -506,10 +510,9 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
             // products, each reduced by a chain of v_min3
             const v4i* sb = sbytes + half * kMfmaStream + row;
             v4i bnext = sb[0];
-            constexpr int kGroups = kMfmaStream / 32;
-            for (int gi = 0; gi < kGroups; ++gi) {
+            for (int gi = 0; gi < gmax; ++gi) {
                 const v4i bv = bnext;
-                if (gi + 1 < kGroups) bnext = sb[(gi + 1) * 32];
+                if (gi + 1 < gmax) bnext = sb[(gi + 1) * 32];
                 const v16i zero = {};
                 auto red = [](const v16i& c) {
                     int m = min(min(c[0], c[1]), c[2]);
@@ -839,6 +842,10 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         const int nch = (int)(min(ch0 + (u64)a.G, stream_chunks) - ch0);
         const u64 s0 = ch0 * (u64)SL;
         u64 sidx = s0 + (u64)(EPT * nch) * (u64)tid;
+        // threads whose run holds a real element; every chunk position from EPT t_real on is
+        // padding, so the tile's groups past it are not tested
+        const u64 t_real = min((u64)(64 * kMfmaWaves), (n_stream - s0 + (u64)(EPT * nch) - 1) / (u64)(EPT * nch));
+        const int gmax = uni((int)min((u64)(SL / NR), (EPT * t_real + NR - 1) / NR));
         u64 smask = (sidx < n_stream && sk) ? colex_mask(sidx, sk, sm, a.binom, a.bk) : 0ull;
         u32 sx[NKB];
 #pragma unroll
@@ -893,13 +900,12 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
             float thrf = (float)min(bound, 127);
             // B operand and stream constant one group ahead
             const i32x4* sb = sbytes + kb * SL + row;
-            constexpr int kGroups = SL / NR;
             i32x4 bnext = sb[0];
             float onext = soff[row];
-            for (int gi = 0; gi < kGroups; ++gi) {
+            for (int gi = 0; gi < gmax; ++gi) {
                 const i32x4 bv = bnext;
                 const float thr = thrf - onext;   // this lane's column: L + sum < bound - off_stream
-                if (gi + 1 < kGroups) {
+                if (gi + 1 < gmax) {
                     bnext = sb[(gi + 1) * NR];
                     onext = soff[(gi + 1) * NR + row];
                 }
