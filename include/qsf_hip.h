@@ -15,6 +15,7 @@
 #ifndef QSF_HIP_H
 #define QSF_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -66,6 +67,22 @@ int qsf_hip_enumerate(const qsf_enum_plan* plan, int* best_out);
  * disables it. The result is the same number qsf_hip_enumerate would return whenever that
  * number is <= floor or no codeword <= floor exists. */
 int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out);
+
+/* qsf_hip_enumerate_floor without allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, result and return codes as
+ * qsf_hip_enumerate_floor, and the same kernels, but every byte of device memory it uses
+ * comes from one static device array of qsf_hip_noalloc_scratch_bytes() bytes (16 MiB) that
+ * the library holds on its GPU until the process ends: the level's bound and counters, the
+ * solve's tables (uploaded once per buffers_key, from pageable host memory) and the run's
+ * block. It calls no hipMalloc, hipFree or hipHostMalloc; host-side tables use the C++ heap.
+ * A level whose tables and run block do not fit returns 4 with the reason in
+ * qsf_hip_last_error(). The array keeps the last call's contents; every call uploads what
+ * must start at zero. Calls are serialised by the library's lock, so two calls that overlap
+ * from different host threads run one after the other on the array (two copies of the
+ * library in one process would each have their own). */
+int qsf_hip_enumerate_floor_noalloc(const qsf_enum_plan* plan, int floor, int* best_out);
+size_t qsf_hip_noalloc_scratch_bytes(void);
 
 /* Batched min mode (no reference counterpart: the reference runs one level of one problem
  * per call). Plan j is one level of its own problem (distinct buffers_key per problem);

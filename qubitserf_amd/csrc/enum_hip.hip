@@ -24,6 +24,16 @@
 #include "kernels.hpp"
 #include "masks.hpp"
 
+// Device memory the library owns for the life of the process (one copy per GPU it runs on).
+// Host code never uses these names as addresses: at set-up the engine reads each array's
+// device address with hipGetSymbolAddress and passes that pointer to kernels and copies.
+//  * tiled_binom: saturating C(n, k), n <= kMaxTiledK, k <= kMaxD, read by the tiled kernels.
+//  * scratch: all device memory of qsf_hip_enumerate_floor_noalloc (bound and counters, the
+//    solve's tables, one run's block), so that entry point allocates nothing.
+constexpr size_t kScratchBytes = 16u << 20;
+static __device__ __align__(256) unsigned long long tiled_binom[(qsa::kMaxTiledK + 1) * (qsa::kMaxD + 1)];
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace qsa {
 
 // =====================================================================================
@@ -1279,7 +1289,8 @@ struct Engine {
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
     static constexpr int kVariants = 22;  // kernel variants of a tiled run (vtab in run_tiled)
     hipEvent_t gev[2 * kVariants] = {};   // start/stop per kernel-variant launch of a tiled run
-    DevBuf<u64> gbinom;                  // (kMaxTiledK + 1) x (kMaxD + 1) saturating C(n, k) for the tiled kernel
+    u64* gbinom = nullptr;               // device address of tiled_binom (filled at probe)
+    unsigned char* scratch_dev = nullptr;   // device address of scratch (no-allocation calls)
     DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
     unsigned char* pin = nullptr;        // its pinned host staging
     size_t pin_cap = 0;
@@ -1312,6 +1323,10 @@ struct Engine {
         for (hipEvent_t& e : gev)
             if (hipEventCreate(&e) != hipSuccess) { probe_msg = "event creation failed"; return ok = false; }
         try {
+            // the library's static device arrays on this device (the engine runs on the
+            // device that is current here, and keeps using it)
+            HIPCHK(hipGetSymbolAddress((void**)&gbinom, HIP_SYMBOL(tiled_binom)));
+            HIPCHK(hipGetSymbolAddress((void**)&scratch_dev, HIP_SYMBOL(scratch)));
             const int bk = kMaxD + 1;
             std::vector<u64> B((size_t)(kMaxTiledK + 1) * bk, 0);
             for (int i = 0; i <= kMaxTiledK; ++i) {
@@ -1319,8 +1334,8 @@ struct Engine {
                 for (int j = 1; j < bk && j <= i; ++j)
                     B[(size_t)i * bk + j] = sat_add(B[(size_t)(i - 1) * bk + j - 1], j <= i - 1 ? B[(size_t)(i - 1) * bk + j] : 0);
             }
-            gbinom.reserve(B.size());
-            HIPCHK(hipMemcpy(gbinom.p, B.data(), B.size() * 8, hipMemcpyHostToDevice));
+            static_assert(sizeof(tiled_binom) == (size_t)(kMaxTiledK + 1) * (kMaxD + 1) * sizeof(u64), "binomial table size");
+            HIPCHK(hipMemcpy(gbinom, B.data(), B.size() * 8, hipMemcpyHostToDevice));
         } catch (const HipError& e) {
             probe_msg = e.what;
             return ok = false;
@@ -1328,7 +1343,78 @@ struct Engine {
         return ok = true;
     }
 
+    // ---- no-allocation calls ------------------------------------------------------------
+    // While `noalloc` is set (inside qsf_hip_enumerate_floor_noalloc) every byte of device
+    // memory comes from `scratch`, laid out as
+    //   [0, kScratchHead): the row sweep's bound and counters
+    //   [kScratchHead, run_at): the tables of the one solve resident there (na_solve)
+    //   [run_at, kScratchBytes): one kernel run's block (bounds, jobs, rectangles, ...)
+    // and host staging is pageable memory (na_host). Nothing is allocated on the device or
+    // pinned; a call that needs more than the array holds is refused.
+    static constexpr size_t kScratchHead = 256;
+    bool noalloc = false;
+    std::unique_ptr<Solve> na_solve;      // host tables of the solve whose arena is in scratch
+    size_t na_run_at = kScratchHead;      // first byte of scratch after that arena
+    std::vector<unsigned char> na_host;   // pageable staging of a no-allocation run block
+
+    Solve& solve_noalloc(const qsf_enum_plan& p) {
+        if (na_solve) {
+            const Solve& s = *na_solve;
+            if (p.buffers_key != 0 && s.key == p.buffers_key && s.n == p.n && s.K == p.K &&
+                s.ngt == p.num_gamma_total && s.kcheck == p.kcheck && s.stride == p.stride)
+                return *na_solve;
+        }
+        na_solve.reset();
+        na_run_at = kScratchHead;
+        auto s = std::make_unique<Solve>();
+        build_host(*s, p);
+        const size_t bytes = s->stage.size() * 8;
+        if (kScratchHead + bytes > kScratchBytes)
+            throw HipError{4, "the solve's tables (" + std::to_string(bytes) + " bytes) exceed the " +
+                                  std::to_string(kScratchBytes >> 20) + " MiB no-allocation scratch"};
+        u64* d = reinterpret_cast<u64*>(scratch_dev + kScratchHead);
+        s->gamma.p = d;
+        s->check.p = d + s->n_gamma;
+        s->dbinom.p = d + s->n_gamma + s->n_check;
+        s->flags.p = d + s->n_gamma + s->n_check + s->n_binom;
+        s->tbl.p = reinterpret_cast<u32*>(d + s->n_gamma + s->n_check + s->n_binom + s->n_flags);
+        // pageable source: the staging vector stays alive (in na_solve) until the stream drains
+        HIPCHK(hipMemcpyAsync(d, s->stage.data(), bytes, hipMemcpyHostToDevice, stream));
+        na_run_at = (kScratchHead + bytes + 255) & ~(size_t)255;
+        na_solve = std::move(s);
+        return *na_solve;
+    }
+
+    // Host staging and device block of one kernel run: pinned staging and a device buffer
+    // grown on demand, or (no-allocation calls) pageable staging and the tail of scratch.
+    void stage_block(size_t total, unsigned char*& host, unsigned char*& dev) {
+        if (!noalloc) {
+            pin_reserve(total);
+            blk.reserve(total);
+            host = pin;
+            dev = blk.p;
+            return;
+        }
+        if (na_run_at + total > kScratchBytes)
+            throw HipError{4, "the level's run block (" + std::to_string(total) + " bytes) does not fit the " +
+                                  std::to_string(kScratchBytes >> 20) + " MiB no-allocation scratch beside the solve's tables"};
+        if (na_host.size() < total) na_host.resize(total);
+        host = na_host.data();
+        dev = scratch_dev + na_run_at;
+    }
+    int* dev_best() {
+        if (noalloc) return reinterpret_cast<int*>(scratch_dev);
+        best.reserve(1);
+        return best.p;
+    }
+    unsigned long long* dev_counter() {
+        if (noalloc) return reinterpret_cast<unsigned long long*>(scratch_dev + 64);
+        counter.reserve(4);
+        return counter.p;
+    }
+
     Solve& solve_for(const qsf_enum_plan& p) {
+        if (noalloc) return solve_noalloc(p);
         for (auto it = cache.begin(); it != cache.end(); ++it) {
             Solve& s = **it;
             if (p.buffers_key != 0 && s.key == p.buffers_key && s.n == p.n && s.K == p.K &&
@@ -1762,18 +1848,37 @@ struct Engine {
             *best_out = std::min(res[0], p.current_best);
             return 0;
         }
-        best.reserve(1);
-        counter.reserve(4);
-        HIPCHK(hipMemcpyAsync(best.p, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
-        launch_sweep(s, p, false, 0, 0);
+        int* bp = dev_best();
+        HIPCHK(hipMemcpyAsync(bp, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
+        launch_sweep(s, p, false, 0, 0, bp, dev_counter());
         int result = 0;
-        HIPCHK(hipMemcpyAsync(&result, best.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(&result, bp, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         stats.kernel_ms = ms;
         *best_out = std::min(result, p.current_best);
         return 0;
+    }
+
+    // qsf_hip_enumerate_floor on the scratch array only (see solve_noalloc).
+    int enumerate_noalloc(const qsf_enum_plan& p, int floor, int* best_out) {
+        if (!scratch_dev) throw HipError{3, "no-allocation scratch address unknown"};
+        struct Reset {
+            bool& f;
+            ~Reset() { f = false; }
+        } reset{noalloc};
+        noalloc = true;
+        try {
+            return enumerate(p, floor, best_out);
+        } catch (...) {
+            // the failed call may have left the stream with work on scratch: drain it, and
+            // forget the resident solve so the next call uploads its tables again
+            (void)hipStreamSynchronize(stream);
+            na_solve.reset();
+            na_run_at = kScratchHead;
+            throw;
+        }
     }
 
     // Many problems' levels at once: every tileable level's tiles share one persistent
@@ -1839,14 +1944,13 @@ struct Engine {
             for (int j = 0; j < n; ++j)
                 if (small_of[j] >= 0) best_out[j] = std::min(res[small_of[j]], plans[j].current_best);
         }
-        best.reserve(1);
-        counter.reserve(4);
         for (int j : sweep) {
             const qsf_enum_plan& p = plans[j];
-            HIPCHK(hipMemcpyAsync(best.p, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
-            launch_sweep(*solves[j], p, false, 0, 0);
+            int* bp = dev_best();
+            HIPCHK(hipMemcpyAsync(bp, &p.current_best, sizeof(int), hipMemcpyHostToDevice, stream));
+            launch_sweep(*solves[j], p, false, 0, 0, bp, dev_counter());
             int result = 0;
-            HIPCHK(hipMemcpyAsync(&result, best.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(&result, bp, sizeof(int), hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -1876,11 +1980,10 @@ struct Engine {
         const int ni = (int)items.size();
         const size_t o_jobs = align16((size_t)ni * sizeof(int));
         const size_t total = align16(o_jobs + (size_t)ni * sizeof(SJob));
-        pin_reserve(total);
-        blk.reserve(total);
-        unsigned char* dev = blk.p;
-        int* hb = reinterpret_cast<int*>(pin);
-        SJob* hj = reinterpret_cast<SJob*>(pin + o_jobs);
+        unsigned char *host = nullptr, *dev = nullptr;
+        stage_block(total, host, dev);
+        int* hb = reinterpret_cast<int*>(host);
+        SJob* hj = reinterpret_cast<SJob*>(host + o_jobs);
         u64 blocks = 0;
         for (int i = 0; i < ni; ++i) {
             const Solve& s = *items[i].s;
@@ -1901,7 +2004,7 @@ struct Engine {
             blocks += (u64)J.blocks_per_g * (u64)p.num_gamma;
             hj[i] = J;
         }
-        HIPCHK(hipMemcpyAsync(dev, pin, total, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, stream));
         SweepJobsArgs a{reinterpret_cast<const SJob*>(dev + o_jobs), ni};
         stats.kernel = 3;
         HIPCHK(hipEventRecord(ev0, stream));
@@ -1910,7 +2013,7 @@ struct Engine {
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(ev1, stream));
-        HIPCHK(hipMemcpyAsync(pin, dev, (size_t)ni * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(host, dev, (size_t)ni * sizeof(int), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         res.assign(hb, hb + ni);
         float ms = 0;
@@ -2254,12 +2357,11 @@ struct Engine {
         const size_t o_rects = align16(o_jobs + (size_t)ni * sizeof(TJob));
         const size_t o_fixed = align16(o_rects + nrect * sizeof(Rect));
         const size_t total = align16(o_fixed + hfixed.size() * sizeof(u32) + 4);
-        pin_reserve(total);
-        blk.reserve(total);
-        std::memset(pin, 0, o_jobs);
-        unsigned char* dev = blk.p;
-        int* hb = reinterpret_cast<int*>(pin + o_best);
-        TJob* hj = reinterpret_cast<TJob*>(pin + o_jobs);
+        unsigned char *host = nullptr, *dev = nullptr;
+        stage_block(total, host, dev);
+        std::memset(host, 0, o_jobs);
+        int* hb = reinterpret_cast<int*>(host + o_best);
+        TJob* hj = reinterpret_cast<TJob*>(host + o_jobs);
         for (int i = 0; i < ni; ++i) {
             const TItem& it = items[i];
             const Solve& s = *it.s;
@@ -2279,12 +2381,12 @@ struct Engine {
             size_t at = o_rects;
             for (int v = 0; v < NV; ++v) {
                 grect_off[v] = at;
-                if (!grect[v].empty()) std::memcpy(pin + at, grect[v].data(), grect[v].size() * sizeof(Rect));
+                if (!grect[v].empty()) std::memcpy(host + at, grect[v].data(), grect[v].size() * sizeof(Rect));
                 at += grect[v].size() * sizeof(Rect);
             }
         }
-        if (!hfixed.empty()) std::memcpy(pin + o_fixed, hfixed.data(), hfixed.size() * sizeof(u32));
-        HIPCHK(hipMemcpyAsync(dev, pin, total, hipMemcpyHostToDevice, stream));
+        if (!hfixed.empty()) std::memcpy(host + o_fixed, hfixed.data(), hfixed.size() * sizeof(u32));
+        HIPCHK(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, stream));
 
         auto* cnt = reinterpret_cast<unsigned long long*>(dev + o_cnt);
         stats.kernel = 2;
@@ -2300,7 +2402,7 @@ struct Engine {
             a.rects = reinterpret_cast<const Rect*>(dev + grect_off[v]);
             a.nrects = (int)grect[v].size();
             a.total_tiles = gtiles[v];
-            a.binom = gbinom.p;
+            a.binom = gbinom;
             a.bk = kMaxD + 1;
             a.S = vS[v];
             a.G = gG[v];
@@ -2352,10 +2454,10 @@ struct Engine {
             // codeword, or 64 / 128 fp4 ones)
             stats.ops_per_codeword = V.mm > 0 ? 0.5 : (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
         }
-        HIPCHK(hipMemcpyAsync(pin, dev, o_jobs, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(host, dev, o_jobs, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         res.assign(hb, hb + ni);
-        const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(pin + o_cnt);
+        const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(host + o_cnt);
         double ms_total = 0;
         stats.slow_hits = stats.medium_hits = 0;
         out_rows_last = 0;
@@ -2377,7 +2479,8 @@ struct Engine {
     }
     unsigned long long out_rows_last = 0;   // collect mode: rows the last tiled run produced
 
-    void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap) {
+    void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap, int* bp,
+                      unsigned long long* cntp) {
         if (p.stride > 16) throw HipError{4, "row-sweep kernel supports n <= 1024"};
         if (p.d > kMaxD) throw HipError{4, "weight level above 64"};
         SweepArgs a{};
@@ -2387,8 +2490,8 @@ struct Engine {
         const u64 target_threads = (u64)std::max(1, cus) * 2048;
         u64 chunk = std::max<u64>(64, (a.total * (u64)p.num_gamma + target_threads - 1) / target_threads);
         a.chunk = chunk;
-        a.best = best.p; a.collect = collect ? 1 : 0; a.keep = keep;
-        a.out_rows = counter.p + 3; a.out = out.p; a.out_cap = cap;
+        a.best = bp; a.collect = collect ? 1 : 0; a.keep = keep;
+        a.out_rows = cntp + 3; a.out = out.p; a.out_cap = cap;
         const u64 threads = ((a.total + chunk - 1) / chunk) * (u64)p.num_gamma;
         const u64 blocks = (threads + 255) / 256;
         stats.kernel = 1;
@@ -2410,8 +2513,8 @@ struct Engine {
         *rows = 0;
         if (per_g == 0 || p.num_gamma == 0) return 0;
         if (stats.codewords > (1ull << 62)) throw HipError{4, "level has more than 2^62 codewords"};
-        best.reserve(1);
-        counter.reserve(4);
+        int* bp = dev_best();
+        unsigned long long* cntp = dev_counter();
         const bool tiled = tileable(s, p) && (policy == 2 || (policy == 0 && stats.codewords >= (1ull << 16)));
         u64 cap = 1 << 16;
         for (;;) {
@@ -2425,9 +2528,9 @@ struct Engine {
                 run_tiled(items, true, keep, cap, res);
                 got = out_rows_last;
             } else {
-                HIPCHK(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned long long), stream));
-                launch_sweep(s, p, true, keep, cap);
-                HIPCHK(hipMemcpyAsync(&got, counter.p + 3, sizeof(got), hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipMemsetAsync(cntp, 0, 4 * sizeof(unsigned long long), stream));
+                launch_sweep(s, p, true, keep, cap, bp, cntp);
+                HIPCHK(hipMemcpyAsync(&got, cntp + 3, sizeof(got), hipMemcpyDeviceToHost, stream));
                 HIPCHK(hipStreamSynchronize(stream));
                 float ms = 0;
                 HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -2508,6 +2611,18 @@ int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out)
     }
     return run_guarded("qsf_hip_enumerate", [&](qsa::Engine& e) { return e.enumerate(*plan, floor, best_out); });
 }
+
+int qsf_hip_enumerate_floor_noalloc(const qsf_enum_plan* plan, int floor, int* best_out) {
+    std::string why;
+    if (!best_out || !qsa::plan_ok(plan, why)) {
+        engine().err = "qsf_hip_enumerate_floor_noalloc: " + (best_out ? why : std::string("null best_out"));
+        return 1;
+    }
+    return run_guarded("qsf_hip_enumerate_floor_noalloc",
+                       [&](qsa::Engine& e) { return e.enumerate_noalloc(*plan, floor, best_out); });
+}
+
+size_t qsf_hip_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
 int qsf_hip_enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out) {
     if (n < 0 || (n > 0 && (!plans || !best_out))) { engine().err = "qsf_hip_enumerate_batch: bad arguments"; return 1; }
