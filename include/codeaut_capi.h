@@ -28,6 +28,23 @@ void codeaut_bz_free(void* h);
 int32_t codeaut_backend_available(const char* name);
 const char* codeaut_version(void);
 
+/* Leon's code-automorphism ABI (cpp/codeaut/capi.h:20-48), which the reference's codeaut
+ * binding attaches without a guard (python/qubitserf/codeaut/_native.py:31-61). Leon is not
+ * part of this engine: qaut_leon_run returns a handle whose qaut_leon_ok is 0 (dim -1, every
+ * count 0, the copy functions write nothing); free it with qaut_leon_free. */
+void* qaut_leon_run(const uint8_t* G, int32_t m, int32_t n, int32_t max_dim, int32_t use_invariant);
+int32_t qaut_leon_ok(void* h);
+int32_t qaut_leon_dim(void* h);
+int32_t qaut_leon_n(void* h);
+int32_t qaut_leon_num_codewords(void* h);
+int32_t qaut_leon_num_classes(void* h);
+int32_t qaut_leon_num_gens(void* h);
+int32_t qaut_leon_num_factors(void* h);
+void qaut_leon_copy_gens(void* h, int32_t* buf);
+void qaut_leon_copy_factors(void* h, int64_t* buf);
+void qaut_leon_copy_weights(void* h, int32_t* buf);
+void qaut_leon_free(void* h);
+
 #ifdef __cplusplus
 }
 #endif

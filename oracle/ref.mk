@@ -28,7 +28,7 @@ SRCS = \
 OBJS = $(patsubst $(REF)/cpp/%.cpp,$(OUT)/obj/%.o,$(SRCS))
 SHIM = $(OUT)/obj/ref_shim.o
 
-all: $(OUT)/libqubitserf_ref.so $(OUT)/cpu_enum_ref
+all: $(OUT)/libqubitserf_ref.so $(OUT)/cpu_enum_ref $(OUT)/libqubitserf_ref_hip.so
 
 $(OUT)/obj/%.o: $(REF)/cpp/%.cpp
 	@mkdir -p $(dir $@)
@@ -45,6 +45,31 @@ $(OUT)/libqubitserf_ref.so: $(OBJS) $(SHIM)
 # The reference's own randomized-plan self-check of its CPU enumerator.
 $(OUT)/cpu_enum_ref: $(REF)/cpp/distfind/tests/cpu_enum_ref.cpp $(OUT)/libqubitserf_ref.so
 	$(CXX) $(CXXFLAGS) -I$(REF)/cpp $< $(OBJS) -o $@
+
+# The reference with the gfx950 plugin in its accelerator slot: the same sources, with
+# backend_cpu.cpp built -DQSF_CUDA (its cuda_backend() stub left out), the plugin
+# qubitserf_amd/plugin/backend_hip.cpp, and ref_hip_slot.cpp (cuda_backend() = hip_backend()),
+# linked against the engine library. -Bsymbolic keeps the reference's own distfind_* /
+# codeaut_* calls inside this library, apart from the engine's same-named exports.
+HIPLIB ?= qubitserf_amd/_lib
+HIP_OBJS = $(filter-out $(OUT)/obj/qsf/backend_cpu.o,$(OBJS)) $(OUT)/obj/hip/backend_cpu.o \
+           $(OUT)/obj/hip/backend_hip.o $(OUT)/obj/hip/ref_hip_slot.o
+
+$(OUT)/obj/hip/backend_cpu.o: $(REF)/cpp/qsf/backend_cpu.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -DQSF_CUDA -I$(REF)/cpp -c $< -o $@
+
+$(OUT)/obj/hip/backend_hip.o: qubitserf_amd/plugin/backend_hip.cpp include/qsf_hip.h
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -Wall -I$(REF)/cpp -Iinclude -c $< -o $@
+
+$(OUT)/obj/hip/ref_hip_slot.o: oracle/ref_hip_slot.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -I$(REF)/cpp -c $< -o $@
+
+$(OUT)/libqubitserf_ref_hip.so: $(HIP_OBJS) $(HIPLIB)/libqubitserf_amd.so
+	$(CXX) $(CXXFLAGS) -shared -Wl,-Bsymbolic $(HIP_OBJS) -o $@ -L$(HIPLIB) -lqubitserf_amd \
+	    -Wl,-rpath,'$$ORIGIN/../../qubitserf_amd/_lib'
 
 clean:
 	rm -rf $(OUT)

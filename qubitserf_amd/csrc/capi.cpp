@@ -27,6 +27,7 @@ namespace {
 
 thread_local std::string g_err;
 thread_local std::vector<LevelTrace> g_trace;
+thread_local BatchStats g_batch;   // last distfind_batch_distance call on this thread
 
 constexpr int RC_OK = 0, RC_NULL = 1, RC_ODD = 2, RC_CC_NONCSS = 3, RC_METHOD = 4, RC_BACKEND = 5, RC_DEVICE = 6;
 
@@ -76,6 +77,10 @@ int guarded(F&& f) {
 
 bool is_mitm(const char* method) { return method && std::string(method) == "mitm"; }
 
+// A matrix argument must have rows >= 0 and cols >= 1: negative sizes would otherwise reach
+// the bit-matrix allocation as huge unsigned sizes.
+bool bad_shape(int rows, int cols) { return rows < 0 || cols <= 0; }
+
 // css side / min dispatch, by method (cpp/distfind/capi.cpp:54-60)
 int css_solve(const BitMat& Hx, const BitMat& Hz, char which, int verbose, DistfindResult* out, bool mitm = false) {
     BzOptions o = options(verbose);
@@ -100,6 +105,7 @@ int distfind_css_distance(const uint8_t* Hx, int hx_rows, int hx_cols, const uin
                           int verbose, DistfindResult* out) {
     (void)threads;
     if (!out || !Hx || !Hz) { g_err = "null pointer"; return RC_NULL; }
+    if (bad_shape(hx_rows, hx_cols) || bad_shape(hz_rows, hz_cols)) { g_err = "negative or empty matrix shape"; return RC_NULL; }
     if (int rc = check_args(method, backend)) return rc;
     return guarded([&] {
         return css_solve(BitMat::from_bytes(Hx, hx_rows, hx_cols), BitMat::from_bytes(Hz, hz_rows, hz_cols), which,
@@ -111,6 +117,7 @@ int distfind_classical_distance(const uint8_t* H, int rows, int cols, const char
                                 int threads, int verbose, DistfindResult* out) {
     (void)threads;
     if (!out || !H) { g_err = "null pointer"; return RC_NULL; }
+    if (bad_shape(rows, cols)) { g_err = "negative or empty matrix shape"; return RC_NULL; }
     if (int rc = check_args(method, backend)) return rc;
     return guarded([&] {
         BzOptions o = options(verbose);
@@ -124,9 +131,16 @@ int distfind_stabilizer_distance(const uint8_t* S, int rows, int cols, const cha
                                  const char* backend, int threads, int verbose, DistfindResult* out) {
     (void)threads;
     if (!out || !S) { g_err = "null pointer"; return RC_NULL; }
+    if (bad_shape(rows, cols)) { g_err = "negative or empty matrix shape"; return RC_NULL; }
     if (cols % 2 != 0) { g_err = "symplectic matrix needs an even column count"; return RC_ODD; }
-    BitMat m = BitMat::from_bytes(S, rows, cols);
-    const bool css = is_css_symplectic(m);
+    BitMat m;
+    bool css = false;
+    if (int rc = guarded([&] {
+            m = BitMat::from_bytes(S, rows, cols);
+            css = is_css_symplectic(m);
+            return RC_OK;
+        }))
+        return rc;
     if (int rc = check_args(method, backend, !css)) return rc;
     return guarded([&] {
         if (css) {
@@ -215,12 +229,6 @@ int distfind_matroid_parts(const uint8_t* G, int rows, int cols, int* part_of_co
     return RC_OK;
 }
 
-// ---------------------------------------------------------------- batched solves
-
-namespace {
-thread_local BatchStats g_batch;
-}
-
 }  // extern "C"
 
 namespace {
@@ -231,6 +239,10 @@ namespace {
 int build_problem(const DistfindBatchItem* items, int i, BatchProblem& bp, std::string& err) {
     const DistfindBatchItem& it = items[i];
     if (!it.A || (it.kind == 1 && !it.B)) { err = "null matrix in batch item " + std::to_string(i); return RC_NULL; }
+    if (bad_shape(it.ar, it.ac) || (it.kind == 1 && bad_shape(it.br, it.bc))) {
+        err = "negative or empty matrix shape in batch item " + std::to_string(i);
+        return RC_NULL;
+    }
     if (it.kind == 0) {
         bp.a = classical(BitMat::from_bytes(it.A, it.ar, it.ac));
     } else if (it.kind == 1) {
@@ -311,10 +323,10 @@ extern "C" {
 int distfind_batch_distance(const DistfindBatchItem* items, int count, int64_t work_cap, int is_trials, int verbose,
                             DistfindResult* out) {
     if (count < 0 || (count > 0 && (!items || !out))) { g_err = "null batch"; return RC_NULL; }
-    if (int rc = check_args("bz", "gpu")) return rc;
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<BatchProblem> probs;
-    if (int rc = build_problems(items, count, probs)) return rc;
+    std::vector<BatchProblem> probs;   // malformed items are reported before the device is probed
+    if (int rc = guarded([&] { return build_problems(items, count, probs); })) return rc;
+    if (int rc = check_args("bz", "gpu")) return rc;
     BzOptions o = options(verbose);
     o.trace = nullptr;
     if (is_trials > 0) o.is_trials = is_trials;
@@ -342,7 +354,7 @@ int distfind_session_add(void* handle, const DistfindBatchItem* items, int count
     auto* h = static_cast<SessionHandle*>(handle);
     if (!h || count < 0 || (count > 0 && (!items || !ids))) { g_err = "null session or items"; return RC_NULL; }
     std::vector<BatchProblem> probs;
-    if (int rc = build_problems(items, count, probs)) return rc;
+    if (int rc = guarded([&] { return build_problems(items, count, probs); })) return rc;
     return guarded([&] {
         std::vector<long long> got = h->s->add(std::move(probs));
         for (int i = 0; i < count; ++i) ids[i] = got[i];
@@ -501,6 +513,36 @@ int32_t codeaut_backend_available(const char* name) {
 }
 
 const char* codeaut_version(void) { return "0.1.0+gfx950"; }
+
+// ---------------------------------------------------------------- Leon (not provided)
+// Leon's automorphism search (the reference's cpp/codeaut/leon.cpp) is not part of this
+// engine. The symbols exist because the reference's codeaut binding attaches them without
+// a guard (python/qubitserf/codeaut/_native.py:31-61): a handle reports ok = 0, so its
+// caller (python/qubitserf/codeaut/leon.py:145-151) raises instead of reading a group.
+
+namespace {
+struct LeonHandle {
+    int32_t n = 0;
+};
+}  // namespace
+
+void* qaut_leon_run(const uint8_t*, int32_t, int32_t n, int32_t, int32_t) {
+    g_err = "Leon's automorphism search is not provided by the gfx950 engine";
+    auto* h = new LeonHandle();
+    h->n = n;
+    return h;
+}
+int32_t qaut_leon_ok(void*) { return 0; }
+int32_t qaut_leon_dim(void*) { return -1; }
+int32_t qaut_leon_n(void* h) { return h ? static_cast<LeonHandle*>(h)->n : 0; }
+int32_t qaut_leon_num_codewords(void*) { return 0; }
+int32_t qaut_leon_num_classes(void*) { return 0; }
+int32_t qaut_leon_num_gens(void*) { return 0; }
+int32_t qaut_leon_num_factors(void*) { return 0; }
+void qaut_leon_copy_gens(void*, int32_t*) {}
+void qaut_leon_copy_factors(void*, int64_t*) {}
+void qaut_leon_copy_weights(void*, int32_t*) {}
+void qaut_leon_free(void* h) { delete static_cast<LeonHandle*>(h); }
 
 }  // extern "C"
 

@@ -11,6 +11,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent.parent
 ORACLE_LIB = ROOT / "oracle" / "_build" / "liboracle.so"
 REF_LIB = ROOT / "oracle" / "_ref" / "libqubitserf_ref.so"
+# the reference's sources with the gfx950 plugin in the accelerator slot (oracle/ref.mk)
+REF_HIP_LIB = ROOT / "oracle" / "_ref" / "libqubitserf_ref_hip.so"
 WEIGHT_NONE = 1 << 30
 
 u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -29,6 +31,7 @@ class RefResult(ctypes.Structure):
 
 _o = None
 _r = None
+_rh = None
 
 
 def lib():
@@ -63,24 +66,53 @@ def ref_available() -> bool:
     return REF_LIB.exists()
 
 
+def _bind_ref(L):
+    i, c, s = ctypes.c_int, ctypes.c_char, ctypes.c_char_p
+    rp = ctypes.POINTER(RefResult)
+    L.distfind_css_distance.argtypes = [u8p, i, i, u8p, i, i, s, c, s, i, i, rp]
+    L.distfind_classical_distance.argtypes = [u8p, i, i, s, s, i, i, rp]
+    L.distfind_stabilizer_distance.argtypes = [u8p, i, i, s, c, s, i, i, rp]
+    L.distfind_backend_available.argtypes = [s]
+    L.codeaut_bz_collect.restype = ctypes.c_void_p
+    L.codeaut_bz_collect.argtypes = [u8p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_int32]
+    for f in ("codeaut_bz_count", "codeaut_bz_combos"):
+        getattr(L, f).restype = ctypes.c_int64
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+    for f in ("codeaut_bz_ok",):
+        getattr(L, f).restype = ctypes.c_int32
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+    L.codeaut_bz_backend.restype = s
+    L.codeaut_bz_backend.argtypes = [ctypes.c_void_p]
+    L.codeaut_bz_copy_rows.argtypes = [ctypes.c_void_p, u8p]
+    L.codeaut_bz_free.argtypes = [ctypes.c_void_p]
+    L.codeaut_backend_available.restype = ctypes.c_int32
+    L.codeaut_backend_available.argtypes = [s]
+    return L
+
+
+def ref_hip_available() -> bool:
+    return REF_HIP_LIB.exists()
+
+
+def ref_hip():
+    """The reference's own drivers and C ABI with the gfx950 plugin as its GPU backend."""
+    global _rh
+    if _rh is None:
+        L = _bind_ref(ctypes.CDLL(str(REF_HIP_LIB)))
+        i = ctypes.c_int
+        L.ref_hip_plugin_enumerate.argtypes = [i, i, i, i, i, u64p, i, u64p, i]
+        L.ref_hip_plugin_collect.restype = ctypes.c_int64
+        L.ref_hip_plugin_collect.argtypes = [i, i, i, i, u64p, i, u64p, ctypes.c_int64, ctypes.POINTER(i)]
+        L.ref_hip_plugin_name.restype = ctypes.c_char_p
+        _rh = L
+    return _rh
+
+
 def ref():
     global _r
     if _r is None:
-        L = ctypes.CDLL(str(REF_LIB))
-        i, c, s = ctypes.c_int, ctypes.c_char, ctypes.c_char_p
-        rp = ctypes.POINTER(RefResult)
-        L.distfind_css_distance.argtypes = [u8p, i, i, u8p, i, i, s, c, s, i, i, rp]
-        L.distfind_classical_distance.argtypes = [u8p, i, i, s, s, i, i, rp]
-        L.distfind_stabilizer_distance.argtypes = [u8p, i, i, s, c, s, i, i, rp]
-        L.codeaut_bz_collect.restype = ctypes.c_void_p
-        L.codeaut_bz_collect.argtypes = [u8p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                         ctypes.c_int32, ctypes.c_int32]
-        for f in ("codeaut_bz_count", "codeaut_bz_combos"):
-            getattr(L, f).restype = ctypes.c_int64
-            getattr(L, f).argtypes = [ctypes.c_void_p]
-        L.codeaut_bz_copy_rows.argtypes = [ctypes.c_void_p, u8p]
-        L.codeaut_bz_free.argtypes = [ctypes.c_void_p]
-        _r = L
+        _r = _bind_ref(ctypes.CDLL(str(REF_LIB)))
     return _r
 
 
@@ -244,40 +276,48 @@ def is_trial(gen, n, t, seed=0x9E3779B9):
 
 # ---------------------------------------------------------------- reference CPU library
 
-def ref_css_distance(Hx, Hz, which="M", threads=0) -> RefResult:
+# L: the library to call (default: the reference's CPU build); backend: its backend name
+# ("cpu", or "gpu" to run the plugin in the ref_hip() build).
+
+def ref_css_distance(Hx, Hz, which="M", threads=0, L=None, backend="cpu") -> RefResult:
     ax, az = _u8(Hx), _u8(Hz)
     r = RefResult()
-    assert ref().distfind_css_distance(_p(ax), ax.shape[0], ax.shape[1], _p(az), az.shape[0], az.shape[1], b"bz",
-                                       which.encode()[:1], b"cpu", threads, 0, ctypes.byref(r)) == 0
+    assert (L or ref()).distfind_css_distance(_p(ax), ax.shape[0], ax.shape[1], _p(az), az.shape[0], az.shape[1],
+                                              b"bz", which.encode()[:1], backend.encode(), threads, 0,
+                                              ctypes.byref(r)) == 0
     return r
 
 
-def ref_classical_distance(H, threads=0) -> RefResult:
+def ref_classical_distance(H, threads=0, L=None, backend="cpu") -> RefResult:
     a = _u8(H)
     r = RefResult()
-    assert ref().distfind_classical_distance(_p(a), a.shape[0], a.shape[1], b"bz", b"cpu", threads, 0,
-                                             ctypes.byref(r)) == 0
+    assert (L or ref()).distfind_classical_distance(_p(a), a.shape[0], a.shape[1], b"bz", backend.encode(), threads,
+                                                    0, ctypes.byref(r)) == 0
     return r
 
 
-def ref_stabilizer_distance(S, which="M", threads=0, method="bz") -> RefResult:
+def ref_stabilizer_distance(S, which="M", threads=0, method="bz", L=None, backend="cpu") -> RefResult:
     a = _u8(S)
     r = RefResult()
-    assert ref().distfind_stabilizer_distance(_p(a), a.shape[0], a.shape[1], method.encode(), which.encode()[:1],
-                                              b"cpu", threads, 0, ctypes.byref(r)) == 0
+    assert (L or ref()).distfind_stabilizer_distance(_p(a), a.shape[0], a.shape[1], method.encode(),
+                                                     which.encode()[:1], backend.encode(), threads, 0,
+                                                     ctypes.byref(r)) == 0
     return r
 
 
-def ref_collect(G, p, keep, threads=0):
+def ref_collect(G, p, keep, threads=0, L=None, backend=0, with_meta=False):
+    """(rows, combos) of the reference's codeaut_bz_collect; with_meta adds (ok, backend name)."""
     a = _u8(G)
-    L = ref()
-    h = L.codeaut_bz_collect(_p(a), a.shape[0], a.shape[1], p, keep, 0, threads)
+    L = L or ref()
+    h = L.codeaut_bz_collect(_p(a), a.shape[0], a.shape[1], p, keep, backend, threads)
     try:
+        if with_meta:
+            meta = (L.codeaut_bz_ok(h), L.codeaut_bz_backend(h).decode())
         cnt = L.codeaut_bz_count(h)
         out = np.zeros((cnt, a.shape[1]), np.uint8)
         if cnt:
             L.codeaut_bz_copy_rows(h, _p(out))
-        return out, L.codeaut_bz_combos(h)
+        return (out, L.codeaut_bz_combos(h)) + (meta if with_meta else ())
     finally:
         L.codeaut_bz_free(h)
 
