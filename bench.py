@@ -61,6 +61,34 @@ def traffic_bytes(kernel: str):
         return None
 
 
+def host_cpus():
+    """The host's CPUs as this process sees them: os.cpu_count() (nproc), the affinity
+    mask, a cgroup CPU quota if one is set, and the CPU model. `usable` = every core the
+    process may run on = min(affinity, quota)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = min(affinity, quota) if quota else affinity
+    return {"nproc": nproc, "affinity": affinity, "cgroup_quota": quota, "usable": usable, "model": model}
+
+
 def cpu_baseline(threads: int):
     """The reference CPU enumerator on the gross Z-side level d=8 (C(78,8) = 2.3e10
     codewords, the full-rank information set, bound 12). Falls back to the oracle port
@@ -297,10 +325,13 @@ def main():
         line["batched_learned"] = batched_learned(lambda m: print(m, file=sys.stderr, flush=True), int(args.c4_cap))
         line["config1"] = config1_gpu()
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(min(threads, 16))
+        host = host_cpus()
+        threads = host["usable"]
+        line["cpu_baseline"] = cpu_baseline(threads)
+        line["cpu_baseline"]["host"] = host
         if "config1" in line:
-            line["config1"].update(config1_cpu(min(threads, 16)))
+            line["config1"].update(config1_cpu(threads))
+            line["config1"]["cpu_host"] = host
             line["config1"]["distances_agree"] = line["config1"]["distance"] == line["config1"]["cpu_distance"]
     print(json.dumps(line), flush=True)
 

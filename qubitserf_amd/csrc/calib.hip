@@ -16,10 +16,6 @@
 //                             prints products/s (the matrix cores' rate for this shape).
 //   qsa_calib mfma_valu       8 such products plus 64 v_min3 that do not read them:
 //                             shows how far VALU work overlaps the products on a SIMD.
-//   qsa_calib mfma16          as mfma with v_mfma_i32_16x16x64_i8.
-//   qsa_calib mfma16_and      the packed hot-test pattern: per step 8 16x16x64 products
-//                             (bias accumulator) AND-reduced, two codewords per entry;
-//                             prints codewords/s.
 //   qsa_calib f4              fp4 matrix-core rate: 8 independent chains of
 //                             v_mfma_scale_f32_32x32x64_f8f6f4 (fp4 operands, unit scales);
 //                             prints fp4 ops/s (2 per multiply-add).
@@ -134,42 +130,6 @@ __global__ __launch_bounds__(64) void mfma_issue(int* out, int iters) {
     out[blockIdx.x * 64 + l] = hit;
 }
 
-__global__ __launch_bounds__(64) void mfma16_chain(int* out, int iters) {
-    const int l = threadIdx.x;
-    v4i a[8], b = {l, l * 3, l * 5, l * 7};
-    v4i c[8];
-    for (int k = 0; k < 8; ++k) {
-        a[k] = v4i{l + k, l ^ k, l * k, l - k};
-        c[k] = v4i{};
-    }
-    for (int i = 0; i < iters; ++i)
-#pragma unroll
-        for (int k = 0; k < 8; ++k) c[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], b, c[k], 0, 0, 0);
-    int r = 0;
-    for (int k = 0; k < 8; ++k)
-        for (int v = 0; v < 4; ++v) r ^= c[k][v];
-    out[blockIdx.x * 64 + l] = r;
-}
-
-__global__ __launch_bounds__(64) void mfma16_and(int* out, int iters) {
-    const int l = threadIdx.x;
-    v4i a[8], b = {l, l * 3, l * 5, l * 7};
-    for (int k = 0; k < 8; ++k) a[k] = v4i{l + k, l ^ k, l * k, l - k};
-    const v4i bias = {l, l, l, l};
-    int hit = 0;
-    for (int i = 0; i < iters; ++i) {
-        b[0] += 1;
-        unsigned acc = ~0u;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const v4i p = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], b, bias, 0, 0, 0);
-            acc &= (unsigned)p[0] & (unsigned)p[1] & (unsigned)p[2] & (unsigned)p[3];
-        }
-        hit += __ballot((acc & 0x820u) != 0x820u) != 0ull;
-    }
-    out[blockIdx.x * 64 + l] = hit;
-}
-
 typedef int v8i_ __attribute__((ext_vector_type(8)));
 typedef float v16f_ __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ v16f_ f4mm(v8i_ a, v8i_ b, v16f_ c) {
@@ -274,7 +234,6 @@ int main(int argc, char** argv) {
     }
     const bool m32c = std::strcmp(argv[1], "mfma") == 0, m32r = std::strcmp(argv[1], "mfma_max") == 0;
     const bool m32i = std::strcmp(argv[1], "mfma_issue") == 0, m32v = std::strcmp(argv[1], "mfma_valu") == 0;
-    const bool m16c = std::strcmp(argv[1], "mfma16") == 0, m16r = std::strcmp(argv[1], "mfma16_and") == 0;
     const bool f4c = std::strcmp(argv[1], "f4") == 0, f4r = std::strcmp(argv[1], "f4_max") == 0;
     if (f4c || f4r) {
         const int per_cu = argc >= 3 ? std::atoi(argv[2]) : 12;
@@ -299,18 +258,16 @@ int main(int argc, char** argv) {
         CHECK(hipFree(out));
         return 0;
     }
-    if (m32c || m32r || m16c || m16r || m32i || m32v) {
+    if (m32c || m32r || m32i || m32v) {
         const int per_cu = argc >= 3 ? std::atoi(argv[2]) : 12;
-        const int blocks = 256 * per_cu, iters = (m32c || m16c) ? 4000 : 2000;
+        const int blocks = 256 * per_cu, iters = m32c ? 4000 : 2000;
         int* out;
         CHECK(hipMalloc(&out, (size_t)blocks * 64 * sizeof(int)));
         auto launch = [&](int it) {
             if (m32c) hipLaunchKernelGGL(mfma_chain, dim3(blocks), dim3(64), 0, 0, out, it);
             else if (m32r) hipLaunchKernelGGL(mfma_max, dim3(blocks), dim3(64), 0, 0, out, it);
             else if (m32i) hipLaunchKernelGGL(mfma_issue<0>, dim3(blocks), dim3(64), 0, 0, out, it);
-            else if (m32v) hipLaunchKernelGGL(mfma_issue<1>, dim3(blocks), dim3(64), 0, 0, out, it);
-            else if (m16c) hipLaunchKernelGGL(mfma16_chain, dim3(blocks), dim3(64), 0, 0, out, it);
-            else hipLaunchKernelGGL(mfma16_and, dim3(blocks), dim3(64), 0, 0, out, it);
+            else hipLaunchKernelGGL(mfma_issue<1>, dim3(blocks), dim3(64), 0, 0, out, it);
         };
         launch(10);
         CHECK(hipDeviceSynchronize());
@@ -321,12 +278,12 @@ int main(int argc, char** argv) {
         float ms = 0;
         CHECK(hipEventElapsedTime(&ms, e0, e1));
         const double mfmas = (double)blocks * iters * 8;
-        const double ops_each = (m32c || m32r) ? 65536.0 : 32768.0;
-        if (m32c || m16c)
+        const double ops_each = 65536.0;
+        if (m32c)
             std::printf("{\"probe\": \"%s\", \"waves_per_cu\": %d, \"int8_ops\": %.6e, \"ms\": %.3f, \"tops\": %.1f}\n", argv[1],
                         per_cu, mfmas * ops_each, ms, mfmas * ops_each / (ms * 1e-3) / 1e12);
         else {
-            const double cw = mfmas * (m32r || m32i || m32v ? 1024.0 : 512.0);
+            const double cw = mfmas * 1024.0;
             std::printf("{\"probe\": \"%s\", \"waves_per_cu\": %d, \"products\": %.6e, \"ms\": %.3f, \"products_per_s\": %.4e}\n",
                         argv[1], per_cu, cw, ms, cw / (ms * 1e-3));
         }
