@@ -203,6 +203,15 @@ typedef struct {
 } qsf_hip_stats;
 void qsf_hip_last_stats(qsf_hip_stats* out);
 
+/* The hot-test variant the automatic choice would run a level on (diagnostics and tests;
+ * host work only, no device needed): out[0..3] = fd, xd, rpl, mm as in qsf_hip_stats. The
+ * choice prices every applicable variant with a cost model whose constants were measured
+ * on an MI355X (qubitserf_amd/tune.py -> csrc/tuning.hpp) and whose hot-test pass rates
+ * are fair-coin rates times the solve's structure factor; factor > 0 with at_p > 0 sets
+ * that factor as if a level with fair per-codeword pass probability at_p had measured it
+ * (otherwise the prior, 1). Returns 0, or 1 for bad arguments. */
+int qsf_hip_plan_variant(const qsf_enum_plan* plan, double factor, double at_p, int* out);
+
 /* Kernel choice for later calls (process-wide): 0 automatic (default), 1 always the
  * row-sweep kernel, 2 the tiled kernels wherever they apply (K <= 256, d <= 32). For tests
  * and tuning; results never depend on it. Returns the previous policy. */

@@ -23,6 +23,7 @@
 #include "elim.hpp"
 #include "kernels.hpp"
 #include "masks.hpp"
+#include "tuning.hpp"
 
 // Device memory the library owns for the life of the process (one copy per GPU it runs on).
 // Host code never uses these names as addresses: at set-up the engine reads each array's
@@ -87,6 +88,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
     int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
     const TJob* J = nullptr;
     unsigned slow_entries = 0, medium_entries = 0;
+    unsigned job_hits = 0;   // this wave's medium entries on the current job (flushed to J->hits)
 
     for (;;) {
         unsigned long long t0 = 0;
@@ -105,6 +107,8 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         const Rect& Rr = a.rects[uni(lo)];
         const int job = uni(Rr.job);
         if (job != cur_job) {   // the problem's parameters, and its running bound
+            if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
+            job_hits = 0;
             J = a.jobs + job;
             K = uni(J->K); RW = uni(J->RW); RD = uni(J->RD); SD = uni(J->SD); floor = uni(J->floor);
             bound = a.collect ? a.keep + 1
@@ -237,6 +241,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                         if (tk < thr) cand |= 1u << k;
                     }
                     ++medium_entries;
+                    ++job_hits;
                     if (__ballot(cand != 0) == 0ull) return;
                     // ---- slow path: exact weight and logical test of the light candidates
                     ++slow_entries;
@@ -326,6 +331,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
     }
     if (lane == 0 && slow_entries) atomicAdd(a.slow_hits, (unsigned long long)slow_entries);
     if (lane == 0 && medium_entries) atomicAdd(a.slow_hits + 1, (unsigned long long)medium_entries);
+    if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
 }
 
 // =====================================================================================
@@ -384,6 +390,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
     int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
     const TJob* J = nullptr;
     unsigned slow_entries = 0, medium_entries = 0;
+    unsigned job_hits = 0;   // this wave's medium entries on the current job (flushed to J->hits)
 
     for (;;) {
         __syncthreads();   // every wave is done with the previous tile's LDS
@@ -399,6 +406,8 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         const Rect& Rr = a.rects[uni(lo)];
         const int job = uni(Rr.job);
         if (job != cur_job) {
+            if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
+            job_hits = 0;
             J = a.jobs + job;
             K = uni(J->K); RW = uni(J->RW); RD = uni(J->RD); SD = uni(J->SD); floor = uni(J->floor);
             bound = uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -559,6 +568,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
                 // element, stream element) pairs; both elements are then real (padding gives
                 // products >= 127)
                 ++medium_entries;
+                ++job_hits;
                 int mine = QSF_WEIGHT_NONE;
                 u64 cand[2] = {0ull, 0ull};
 #pragma unroll
@@ -631,6 +641,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
     }
     if (lane == 0 && slow_entries) atomicAdd(a.slow_hits, (unsigned long long)slow_entries);
     if (lane == 0 && medium_entries) atomicAdd(a.slow_hits + 1, (unsigned long long)medium_entries);
+    if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
 }
 
 // =====================================================================================
@@ -753,6 +764,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
     const TJob* J = nullptr;
     unsigned slow_entries = 0, medium_entries = 0;
+    unsigned job_hits = 0;   // this wave's medium entries on the current job (flushed to J->hits)
 
     for (;;) {
         __syncthreads();   // every wave is done with the previous tile's LDS
@@ -768,6 +780,8 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         const Rect& Rr = a.rects[uni(lo)];
         const int job = uni(Rr.job);
         if (job != cur_job) {
+            if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
+            job_hits = 0;
             J = a.jobs + job;
             K = uni(J->K); RW = uni(J->RW); RD = uni(J->RD); SD = uni(J->SD); floor = uni(J->floor);
             bound = uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -954,6 +968,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 // marks its values below its threshold (bit NV b + v), then tests those
                 // (lane element, stream element) pairs; both elements are then real
                 ++medium_entries;
+                ++job_hits;
                 int mine = QSF_WEIGHT_NONE;
                 u64 cand[2] = {0ull, 0ull};
                 // an opaque copy of the B operand: the products are formed again here instead
@@ -1033,6 +1048,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     }
     if (lane == 0 && slow_entries) atomicAdd(a.slow_hits, (unsigned long long)slow_entries);
     if (lane == 0 && medium_entries) atomicAdd(a.slow_hits + 1, (unsigned long long)medium_entries);
+    if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
 }
 
 // =====================================================================================
@@ -1261,6 +1277,17 @@ struct Solve {
     int n = 0, stride = 0, K = 0, ngt = 0, kcheck = 0;
     int h = 0, RD = 0, SD = 0, RW = 0, bk = 0, maxres = 0;
     std::vector<int> nflag;       // flagged rows per generator
+    // Measured structure factor of this solve's codewords: hot-test passes over the
+    // fair-coin expectation at the last level that had enough expected passes to tell
+    // (sf_logp: log of that level's fair per-codeword pass probability; sf = 0: not
+    // measured yet). Deeper in the tail the factor grows: at a fair probability p it is
+    // taken as sf (p / p_observed)^-kElasticity. The kernel choice scales pass rates by it.
+    double sf = 0.0;
+    double sf_logp = 0.0;
+    double factor_at(double p) const {
+        if (sf <= 0.0 || p <= 0.0) return tuning::kPriorFactor;
+        return std::min(4096.0, std::max(1.0, sf * std::exp(-tuning::kElasticity * (std::log(p) - sf_logp))));
+    }
     std::vector<u32> htbl;        // host copy of the residual+syndrome table
     std::vector<u64> hflags;      // host copy of the flag masks
     std::vector<u64> binom;       // (K+1) x bk
@@ -1838,6 +1865,7 @@ struct Engine {
             std::vector<TItem> items{TItem{&s, p, floor, choose_variant(s, p)}};
             std::vector<int> res;
             run_tiled(items, false, 0, 0, res);
+            if (res[0] > floor) observe(s, p, items[0].V, item_hits[0]);   // ran to its end
             *best_out = std::min(res[0], p.current_best);
             return 0;
         }
@@ -1934,6 +1962,8 @@ struct Engine {
             std::vector<int> res;
             run_tiled(items, false, 0, 0, res);
             ms_total += stats.kernel_ms;
+            for (size_t i = 0; i < items.size(); ++i)
+                if (res[i] > items[i].floor) observe(*items[i].s, items[i].p, items[i].V, item_hits[i]);
             for (int j = 0; j < n; ++j)
                 if (item_of[j] >= 0) best_out[j] = std::min(res[item_of[j]], plans[j].current_best);
         }
@@ -2042,14 +2072,14 @@ struct Engine {
         return cdf;
     }
     // The matrix tests run 4-wave workgroups over tiles of 1024 (512) lane elements x 512
-    // (256) stream elements: a launch of less than ~2^33 codewords leaves most of the chip
-    // idle or pads heavily, and the VALU kernel's finer tiles are faster. A single level
-    // must hold that much work itself; in a batched run the levels of one variant share a
-    // launch, so levels of >= 2^27 codewords qualify when their variant's launch reaches
-    // 2^33 in total (enumerate_batch).
+    // (256) stream elements: a launch of less than ~2^28 codewords cannot fill the chip,
+    // and the VALU kernel's finer tiles win there. Above it the cost model decides. A
+    // single level must hold that much work itself; in a batched run the levels of one
+    // variant share a launch, so levels of >= 2^22 codewords qualify when their variant's
+    // launch reaches 2^28 in total (enumerate_batch).
     static u64 matrix_min_work() {
         const char* mw = std::getenv("QSA_MATRIX_MIN_WORK");   // tuning/testing override
-        return mw ? (u64)std::strtoull(mw, nullptr, 10) : (1ull << 33);
+        return mw ? (u64)std::strtoull(mw, nullptr, 10) : (1ull << 28);
     }
     // Matrix-core tests (fd == 0): rpl = lane elements per wave / 64. min_work: smallest
     // level (codewords) a matrix test may take (default matrix_min_work()).
@@ -2059,6 +2089,59 @@ struct Engine {
         Variant v = choose_variant_k128(s, p, false, min_work);   // two-word masks: VALU kernel only
         v.w = 2;
         return v;
+    }
+    // The hot test of a variant: its unit (the matrix tests count 8192-codeword groups
+    // holding a value below the threshold, the VALU test stream steps of 64 x RPL
+    // codewords) and its test bits (30 for int8, 60 for fp4 32x32x64, 32 FD for VALU).
+    static double test_unit(const Variant& V) { return V.mm > 0 ? 8192.0 : 64.0 * V.rpl; }
+    static int test_bits(const Variant& V, int res) {
+        return std::min(V.mm == 1 ? 30 : V.mm == 2 ? 60 : V.mm == 3 ? 124 : 32 * V.fd, res);
+    }
+    // Hot-test passes on a level of cw codewords with threshold thr if the residual bits
+    // were fair coins (per-codeword pass probability p = P(Binomial(bits, 1/2) < thr)).
+    static double fair_hits(const Variant& V, u64 cw, double p) {
+        const double unit = test_unit(V);
+        return (double)cw / unit * std::min(1.0, unit * p);
+    }
+    static int valu_index(const Variant& V) {   // position in tuning::kValuCost
+        static const int fdxr[8][3] = {{1, 1, 32}, {1, 2, 16}, {1, 2, 24}, {1, 2, 32}, {2, 2, 16}, {2, 4, 12}, {3, 4, 12}, {4, 4, 12}};
+        for (int k = 0; k < 8; ++k)
+            if (fdxr[k][0] == V.fd && fdxr[k][1] == V.xd && fdxr[k][2] == V.rpl) return k;
+        return 7;
+    }
+    // Predicted device seconds of a level on variant V: a launch constant, a cost per
+    // codeword, and a cost per hot-test pass, the passes being the fair-coin expectation
+    // scaled by the solve's structure factor; where a passing unit holds many passing
+    // codewords (loose thresholds) the exact tests' work grows with the codewords, so the
+    // count is max(passing units, passing codewords). Constants: tuning.hpp, measured by
+    // qubitserf_amd/tune.py at about one passing codeword per passing unit.
+    static double predict_s(const Solve& s, const Variant& V, u64 cw, long thr, int res) {
+        const double p = binom_cdf_below(test_bits(V, res), thr);
+        const double f = s.factor_at(p);
+        const double hits = std::max(fair_hits(V, cw, p) * f, (double)cw * std::min(1.0, p * f));
+        if (V.mm == 1) return tuning::kMatrixLaunch + (double)cw * tuning::kInt8Cost + hits * tuning::kInt8Hit;
+        if (V.mm >= 2) return tuning::kMatrixLaunch + (double)cw * tuning::kF4Cost + hits * tuning::kF4Hit;
+        const int k = valu_index(V);
+        return tuning::kValuLaunch + (double)cw * tuning::kValuCost[k] + hits * tuning::kValuHit[k];
+    }
+    // The level's threshold on residual weight: bound minus the fewest flagged rows any
+    // d-subset of an active generator contains.
+    static long level_thr(const Solve& s, const qsf_enum_plan& p) {
+        int min_off = p.d;
+        for (int g = 0; g < p.num_gamma; ++g) min_off = std::min(min_off, std::max(0, p.d - (p.K - s.nflag[g])));
+        return (long)p.current_best - min_off;
+    }
+    // After a tiled level ran to its end: the structure factor it measured, when the test
+    // was in the tail (fewer than half of its units would pass on fair coins) and expected
+    // at least 8 passes.
+    static void observe(Solve& s, const qsf_enum_plan& p, const Variant& V, u64 hits) {
+        if (V.mm == 3) return;
+        const u64 cw = sat_mul(s.C(p.K, p.d), (u64)p.num_gamma);
+        const double pr = binom_cdf_below(test_bits(V, std::max(1, s.maxres)), level_thr(s, p));
+        const double e = fair_hits(V, cw, pr);
+        if (e < 8.0 || test_unit(V) * pr > 0.5) return;
+        s.sf = std::min(4096.0, std::max(1.0, (double)hits / e));
+        s.sf_logp = std::log(pr);
     }
     static Variant choose_variant_k128(const Solve& s, const qsf_enum_plan& p, bool allow_mfma, u64 min_work) {
         static const Variant table[] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {2, 2, 16, 0}, {2, 4, 12, 0}, {3, 4, 12, 0},
@@ -2084,40 +2167,35 @@ struct Engine {
             if (f == 4) return table[5];
         }
         if (min_work == ~0ull) min_work = matrix_min_work();
-        allow_mfma = allow_mfma && sat_mul(s.C(p.K, p.d), (u64)p.num_gamma) >= min_work;
-        if (allow_mfma && !std::getenv("QSA_NO_MFMA")) {
-            // 30 residual bits + the flagged counts per codeword. Structured codes put more
-            // light codewords on those bits than fair coins would (the gross code: 12-16x at
-            // d = 9, 10); the int8 test is taken where a 32 x 256 group of codewords on
-            // fair-coin bits would reach the exact tests < 5% of the time. It stays ahead of
-            // the fp4 test even with a quarter of its groups in the exact tests (the l = 15
-            // bivariate-bicycle code at d = 8: 15.7 vs 17.4 ms).
-            int min_off = p.d;
-            for (int g = 0; g < p.num_gamma; ++g) min_off = std::min(min_off, std::max(0, p.d - (p.K - s.nflag[g])));
-            const long thr = (long)p.current_best - min_off;
-            const int res = std::max(1, s.maxres);
-            if (8192.0 * binom_cdf_below(std::min(30, res), thr) < 0.05)
-                return s.RD >= 3 ? table[10] : s.RD == 2 ? table[9] : table[8];
-            // fp4 32x32x64 products: 60 test bits per 8192-codeword group (< 2%). The 16x16x128
-            // form (124 bits, table[13]) is not chosen automatically: its 256-element stream
-            // chunks pay the per-chunk bound reload and barrier four times as often, and on
-            // the quasi-cyclic evaluation codes it ran at half the VALU kernel's rate.
-            const char* f4_rate = std::getenv("QSA_F4_RATE");   // tuning: the fp4 rule's bound (default 2%)
-            const double f4_max = f4_rate ? std::atof(f4_rate) : 0.02;
-            if (!std::getenv("QSA_NO_F4") && s.RD >= 2 && 8192.0 * binom_cdf_below(std::min(60, res), thr) < f4_max)
-                return s.RD >= 3 ? table[12] : table[11];
-        }
-        if (s.RD == 1) return table[0];
-        int min_off = p.d;
-        for (int g = 0; g < p.num_gamma; ++g) min_off = std::min(min_off, std::max(0, p.d - (p.K - s.nflag[g])));
-        const long thr = (long)p.current_best - min_off;
+        const u64 cw = sat_mul(s.C(p.K, p.d), (u64)p.num_gamma);
+        allow_mfma = allow_mfma && cw >= min_work && !std::getenv("QSA_NO_MFMA");
+        const long thr = level_thr(s, p);
         const int res = std::max(1, s.maxres);
-        auto rate = [&](int fd, int rpl) { return 64.0 * rpl * binom_cdf_below(std::min(32 * fd, res), thr); };
-        if (rate(1, 24) < 0.05) return table[6];   // {1, 2, 24}
-        if (s.RD == 2) return table[2];
-        if (rate(2, 12) < 0.05) return table[3];
-        if (s.RD >= 3 && rate(3, 12) < 0.05) return table[4];
-        return table[5];
+        // Every applicable variant is priced by the measured cost model; the cheapest runs.
+        // VALU variants test 32 FD of the RD residual dwords in the hot loop and hold XD.
+        Variant best = table[0];
+        double best_s = 1e300;
+        auto consider = [&](const Variant& v) {
+            const double t = predict_s(s, v, cw, thr, res);
+            if (t < best_s) { best_s = t; best = v; }
+        };
+        if (s.RD == 1) {
+            consider(table[0]);
+        } else {
+            consider(table[6]); consider(table[7]); consider(table[1]);
+            consider(s.RD == 2 ? table[2] : table[3]);
+            if (s.RD >= 3) { consider(table[4]); consider(table[5]); }
+        }
+        if (allow_mfma) {
+            // int8 32x32x32 products over 30 test bits; fp4 32x32x64 products over 60. The
+            // 16x16x128 fp4 form (124 bits, table[13]) is not chosen automatically: its
+            // 256-element stream chunks pay the per-chunk bound reload and barrier four
+            // times as often, and on the quasi-cyclic evaluation codes it ran at half the
+            // VALU kernel's rate.
+            consider(s.RD >= 3 ? table[10] : s.RD == 2 ? table[9] : table[8]);
+            if (!std::getenv("QSA_NO_F4") && s.RD >= 2) consider(s.RD >= 3 ? table[12] : table[11]);
+        }
+        return best;
     }
 
     // Decomposition of one level into products  F + subsets(L, lk) x subsets(Sr, sk).
@@ -2353,7 +2431,8 @@ struct Engine {
             nrect += grect[v].size();
         }
         const size_t o_cnt = 0, o_best = NV * 4 * sizeof(u64);
-        const size_t o_jobs = align16(o_best + (size_t)ni * sizeof(int));
+        const size_t o_hits = align16(o_best + (size_t)ni * sizeof(int));
+        const size_t o_jobs = align16(o_hits + (size_t)ni * sizeof(u64));
         const size_t o_rects = align16(o_jobs + (size_t)ni * sizeof(TJob));
         const size_t o_fixed = align16(o_rects + nrect * sizeof(Rect));
         const size_t total = align16(o_fixed + hfixed.size() * sizeof(u32) + 4);
@@ -2372,6 +2451,7 @@ struct Engine {
             J.fixed = reinterpret_cast<const u32*>(dev + o_fixed) + fixed_off[i];
             J.gamma = s.gamma.p;
             J.best = reinterpret_cast<int*>(dev + o_best) + i;
+            J.hits = reinterpret_cast<unsigned long long*>(dev + o_hits) + i;
             J.K = it.p.K; J.RD = s.RD; J.SD = s.SD; J.RW = s.RW; J.stride = it.p.stride;
             J.floor = it.floor;
             hj[i] = J;
@@ -2457,6 +2537,8 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(host, dev, o_jobs, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         res.assign(hb, hb + ni);
+        const unsigned long long* hh = reinterpret_cast<const unsigned long long*>(host + o_hits);
+        item_hits.assign(hh, hh + ni);
         const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(host + o_cnt);
         double ms_total = 0;
         stats.slow_hits = stats.medium_hits = 0;
@@ -2478,6 +2560,7 @@ struct Engine {
         stats.kernel_ms = ms_total;
     }
     unsigned long long out_rows_last = 0;   // collect mode: rows the last tiled run produced
+    std::vector<u64> item_hits;             // the last tiled run's hot-test passes per item
 
     void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap, int* bp,
                       unsigned long long* cntp) {
@@ -2623,6 +2706,30 @@ int qsf_hip_enumerate_floor_noalloc(const qsf_enum_plan* plan, int floor, int* b
 }
 
 size_t qsf_hip_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int qsf_hip_plan_variant(const qsf_enum_plan* plan, double factor, double at_p, int* out) {
+    std::string why;
+    if (!out || !qsa::plan_ok(plan, why)) {
+        engine().err = "qsf_hip_plan_variant: " + (out ? why : std::string("null out"));
+        return 1;
+    }
+    auto& e = engine();
+    std::lock_guard<std::mutex> lk(e.mu);
+    try {
+        qsa::Solve s;
+        e.build_host(s, *plan);   // host tables only: no device needed
+        if (factor > 0.0 && at_p > 0.0) {
+            s.sf = factor;
+            s.sf_logp = std::log(at_p);
+        }
+        const qsa::Engine::Variant v = qsa::Engine::choose_variant(s, *plan);
+        out[0] = v.fd; out[1] = v.xd; out[2] = v.rpl; out[3] = v.mm;
+        return 0;
+    } catch (const std::exception& x) {
+        e.err = std::string("qsf_hip_plan_variant: ") + x.what();
+        return 3;
+    }
+}
 
 int qsf_hip_enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out) {
     if (n < 0 || (n > 0 && (!plans || !best_out))) { engine().err = "qsf_hip_enumerate_batch: bad arguments"; return 1; }

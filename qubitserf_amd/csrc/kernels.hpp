@@ -78,6 +78,7 @@ struct TJob {
     const u32* fixed;        // [f][RW + 1]: residual+syndrome of a fixed row set, then its flagged count
     const u64* gamma;        // [g][row][stride] full rows (collect mode)
     int* best;               // running bound; starts at current_best
+    unsigned long long* hits;   // hot-test passes (medium-test entries) on this job, for the tuner
     int K, RD, SD, RW, stride;
     int floor;               // stop once best <= floor (floor < 0: never)
 };

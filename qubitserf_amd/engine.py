@@ -60,6 +60,20 @@ def enumerate_level(gamma, n, d, *, check=None, num_gamma=None, current_best=WEI
     return out.value
 
 
+def plan_variant(gamma, n, d, *, check=None, num_gamma=None, current_best=WEIGHT_NONE, factor=0.0, at_p=0.0):
+    """(fd, xd, rpl, mm) of the variant the automatic kernel choice picks for this level
+    (host work only). factor/at_p: a structure factor as if measured at a level with fair
+    per-codeword pass probability at_p (0: none measured)."""
+    L = _native.load()
+    p, keep = _plan(gamma, check, n, d, num_gamma, current_best, None)
+    out = (ctypes.c_int * 4)()
+    rc = L.qsf_hip_plan_variant(ctypes.byref(p), float(factor), float(at_p), out)
+    if rc:
+        _err(rc, "qsf_hip_plan_variant")
+    del keep
+    return tuple(out)
+
+
 def is_bound(gen, n, *, check=None, trials=256, seed=0x9E3779B9, init=WEIGHT_NONE) -> int:
     """Random information-set upper bound on the device: min(init, lightest nonzero logical
     row over `trials` systematic forms of the packed generator `gen` (K x stride uint64)).

@@ -319,3 +319,23 @@ def test_f4_chosen_on_high_thresholds(hip, monkeypatch):
         assert got == want, (K, d, best, got, want, st)
         seen.add(st["mfma"])
     assert seen == {0, 2}, seen
+
+
+def test_automatic_choice_runs_the_cost_models_pick(hip, monkeypatch):
+    """The engine runs, on a fresh solve, exactly the variant its measured cost model picks
+    (qsf_hip_plan_variant, host only), across tight and loose bounds, with results equal to
+    the oracle's. The size rule is lowered so that the oracle can check the levels."""
+    monkeypatch.setenv("QSA_MATRIX_MIN_WORK", "0")
+    rng = np.random.default_rng(41)
+    kinds = set()
+    for K, d, best in [(80, 5, 7), (80, 5, 13), (72, 4, 16), (64, 4, 30), (96, 4, 9), (60, 5, 22)]:
+        gamma, check = rand_plan(rng, K, 3, 1, 2, systematic=True)
+        n = 192
+        pick = hip.plan_variant(gamma, n, d, check=check, current_best=best)
+        got = hip.enumerate_level(gamma, n, d, check=check, current_best=best)
+        st = hip.last_stats()
+        ran = (st["fd"], st["xd"], st["rpl"], st["mfma"])
+        assert ran == pick, (K, d, best, ran, pick)
+        assert got == O.enumerate_plan(gamma, n, d, check=check, current_best=best), (K, d, best)
+        kinds.add(pick[3])
+    assert {0, 1, 2} <= kinds, kinds
