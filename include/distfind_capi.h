@@ -156,6 +156,9 @@ int distfind_session_add(void* session, const DistfindBatchItem* items, int coun
 int distfind_session_step(void* session, int* active);
 int distfind_session_poll(void* session, int64_t* ids, DistfindResult* out, int cap);
 int distfind_session_active(void* session);
+/* Device bytes the session's unfinished problems hold now (the arenas of their cached
+ * solves, measured with qsf_hip_solve_bytes); -1 for a NULL session. */
+int64_t distfind_session_device_bytes(void* session);
 void distfind_session_stats(void* session, DistfindBatchStats* out);
 void distfind_session_free(void* session);
 

@@ -84,6 +84,13 @@ int qsf_hip_enumerate_floor(const qsf_enum_plan* plan, int floor, int* best_out)
 int qsf_hip_enumerate_floor_noalloc(const qsf_enum_plan* plan, int floor, int* best_out);
 size_t qsf_hip_noalloc_scratch_bytes(void);
 
+/* Device memory accounting (no reference counterpart; for admission control):
+ * qsf_hip_solve_bytes: bytes of the device arena holding the solve cached under `key`
+ * (its generators, detector rows, binomials, flags and residual table), 0 if none is cached;
+ * qsf_hip_mem_info: the device's free and total memory (hipMemGetInfo). */
+uint64_t qsf_hip_solve_bytes(uint64_t key);
+int qsf_hip_mem_info(size_t* free_bytes, size_t* total_bytes);
+
 /* Batched min mode (no reference counterpart: the reference runs one level of one problem
  * per call). Plan j is one level of its own problem (distinct buffers_key per problem);
  * best_out[j] is what qsf_hip_enumerate_floor(&plans[j], floors[j], ...) would return

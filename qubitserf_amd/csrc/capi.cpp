@@ -391,6 +391,11 @@ int distfind_session_active(void* handle) {
     return h ? h->s->active() : -1;
 }
 
+int64_t distfind_session_device_bytes(void* handle) {
+    auto* h = static_cast<SessionHandle*>(handle);
+    return h ? (int64_t)h->s->device_bytes() : -1;
+}
+
 void distfind_session_stats(void* handle, DistfindBatchStats* out) {
     auto* h = static_cast<SessionHandle*>(handle);
     if (!h || !out) return;

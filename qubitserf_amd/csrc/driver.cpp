@@ -633,6 +633,15 @@ BatchSession::~BatchSession() = default;
 
 int BatchSession::active() const { return (int)impl_->live.size(); }
 
+unsigned long long BatchSession::device_bytes() const {
+    unsigned long long b = 0;
+    for (const auto& e : impl_->live) {
+        if (e.st->A.has) b += qsf_hip_solve_bytes(e.st->A.key);
+        if (e.st->pair && e.st->B.has) b += qsf_hip_solve_bytes(e.st->B.key);
+    }
+    return b;
+}
+
 std::vector<long long> BatchSession::add(std::vector<BatchProblem>&& probs) {
     Impl& I = *impl_;
     auto t0 = clk::now();

@@ -99,6 +99,9 @@ class BatchSession {
     std::vector<long long> add(std::vector<BatchProblem>&& probs);
     int step();                                  // returns the problems still unfinished
     int active() const;
+    // Device bytes the session's unfinished problems hold now: the arenas of their sides'
+    // cached solves (qsf_hip_solve_bytes), measured, not estimated.
+    unsigned long long device_bytes() const;
     std::vector<std::pair<long long, BzResult>> take_finished();
     BatchStats stats;
 

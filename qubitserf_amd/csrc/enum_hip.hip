@@ -2707,6 +2707,24 @@ int qsf_hip_enumerate_floor_noalloc(const qsf_enum_plan* plan, int floor, int* b
 
 size_t qsf_hip_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
+uint64_t qsf_hip_solve_bytes(uint64_t key) {
+    auto& e = engine();
+    std::lock_guard<std::mutex> lk(e.mu);
+    for (const auto& s : e.cache)
+        if (s->key == key && key != 0) return (uint64_t)s->arena.n * sizeof(qsa::u64);
+    return 0;
+}
+
+int qsf_hip_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    return run_guarded("qsf_hip_mem_info", [&](qsa::Engine&) {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) != hipSuccess) throw qsa::HipError{3, "hipMemGetInfo failed"};
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+        return 0;
+    });
+}
+
 int qsf_hip_plan_variant(const qsf_enum_plan* plan, double factor, double at_p, int* out) {
     std::string why;
     if (!out || !qsa::plan_ok(plan, why)) {

@@ -250,6 +250,11 @@ class Session:
     def active(self) -> int:
         return self.L.distfind_session_active(self.h)
 
+    @property
+    def device_bytes(self) -> int:
+        """Device bytes the unfinished problems hold now (measured solve arenas)."""
+        return int(self.L.distfind_session_device_bytes(self.h))
+
     def stats(self) -> dict:
         s = _native.DistfindBatchStats()
         self.L.distfind_session_stats(self.h, ctypes.byref(s))
@@ -262,6 +267,14 @@ class Session:
 
     def __del__(self):
         self.close()
+
+
+def device_memory():
+    """(free, total) device bytes (hipMemGetInfo)."""
+    L = _native.load()
+    f, t = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _check(L.qsf_hip_mem_info(ctypes.byref(f), ctypes.byref(t)), "device_memory")
+    return int(f.value), int(t.value)
 
 
 def last_batch_stats() -> dict:

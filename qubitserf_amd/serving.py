@@ -39,47 +39,58 @@ def make_requests(count, seed=0, classical_cap=int(1e10), css_cap=int(1e13)):
 
 
 def device_bytes(req):
-    """Device memory a request's solve holds (generators, residual tables, cached plans):
-    a bound from n alone, K <= n, up to n / 8 information sets."""
+    """Device memory a request's solve will hold before it has been built (generators,
+    residual tables, cached plans): a bound from n alone, K <= n, up to n / 8 information
+    sets. Once built, the session's measured bytes replace it."""
     n = req["n"]
     st = (n + 63) // 64
     sets = max(2, n // 8)
     return sets * n * (st * 8 + 8) + 64 * 1024
 
 
-def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30):
+def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_results=False):
     """Run the loop with continuous batching; returns latency and load statistics.
 
     Between rounds the server admits every request that has arrived, in arrival order,
-    while the device memory of the admitted solves stays within `mem_budget` (a request
-    that alone exceeds it is admitted only onto an idle device) and at most `max_active`
-    are in flight. Each round enumerates one level of every solve in flight; a solve
-    leaves as soon as it finishes, so small requests do not wait for large ones."""
+    while device memory stays within the budget (min(mem_budget, half the device's free
+    memory at start, hipMemGetInfo)) and at most `max_active` are in flight. The memory
+    in use is measured: the session's solve arenas as the engine holds them
+    (Session.device_bytes), plus the estimate of requests admitted in this round whose
+    solves are not built yet. A request that alone exceeds the budget is admitted only
+    onto an idle device. Each round enumerates one level of every solve in flight; a solve
+    leaves as soon as it finishes, so small requests do not wait for large ones.
+    keep_results: also return each request's (distance, lower_bound, proven, levels)
+    under "results", in request order."""
     from collections import deque
 
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
     sess = df.Session()
+    free, _total = df.device_memory()
+    budget = min(mem_budget, free // 2)
     clock = busy = 0.0
     nxt = 0
     queue = deque()
-    mem = 0
-    owner = {}                       # session id -> (request index, bytes)
+    peak = 0
+    owner = {}                       # session id -> request index
     lat = np.zeros(len(requests))
     proven = np.zeros(len(requests), bool)
+    results = [None] * len(requests)
     done = rounds = admitted_batches = 0
     while done < len(requests):
         while nxt < len(requests) and arrivals[nxt] <= clock:
             queue.append(nxt)
             nxt += 1
         take = []
+        held = sess.device_bytes      # measured arenas of the solves in flight
+        pending = 0                   # estimates of this round's admissions
         while queue and len(owner) + len(take) < max_active:
             b = device_bytes(requests[queue[0]])
-            if mem + b > mem_budget and (owner or take):
+            if held + pending + b > budget and (owner or take):
                 break
             r = queue.popleft()
             take.append(r)
-            mem += b
+            pending += b
         if take:
             t0 = time.perf_counter()
             ids = sess.add([{"problem": requests[r]["problem"], "work_cap": requests[r]["work_cap"]} for r in take])
@@ -88,28 +99,33 @@ def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30):
             busy += dt
             admitted_batches += 1
             for i, r in zip(ids, take):
-                owner[i] = (r, device_bytes(requests[r]))
+                owner[i] = r
         if sess.active > 0:
             t0 = time.perf_counter()
             sess.step()
             dt = time.perf_counter() - t0
+            peak = max(peak, sess.device_bytes)
             clock += dt
             busy += dt
             rounds += 1
         for i, res in sess.poll():
-            r, b = owner.pop(i)
-            mem -= b
+            r = owner.pop(i)
             lat[r] = clock - arrivals[r]
             proven[r] = bool(res.proven)
+            results[r] = (res.distance, res.lower_bound, res.proven, res.levels)
             done += 1
         if not owner and not queue and nxt < len(requests):
             clock = max(clock, arrivals[nxt])   # idle until the next arrival
     sess.close()
     makespan = clock - arrivals[0] if len(requests) else 0.0
-    return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
-                mean_ms=float(lat.mean() * 1e3), throughput_rps=len(requests) / max(makespan, 1e-9),
-                busy=busy / max(makespan, 1e-9), rounds=rounds, admissions=admitted_batches,
-                proven=int(proven.sum()), requests=len(requests), rate_rps=rate)
+    out = dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+               mean_ms=float(lat.mean() * 1e3), throughput_rps=len(requests) / max(makespan, 1e-9),
+               busy=busy / max(makespan, 1e-9), rounds=rounds, admissions=admitted_batches,
+               proven=int(proven.sum()), requests=len(requests), rate_rps=rate,
+               device_bytes_peak=int(peak), mem_budget=int(budget), clock="virtual")
+    if keep_results:
+        out["results"] = results
+    return out
 
 
 def calibrate_rate(requests, target_busy=0.7, seed=1, rounds=3):
