@@ -156,11 +156,22 @@ int distfind_session_add(void* session, const DistfindBatchItem* items, int coun
 int distfind_session_step(void* session, int* active);
 int distfind_session_poll(void* session, int64_t* ids, DistfindResult* out, int cap);
 int distfind_session_active(void* session);
+
 /* Device bytes the session's unfinished problems hold now (the arenas of their cached
  * solves, measured with qsf_hip_solve_bytes); -1 for a NULL session. */
 int64_t distfind_session_device_bytes(void* session);
 void distfind_session_stats(void* session, DistfindBatchStats* out);
 void distfind_session_free(void* session);
+
+/* Level schedule of later solves (process-wide; no reference counterpart). 0 (default) =
+ * the reference's: a generator of rank r < K is enumerated only at levels d with
+ * (d+1) - (K-r) > 0 (cpp/distfind/bz.cpp:142-144) but counted in the lower bound at every
+ * level (bz.cpp:159-161), which can certify a distance above the true one when a lighter
+ * codeword lies in that generator's skipped levels 1..K-r-1. 1 = sound: the generators that
+ * contribute before the bound can reach the upper bound are enumerated from level 1 and
+ * only they are counted. The initial setting is QSA_SOUND_BOUNDS from the environment.
+ * Returns the previous setting. Solves already in flight (sessions) keep theirs. */
+int distfind_set_sound_bounds(int on);
 
 /* Host logic exposed for tests (no device needed): the Edmonds matroid partition of the
  * columns of G (rows x cols uint8) into independent classes (cpp/distfind/css.cpp:274-328),

@@ -481,6 +481,27 @@ int is_seed(const Problem& pr, int init, int trials, unsigned seed, int nthreads
     return b;
 }
 
+// Sound level schedule (not in the reference). The reference enumerates a generator of
+// rank r < K only at levels d with (d+1) - (K-r) > 0 (bz.cpp:142-144) yet adds its
+// max(0, (d+1) - (K-r)) to the bound (bz.cpp:159-161); that contribution assumes every
+// combination of at most d rows of that generator was enumerated, including levels
+// 1..K-r-1, which were skipped, so a lighter codeword there can be missed. In sound mode
+// the generators that contribute by the level D where the bound first reaches the upper
+// bound (those with K - r <= D) are enumerated from level 1, and only they enter the bound.
+static bool g_sound = false;
+int sound_prefix(int K, const std::vector<int>& ranks, bool even, int upper) {
+    int D = K;
+    for (int d = 1; d <= K; ++d) {
+        int o = 0;
+        for (int r : ranks) o += std::max(0, (d + 1) - (K - r));
+        if (even) o += o & 1;
+        if (o >= upper) { D = d; break; }
+    }
+    int a = 0;
+    while (a < (int)ranks.size() && K - ranks[a] <= D) ++a;   // ranks sorted descending
+    return a;
+}
+
 // One BZ subproblem, stepped one weight level at a time (bz.cpp:196-272).
 struct Side {
     bool has = false, even = false;
@@ -513,7 +534,12 @@ struct Side {
         inner = pr.seed_upper > 0 ? pr.seed_upper : NONE;
         inner = is_seed(pr, inner, 256, 0x9e3779b9u, is_threads);
     }
+    mutable int nsound = -1;   // sound mode: generators enumerated from level 1
     int active(int d) const {
+        if (g_sound) {
+            if (nsound < 0) nsound = sound_prefix(K, ranks, even, inner);
+            return nsound;
+        }
         int a = 0;
         while (a < ng && (d + 1) - (K - ranks[a]) > 0) ++a;
         return a;
@@ -528,7 +554,8 @@ struct Side {
         inner = std::min(inner, found);
         ++level;
         outer = 0;
-        for (int g = 0; g < ng; ++g) outer += std::max(0, (d + 1) - (K - ranks[g]));
+        const int counted = g_sound ? nsound : ng;
+        for (int g = 0; g < counted; ++g) outer += std::max(0, (d + 1) - (K - ranks[g]));
         if (even) outer += outer & 1;
     }
 };
@@ -762,6 +789,13 @@ static void fill(OracleResult* o, const orc::Result& r) {
     o->levels = r.levels; o->seconds = r.seconds;
 }
 static int hw_threads(int t) { return t > 0 ? t : (int)std::max(1u, std::thread::hardware_concurrency()); }
+
+// Sound level schedule on (1) or off (0, the reference's); returns the previous setting.
+int oracle_set_sound(int on) {
+    const int old = orc::g_sound ? 1 : 0;
+    orc::g_sound = on != 0;
+    return old;
+}
 
 int oracle_enumerate(int n, int stride, int K, int d, int ng, const uint64_t* gamma,
                      int kcheck, const uint64_t* check, int current_best, int threads) {

@@ -174,6 +174,7 @@ def load() -> ctypes.CDLL:
     sig("distfind_session_poll", c_int, [hp, ctypes.POINTER(ctypes.c_int64), rp, c_int])
     sig("distfind_session_active", c_int, [hp])
     sig("distfind_session_device_bytes", ctypes.c_int64, [hp])
+    sig("distfind_set_sound_bounds", c_int, [c_int])
     sig("qsf_hip_solve_bytes", ctypes.c_uint64, [ctypes.c_uint64])
     sig("qsf_hip_mem_info", c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)])
     sig("distfind_session_stats", None, [hp, ctypes.POINTER(DistfindBatchStats)])

@@ -53,9 +53,23 @@ int check_args(const char* method, const char* backend, bool noncss = false) {
     return RC_OK;
 }
 
+// The sound level schedule (BzOptions::sound), process-wide: QSA_SOUND_BOUNDS=1 in the
+// environment at first use, or distfind_set_sound_bounds.
+std::atomic<int> g_sound{-1};
+bool sound_on() {
+    int v = g_sound.load();
+    if (v < 0) {
+        const char* e = std::getenv("QSA_SOUND_BOUNDS");
+        v = (e && std::atoi(e) != 0) ? 1 : 0;
+        g_sound.store(v);
+    }
+    return v != 0;
+}
+
 BzOptions options(int verbose) {
     BzOptions o;
     o.verbose = verbose != 0;
+    o.sound = sound_on();
     o.profile = std::getenv("DISTFIND_PROFILE") != nullptr;
     g_trace.clear();
     o.trace = &g_trace;
@@ -199,6 +213,12 @@ int distfind_backend_available(const char* name) {
 }
 
 const char* distfind_version(void) { return "0.1.0+gfx950"; }
+
+int distfind_set_sound_bounds(int on) {
+    const int old = sound_on() ? 1 : 0;
+    g_sound.store(on ? 1 : 0);
+    return old;
+}
 
 const char* distfind_last_error(void) { return g_err.c_str(); }
 

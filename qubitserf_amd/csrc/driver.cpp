@@ -138,6 +138,22 @@ void device_sets(std::vector<SetRequest*>& reqs, int trials = 0, u64 seed = 0) {
     }
 }
 
+// Sound schedule: the first level D at which the bound over every generator reaches
+// `upper` (K if never), and the generators that contribute by then (K - r <= D; ranks
+// sorted descending, so a prefix). Those are enumerated from level 1.
+int sound_prefix(int K, const std::vector<int>& ranks, bool even, int upper) {
+    int D = K;
+    for (int d = 1; d <= K; ++d) {
+        int o = 0;
+        for (int r : ranks) o += std::max(0, (d + 1) - (K - r));
+        if (even) o += o & 1;
+        if (o >= upper) { D = d; break; }
+    }
+    int a = 0;
+    while (a < (int)ranks.size() && K - ranks[a] <= D) ++a;
+    return a;
+}
+
 // One subproblem's level-stepping state. Owns the flat generator / detector arrays the
 // device caches under `key` for the whole solve.
 struct Side {
@@ -172,7 +188,12 @@ struct Side {
         inner = p.seed_upper > 0 ? p.seed_upper : kNoWeight;
     }
 
+    mutable int nsound = -1;   // sound schedule: generators enumerated from level 1 (a prefix)
     int active(int d) const {
+        if (opt->sound) {
+            if (nsound < 0) nsound = sound_prefix(K, ranks, even, inner);
+            return nsound;
+        }
         int a = 0;
         while (a < ng && (d + 1) - (K - ranks[a]) > 0) ++a;
         return a;
@@ -205,7 +226,8 @@ struct Side {
         ++level;
         work += st.codewords;
         outer = 0;
-        for (int g = 0; g < ng; ++g) outer += std::max(0, (d + 1) - (K - ranks[g]));
+        const int counted = opt->sound ? active(d) : ng;   // sound: only the generators enumerated from level 1
+        for (int g = 0; g < counted; ++g) outer += std::max(0, (d + 1) - (K - ranks[g]));
         if (even) outer += outer & 1;
         if (opt->profile)
             std::fprintf(stderr, "[prof hip] d=%d K=%d ng=%d/%d stride=%d work=%llu kernel=%d fd=%d/%d medium=%llu slow=%llu -> %.3f ms (device %.3f ms)\n",

@@ -30,6 +30,13 @@ struct BzOptions {
     bool verbose = false;
     bool profile = false;           // DISTFIND_PROFILE-style per-level lines on stderr
     int is_trials = 256;            // random information-set trials for the upper bound
+    // Sound level schedule (off = the reference's). The reference enumerates a generator of
+    // rank r < K only at levels d with (d+1) - (K-r) > 0 (cpp/distfind/bz.cpp:142-144) but
+    // adds its max(0, (d+1) - (K-r)) to the lower bound (bz.cpp:159-161), which assumes its
+    // levels 1..K-r-1 were enumerated too; a lighter codeword there can be missed. On: the
+    // generators that contribute by the level where the bound first reaches the upper bound
+    // are enumerated from level 1, and only they enter the bound.
+    bool sound = false;
     unsigned long long seed = 0x9e3779b9ull;
     std::vector<LevelTrace>* trace = nullptr;
     // Optional column order for the greedy information-set peel of a single-side solve (a

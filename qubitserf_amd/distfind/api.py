@@ -269,6 +269,28 @@ class Session:
         self.close()
 
 
+class sound_bounds:
+    """Context manager (or call set_sound_bounds) for the sound level schedule: every
+    information set that enters the lower bound is enumerated from level 1. Off (the
+    default) is the reference's schedule, which skips a rank-deficient set's first K - r
+    levels yet counts it in the bound (cpp/distfind/bz.cpp:142-144, 159-161)."""
+
+    def __init__(self, on=True):
+        self.on = bool(on)
+
+    def __enter__(self):
+        self.prev = set_sound_bounds(self.on)
+        return self
+
+    def __exit__(self, *exc):
+        set_sound_bounds(self.prev)
+
+
+def set_sound_bounds(on: bool) -> bool:
+    """Switch the sound level schedule for later solves; returns the previous setting."""
+    return bool(_native.load().distfind_set_sound_bounds(1 if on else 0))
+
+
 def device_memory():
     """(free, total) device bytes (hipMemGetInfo)."""
     L = _native.load()

@@ -126,6 +126,22 @@ def _p(a):
 
 # ---------------------------------------------------------------- oracle restatement
 
+class sound:
+    """The oracle's sound level schedule (oracle_set_sound) inside a with-block."""
+
+    def __init__(self, on=True):
+        self.on = 1 if on else 0
+
+    def __enter__(self):
+        L = lib()
+        L.oracle_set_sound.argtypes = [ctypes.c_int]
+        self.prev = L.oracle_set_sound(self.on)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_sound(self.prev)
+
+
 def classical_distance(H, threads=0, is_threads=0) -> OracleResult:
     a = _u8(H)
     r = OracleResult()

@@ -7,8 +7,10 @@ budget; a code over that budget carries no answer).
   batch_distance call for all 256 under the benchmark's work cap equals 256 one-code calls
   (distance, lower bound, proven, levels); every code the reference certified is either
   certified with the reference's distance or bracketed (lower bound <= d <= upper bound);
-  uncapped, every such code is certified with the reference's distance, and equals the
-  single-code entry point.
+  uncapped, on the sound level schedule, every such code is certified with the reference's
+  distance, and equals the single-code entry point. One reference certificate (code 32:
+  12) is refuted by a codeword of weight 11 in a level the reference's schedule skips
+  (tests/test_cpu_soundness.py); there the expected distance is 11.
 * Learned proposal (16 quasi-cyclic codes, L ~ U{20..40}, seed 0): the trained scorer's
   proposals (qubitserf_amd/proposal/weights.npz), uniform proposals and the natural
   information sets give identical distances wherever certified, and the reference's where
@@ -50,8 +52,15 @@ def _key(r):
     return (r.distance, r.lower_bound, r.proven, r.levels)
 
 
+# Reference certificates refuted by an explicit lighter codeword (the reference's level
+# schedule skips a rank-deficient information set's first K - r levels but counts it in the
+# bound; tests/test_cpu_soundness.py exhibits the codeword): (workload, index) -> distance.
+REFUTED = {("c3", 32): 11}
+
+
 def _against_reference(name, recs, results):
-    """Certified: the reference's distance. Capped: lower bound <= reference d <= upper."""
+    """Certified: the reference's distance (or, where a lighter codeword refutes the
+    reference's certificate, that codeword's weight). Capped: lower bound <= d <= upper."""
     checked = 0
     for rec, r in zip(recs, results):
         ref = rec["ref"]
@@ -59,10 +68,11 @@ def _against_reference(name, recs, results):
             continue
         checked += 1
         assert ref["proven"] == 1
+        true_d = REFUTED.get((name, rec["index"]), ref["distance"])
         if r.proven:
-            assert r.distance == ref["distance"], (name, rec["index"], _key(r), ref)
+            assert r.distance == true_d, (name, rec["index"], _key(r), ref)
         else:
-            assert r.lower_bound <= ref["distance"] <= r.distance, (name, rec["index"], _key(r), ref)
+            assert r.lower_bound <= true_d <= r.distance, (name, rec["index"], _key(r), ref)
     return checked
 
 
@@ -80,16 +90,19 @@ def test_c3_batched_equals_single_and_reference(hip):
         assert _key(a) == _key(b), (i, _key(a), _key(b))
     n = _against_reference("c3", recs, batched)
     assert n >= 100, f"only {n} reference-certified codes"
-    # the reference-certified codes without a cap: all certified, the reference's distance,
-    # and equal to the single-code entry point
+    # the reference-certified codes without a cap, on the sound level schedule: all
+    # certified, the reference's distance (the refuted certificate: the lighter codeword's
+    # weight), and equal to the single-code entry point
     idx = [rec["index"] for rec in recs if "timeout" not in rec["ref"]]
-    print(f"c3: {len(idx)} reference-certified codes uncapped", flush=True)
-    full = df.batch_distance([("classical", Hs[i]) for i in idx])
-    for i, r in zip(idx, full):
-        assert r.proven and r.distance == recs[i]["ref"]["distance"], (i, _key(r), recs[i]["ref"])
-    for i in idx[:24]:
-        one = df.api.classical_distance_raw(Hs[i], "bz", "gpu")
-        assert (one.distance, one.proven) == (recs[i]["ref"]["distance"], 1)
+    print(f"c3: {len(idx)} reference-certified codes uncapped, sound schedule", flush=True)
+    with df.sound_bounds():
+        full = df.batch_distance([("classical", Hs[i]) for i in idx])
+        for i, r in zip(idx, full):
+            want = REFUTED.get(("c3", i), recs[i]["ref"]["distance"])
+            assert r.proven and r.distance == want, (i, _key(r), recs[i]["ref"])
+        for i in idx[:24] + [32]:
+            one = df.api.classical_distance_raw(Hs[i], "bz", "gpu")
+            assert (one.distance, one.proven) == (REFUTED.get(("c3", i), recs[i]["ref"]["distance"]), 1)
 
 
 def test_c4_learned_uniform_and_natural_agree(hip):
