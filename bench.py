@@ -24,11 +24,14 @@ Prints ONE JSON line. Fields beyond the common contract:
                       same workload, on this host's cores
   config1             time-to-certify of the [64,32] random code of seed 0 on the device and
                       by the reference's CPU bz on this host (median of repeated solves)
+  collect             collect-mode throughput (codeaut_bz_collect, gross Z-side generator,
+                      subset sizes 1..8, weights 1..12): codewords/s, rows, device sort time
   batched_learned     plain (one code per call, natural information sets) vs batched +
                       learned (one call, trained scorer's proposals) on 256 random codes
                       (n 64..160, rate 0.3..0.6) and 16 quasi-cyclic codes (L 20..40), under
                       a per-code work cap; the elimination kernel's HBM rate; the serving
-                      loop's p50/p99 latency and throughput at 70% device load
+                      loop's p50/p99 latency and throughput at 70% device load (wall clock;
+                      the same rate on the reproducible virtual clock beside it)
 """
 from __future__ import annotations
 
@@ -210,10 +213,36 @@ def batched_learned(log, c4_cap=int(1e11)):
     out["learned_c4"] = compare("learned", codes.config4_eval(0, 16, 20, 40), int(c4_cap), 16)
     out["elimination"] = elimination(c3)
     reqs = serving.make_requests(96, seed=0)
-    rate, stats = serving.calibrate_rate(reqs, 0.7, rounds=3)
+    rate, stats = serving.calibrate_rate(reqs, 0.7, rounds=3)   # wall clock
     out["serving"] = stats
     log(f"serving: {json.dumps(stats)}")
+    out["serving_virtual_clock"] = serving.serve(reqs, rate, live=False)   # same rate, reproducible clock
     return out
+
+
+def collect_figure(p=8, keep=12, reps=3):
+    """Collect mode: codeaut_bz_collect of the gross code's Z-side generator in systematic
+    form (ker Hx in reduced row echelon form, K = 78, n = 144), subset sizes 1..p, weights
+    1..keep: one device pass over the p levels, device sort and dedup."""
+    from qubitserf_amd import codes, engine
+    from qubitserf_amd.codeaut import bz_collect
+
+    Hx, _ = codes.gross()
+    G = codes.gf2_rref(codes.gf2_kernel(Hx))
+    bz_collect(G, p, keep)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rows, combos = bz_collect(G, p, keep)
+        ts.append(time.perf_counter() - t0)
+    st = engine.last_stats()
+    cs = engine.last_collect_stats()
+    t = min(ts)
+    return {"code": "gross Z side, systematic generator K=78 n=144", "p": p, "keep": keep,
+            "codewords": st["codewords"], "combos": combos, "rows": int(rows.shape[0]),
+            "appended": cs["appended"], "wall_s": t, "codewords_per_s": st["codewords"] / t,
+            "kernel_ms": st["kernel_ms"], "sort_ms": cs["sort_ms"],
+            "device_codewords_per_s": st["codewords"] / (st["kernel_ms"] * 1e-3) if st["kernel_ms"] > 0 else None}
 
 
 def elimination(Hs=None, reps=1):
@@ -353,6 +382,7 @@ def main():
     if not args.no_extras:
         line["batched_learned"] = batched_learned(lambda m: print(m, file=sys.stderr, flush=True), int(args.c4_cap))
         line["config1"] = config1_gpu()
+        line["collect"] = collect_figure()
     if not args.no_cpu_baseline:
         host = host_cpus()
         threads = host["usable"]

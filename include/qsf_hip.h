@@ -106,6 +106,19 @@ int qsf_hip_enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n
 int qsf_hip_collect(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits, int64_t* rows);
 void qsf_hip_free(void* p);
 
+/* Collect mode over levels 1..plan->d at once (no reference counterpart; what
+ * cpp/codeaut/bz.cpp:78-113 does with one Backend::collect per level, a host sort and a
+ * dedup): every codeword of Hamming weight in [1, keep_weight] that is the XOR of at most
+ * plan->d rows of one of the first num_gamma generators, each once, sorted
+ * lexicographically by u64 word (word 0 first, unsigned; the order of the reference's
+ * RowLess, codeaut/bz.cpp:41-51). The levels append to one device buffer (one atomic per
+ * wave and append); the rows are sorted and deduplicated on the device and copied back once.
+ * *hits (free with qsf_hip_free) holds *rows rows of `stride` words. qsf_hip_last_stats:
+ * codewords of all levels and their enumeration time; qsf_hip_last_collect_stats: rows
+ * appended before deduplication and the device sort time. Returns as qsf_hip_collect. */
+int qsf_hip_collect_upto(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits, int64_t* rows);
+void qsf_hip_last_collect_stats(uint64_t* appended, double* sort_ms);
+
 /* Elimination phase on the device, many problems per call (one wave per problem, or per
  * problem and trial). Each job describes one BZ problem: `gen` K rows of stride =
  * (n+63)/64 u64 words spanning the code searched, `check` kcheck detector rows (a row is a

@@ -200,6 +200,8 @@ def load() -> ctypes.CDLL:
     sig("qsf_hip_noalloc_scratch_bytes", ctypes.c_size_t, [])
     sig("qsf_hip_plan_variant", c_int, [pp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(c_int)])
     sig("qsf_hip_collect", c_int, [pp, c_int, ctypes.POINTER(u64p), ctypes.POINTER(ctypes.c_int64)])
+    sig("qsf_hip_collect_upto", c_int, [pp, c_int, ctypes.POINTER(u64p), ctypes.POINTER(ctypes.c_int64)])
+    sig("qsf_hip_last_collect_stats", None, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)])
     sig("qsf_hip_free", None, [vp])
     sig("qsf_hip_last_stats", None, [ctypes.POINTER(HipStats)])
     sig("qsf_hip_last_error", c_char_p, [])

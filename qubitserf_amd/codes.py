@@ -34,6 +34,27 @@ def gf2_rank(M: np.ndarray) -> int:
     return rank
 
 
+def gf2_rref(M: np.ndarray) -> np.ndarray:
+    """Reduced row echelon form over GF(2) of the rows of M, zero rows dropped: a generator
+    of rowspace(M) that is the identity on its pivot columns (a systematic generator)."""
+    A = (np.asarray(M, np.uint8) & 1).copy()
+    m, n = A.shape
+    r = 0
+    for c in range(n):
+        if r == m:
+            break
+        nz = np.flatnonzero(A[r:, c])
+        if nz.size == 0:
+            continue
+        k = r + int(nz[0])
+        A[[r, k]] = A[[k, r]]
+        hit = np.flatnonzero(A[:, c])
+        hit = hit[hit != r]
+        A[hit] ^= A[r]
+        r += 1
+    return A[:r].copy()
+
+
 def gf2_kernel(H: np.ndarray) -> np.ndarray:
     """A basis (rows) of {x : H x = 0} over GF(2): reduced row echelon form of H, then one
     basis vector per free column (the free column set to 1, pivots solved)."""

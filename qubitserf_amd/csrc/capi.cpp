@@ -470,43 +470,29 @@ void* codeaut_bz_collect(const uint8_t* G, int32_t m, int32_t n, int32_t p, int3
     const int maxsw = std::min<int>(p, m);
     static std::atomic<u64> keys{0xc011ec7000000000ull};
     const u64 key = ++keys;
-    std::vector<u64> hits;
-    try {
-        for (int sw = 1; sw <= maxsw; ++sw) {
-            qsf_enum_plan pl{};
-            pl.n = n; pl.stride = st; pl.K = m; pl.d = sw;
-            pl.num_gamma = 1; pl.num_gamma_total = 1;
-            pl.gamma = Gm.data.data(); pl.kcheck = 0; pl.check = nullptr;
-            pl.current_best = kNoWeight; pl.buffers_key = key;
-            uint64_t* buf = nullptr;
-            int64_t rows = 0;
-            int rc = qsf_hip_collect(&pl, keep_weight, &buf, &rows);
-            if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
-            hits.insert(hits.end(), buf, buf + rows * st);
-            qsf_hip_free(buf);
-            // combination count, saturating like the reference's int64 counter (codeaut/bz.cpp:88-95)
-            long double c = 1;
-            for (int i = 0; i < sw; ++i) c = c * (m - i) / (i + 1);
-            int64_t total = c > (long double)kComboCap ? kComboCap : (int64_t)(c + 0.5L);
-            H->combos = H->combos > kComboCap - total ? kComboCap : H->combos + total;
-        }
-    } catch (const EngineError& e) {
-        g_err = e.what;
+    // every subset size 1..maxsw in one device pass; rows sorted (lexicographic over the
+    // packed words, cpp/codeaut/bz.cpp:98-113) and deduplicated on the device
+    qsf_enum_plan pl{};
+    pl.n = n; pl.stride = st; pl.K = m; pl.d = maxsw;
+    pl.num_gamma = 1; pl.num_gamma_total = 1;
+    pl.gamma = Gm.data.data(); pl.kcheck = 0; pl.check = nullptr;
+    pl.current_best = kNoWeight; pl.buffers_key = key;
+    uint64_t* buf = nullptr;
+    int64_t rows = 0;
+    const int rc = qsf_hip_collect_upto(&pl, keep_weight, &buf, &rows);
+    if (rc != 0) {
+        g_err = qsf_hip_last_error();
         H->ok = false;
         return H;
     }
-    // sort and deduplicate (lexicographic over the packed words, cpp/codeaut/bz.cpp:98-113)
-    const int64_t nrows = (int64_t)hits.size() / st;
-    std::vector<int64_t> idx(nrows);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
-        return std::lexicographical_compare(hits.begin() + a * st, hits.begin() + a * st + st, hits.begin() + b * st,
-                                            hits.begin() + b * st + st);
-    });
-    for (int64_t i = 0; i < nrows; ++i) {
-        const u64* a = hits.data() + idx[i] * st;
-        if (i > 0 && std::memcmp(a, hits.data() + idx[i - 1] * st, (size_t)st * 8) == 0) continue;
-        H->rows.insert(H->rows.end(), a, a + st);
+    H->rows.assign(buf, buf + rows * st);
+    qsf_hip_free(buf);
+    for (int sw = 1; sw <= maxsw; ++sw) {
+        // combination count, saturating like the reference's int64 counter (codeaut/bz.cpp:88-95)
+        long double c = 1;
+        for (int i = 0; i < sw; ++i) c = c * (m - i) / (i + 1);
+        int64_t total = c > (long double)kComboCap ? kComboCap : (int64_t)(c + 0.5L);
+        H->combos = H->combos > kComboCap - total ? kComboCap : H->combos + total;
     }
     H->count = (int64_t)H->rows.size() / st;
     H->ok = true;

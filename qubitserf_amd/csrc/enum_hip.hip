@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/qsf_hip.h"
+#include "collect.hpp"
 #include "elim.hpp"
 #include "kernels.hpp"
 #include "masks.hpp"
@@ -58,6 +59,19 @@ __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
     return v;
+}
+
+// Collect mode: one atomic per wave for the rows its active lanes append. Every active
+// lane calls it; the lanes with `hit` get consecutive slots (lane order), the others ~0.
+__device__ __forceinline__ unsigned long long wave_append(unsigned long long* rows, bool hit) {
+    const u64 m = __ballot(hit);
+    if (m == 0ull) return ~0ull;
+    const int leader = __builtin_ctzll(m);
+    const int lane = (int)(threadIdx.x & 63);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(rows, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return hit ? base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull)) : ~0ull;
 }
 
 template <int FD>
@@ -269,8 +283,9 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                             w += __popc(v);
                         }
                         if (a.collect) {
-                            if (w >= 1 && w < bound) {   // bound = keep + 1
-                                const unsigned long long slot = atomicAdd(a.out_rows, 1ull);
+                            // bound = keep + 1; the lanes of this candidate round append together
+                            const unsigned long long slot = wave_append(a.out_rows, w >= 1 && w < bound);
+                            if (slot != ~0ull) {
                                 if (slot < a.out_cap) {
                                     const int stride = J->stride;
                                     const u64* G = J->gamma + (size_t)g * K * stride;
@@ -1098,14 +1113,13 @@ __global__ __launch_bounds__(256) void bz_rowsweep(SweepArgs a) {
                 cw[w] = base[w] ^ G[(size_t)last * st + w];
                 wt += __popcll(cw[w]);
             }
-            if (wt >= bound) continue;
-            if (a.collect) {
-                if (wt == 0) continue;
-                unsigned long long slot = atomicAdd(a.out_rows, 1ull);
+            if (a.collect) {   // the lanes of this step append together (one atomic per wave)
+                const unsigned long long slot = wave_append(a.out_rows, wt >= 1 && wt < bound);
                 if (slot < a.out_cap)
                     for (int w = 0; w < st; ++w) a.out[slot * st + w] = cw[w];
                 continue;
             }
+            if (wt >= bound) continue;
             bool logical = a.kcheck == 0;
             for (int c = 0; c < a.kcheck && !logical; ++c) {
                 u64 acc = 0;
@@ -2309,7 +2323,11 @@ struct Engine {
     //   [groups x 4 u64 counters | items x int bounds | items x TJob | rects | fixed rows],
     // one persistent launch per variant (tiles pulled from the group's counter), one
     // read-back of counters and bounds. res[i] = the bound job i ended with.
-    void run_tiled(std::vector<TItem>& items, bool collect, int keep, u64 cap, std::vector<int>& res) {
+    // shared_rows (collect mode): one device row counter for every launch of this run and
+    // of earlier runs appending to the same output (levels 1..d of one collection);
+    // nullptr: each variant counts its own rows from 0.
+    void run_tiled(std::vector<TItem>& items, bool collect, int keep, u64 cap, std::vector<int>& res,
+                   unsigned long long* shared_rows = nullptr) {
         constexpr int NV = kVariants;
         static const Variant vtab[NV] = {
             {1, 1, 32, 0},    {1, 2, 16, 0},    {1, 2, 24, 0},    {1, 2, 32, 0},    {2, 2, 16, 0},    {2, 4, 12, 0},
@@ -2491,7 +2509,7 @@ struct Engine {
             a.slow_hits = cnt + 4 * v + 1;
             a.collect = collect ? 1 : 0;
             a.keep = keep;
-            a.out_rows = cnt + 4 * v + 3;
+            a.out_rows = shared_rows ? shared_rows : cnt + 4 * v + 3;
             a.out = out.p;
             a.out_cap = cap;
             const u64 grid = std::min<u64>(gtiles[v], V.mm > 0 ? mfma_groups : waves);
@@ -2560,6 +2578,7 @@ struct Engine {
         stats.kernel_ms = ms_total;
     }
     unsigned long long out_rows_last = 0;   // collect mode: rows the last tiled run produced
+    DevBuf<u64> sorted;                      // collect_upto: the deduplicated rows
     std::vector<u64> item_hits;             // the last tiled run's hot-test passes per item
 
     void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap, int* bp,
@@ -2585,6 +2604,76 @@ struct Engine {
         }
         HIPCHK(hipEventRecord(ev1, stream));
     }
+
+    // ---- collect mode, levels 1..d at once -------------------------------------------
+    // Every level's rows are appended on the device to one buffer (one shared counter, one
+    // atomic per wave and append), then sorted and deduplicated on the device; one copy back.
+    int collect_upto(const qsf_enum_plan& p0, int keep, uint64_t** hits, int64_t* rows) {
+        stats = qsf_hip_stats{};
+        *hits = nullptr;
+        *rows = 0;
+        Solve& s = solve_for(p0);
+        const int st = p0.stride;
+        for (int d = 1; d <= p0.d; ++d) stats.codewords = sat_add(stats.codewords, sat_mul(s.C(p0.K, d), (u64)p0.num_gamma));
+        if (keep <= 0 || p0.num_gamma == 0 || stats.codewords == 0) return 0;
+        if (stats.codewords > (1ull << 62)) throw HipError{4, "collection has more than 2^62 codewords"};
+        int* bp = dev_best();
+        unsigned long long* cntp = dev_counter();
+        u64 cap = std::max<u64>(1ull << 16, collect_cap);
+        for (;;) {
+            out.reserve(cap * (u64)st);
+            HIPCHK(hipMemsetAsync(cntp, 0, 4 * sizeof(unsigned long long), stream));
+            double ms_total = 0;
+            for (int d = 1; d <= p0.d; ++d) {
+                qsf_enum_plan p = p0;
+                p.d = d;
+                const u64 cw = sat_mul(s.C(p.K, d), (u64)p.num_gamma);
+                if (cw == 0) continue;
+                if (tileable(s, p) && (policy == 2 || (policy == 0 && cw >= (1ull << 16)))) {
+                    qsf_enum_plan pv = p;
+                    pv.current_best = keep + 1;
+                    std::vector<TItem> items{TItem{&s, p, -1, choose_variant(s, pv, false)}};
+                    std::vector<int> res;
+                    run_tiled(items, true, keep, cap, res, cntp + 3);
+                    ms_total += stats.kernel_ms;
+                } else {
+                    launch_sweep(s, p, true, keep, cap, bp, cntp);
+                    HIPCHK(hipStreamSynchronize(stream));
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+                    ms_total += ms;
+                }
+            }
+            unsigned long long got = 0;
+            HIPCHK(hipMemcpyAsync(&got, cntp + 3, sizeof(got), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            stats.kernel_ms = ms_total;
+            if (got > cap) { cap = got; continue; }   // overflow: run again with room for all
+            collect_cap = std::max<u64>(collect_cap, got);
+            u64 kept = 0;
+            if (got) {
+                sorted.reserve(got * (u64)st);
+                std::string why;
+                HIPCHK(hipEventRecord(ev0, stream));
+                if (int rc = sort_dedup_rows(out.p, got, st, sorted.p, &kept, stream, why)) throw HipError{rc, why};
+                HIPCHK(hipEventRecord(ev1, stream));
+                HIPCHK(hipEventSynchronize(ev1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+                sort_ms = ms;
+            }
+            u64* h = (u64*)std::malloc(std::max<u64>(1, kept) * (u64)st * 8);
+            if (!h) throw HipError{3, "host allocation failed"};
+            if (kept) HIPCHK(hipMemcpy(h, sorted.p, kept * (u64)st * 8, hipMemcpyDeviceToHost));
+            appended_last = got;
+            *hits = h;
+            *rows = (int64_t)kept;
+            return 0;
+        }
+    }
+    u64 collect_cap = 0;           // largest collection so far (the next one's first capacity)
+    u64 appended_last = 0;         // rows the last collect_upto appended before deduplication
+    double sort_ms = 0;            // its device sort + dedup time
 
     // ---- collect mode ---------------------------------------------------------------
     int collect(const qsf_enum_plan& p, int keep, uint64_t** hits, int64_t* rows) {
@@ -2804,6 +2893,21 @@ int qsf_hip_is_bound(const uint64_t* gen, int K, int n, const uint64_t* check, i
 
 void qsf_hip_last_elim_stats(qsf_hip_elim_stats* out) {
     if (out) *out = engine().el_stats;
+}
+
+int qsf_hip_collect_upto(const qsf_enum_plan* plan, int keep_weight, uint64_t** hits, int64_t* rows) {
+    std::string why;
+    if (!hits || !rows || !qsa::plan_ok(plan, why)) {
+        engine().err = "qsf_hip_collect_upto: " + (hits && rows ? why : std::string("null output"));
+        return 1;
+    }
+    return run_guarded("qsf_hip_collect_upto", [&](qsa::Engine& e) { return e.collect_upto(*plan, keep_weight, hits, rows); });
+}
+
+void qsf_hip_last_collect_stats(uint64_t* appended, double* sort_ms) {
+    auto& e = engine();
+    if (appended) *appended = e.appended_last;
+    if (sort_ms) *sort_ms = e.sort_ms;
 }
 
 void qsf_hip_free(void* p) { std::free(p); }

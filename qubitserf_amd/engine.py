@@ -187,6 +187,34 @@ def collect_level(gamma, n, d, keep_weight, *, num_gamma=None, key=None):
         del keep
 
 
+def collect_upto(gamma, n, d, keep_weight, *, num_gamma=None, key=None):
+    """(rows, stride) uint64 array of every distinct XOR of 1..d rows with weight in
+    [1, keep_weight], sorted lexicographically by word (qsf_hip_collect_upto: one device
+    pass over all levels, device sort and dedup)."""
+    L = _native.load()
+    p, keep = _plan(gamma, None, n, d, num_gamma, WEIGHT_NONE, key)
+    buf = _native.u64p()
+    rows = ctypes.c_int64(0)
+    rc = L.qsf_hip_collect_upto(ctypes.byref(p), int(keep_weight), ctypes.byref(buf), ctypes.byref(rows))
+    if rc:
+        _err(rc, "qsf_hip_collect_upto")
+    stride = p.stride
+    try:
+        if rows.value == 0:
+            return np.zeros((0, stride), np.uint64)
+        return np.ctypeslib.as_array(buf, shape=(rows.value * stride,)).copy().reshape(rows.value, stride)
+    finally:
+        L.qsf_hip_free(ctypes.cast(buf, ctypes.c_void_p))
+        del keep
+
+
+def last_collect_stats() -> dict:
+    """Rows the last collect_upto appended before deduplication, and its device sort ms."""
+    a, ms = ctypes.c_uint64(0), ctypes.c_double(0)
+    _native.load().qsf_hip_last_collect_stats(ctypes.byref(a), ctypes.byref(ms))
+    return {"appended": int(a.value), "sort_ms": float(ms.value)}
+
+
 class kernel_policy:
     """Context manager forcing the kernel choice: "auto", "sweep" or "tiled"."""
 

@@ -2,12 +2,17 @@
 process, the server admits what has arrived (within a device-memory budget) into a
 continuous batch, and records each request's latency.
 
-The arrival clock is virtual and the service is real: arrivals are drawn in advance; the
-server admits arrived requests into a continuous batch (distfind.Session), runs one round
-at a time on the device and advances the clock by each admission's and round's measured
-wall time; when idle it jumps the clock to the next arrival. Latencies and device load are thus those of the real engine under that
-arrival process, reproducible from the seed. The rate is tuned so that the device is busy
-for the target fraction of the time (busy = sum of batch times / makespan).
+Arrivals are drawn in advance from the seed. Two clocks:
+* live (the default): the wall clock. A request counts as arrived once its arrival time
+  has passed since the start; the server admits arrived requests into a continuous batch
+  (distfind.Session), runs one round at a time on the device, and sleeps until the next
+  arrival when it has nothing to do. Latency = wall time from arrival to the poll that
+  returns the result.
+* virtual: the same loop, with the clock advanced by each admission's and round's measured
+  wall time and jumped to the next arrival when idle, so the figures are reproducible and
+  free of host scheduling noise.
+The rate is tuned so that the device is busy for the target fraction of the time
+(busy = time spent in admissions and rounds / makespan).
 
 Request mix: half classical codes (n ~ U{64..256}, rate ~ U[0.3, 0.6], Bernoulli(1/2) H,
 full rank), half bivariate-bicycle CSS codes (l in {6, 9, 12, 15}, m = 6, A = x^3 + y + y^2,
@@ -48,7 +53,7 @@ def device_bytes(req):
     return sets * n * (st * 8 + 8) + 64 * 1024
 
 
-def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_results=False):
+def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_results=False, live=True):
     """Run the loop with continuous batching; returns latency and load statistics.
 
     Between rounds the server admits every request that has arrived, in arrival order,
@@ -60,7 +65,7 @@ def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_resul
     onto an idle device. Each round enumerates one level of every solve in flight; a solve
     leaves as soon as it finishes, so small requests do not wait for large ones.
     keep_results: also return each request's (distance, lower_bound, proven, levels)
-    under "results", in request order."""
+    under "results", in request order. live: wall clock (True) or virtual clock."""
     from collections import deque
 
     rng = np.random.default_rng(seed)
@@ -68,7 +73,30 @@ def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_resul
     sess = df.Session()
     free, _total = df.device_memory()
     budget = min(mem_budget, free // 2)
-    clock = busy = 0.0
+    busy = 0.0
+    if live:
+        t_start = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t_start
+
+        def spend(dt):   # wall time passes by itself
+            pass
+
+        def idle_until(t):
+            time.sleep(max(0.0, t - now()))
+    else:
+        vclock = [0.0]
+
+        def now():
+            return vclock[0]
+
+        def spend(dt):
+            vclock[0] += dt
+
+        def idle_until(t):
+            vclock[0] = max(vclock[0], t)
+
     nxt = 0
     queue = deque()
     peak = 0
@@ -78,7 +106,7 @@ def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_resul
     results = [None] * len(requests)
     done = rounds = admitted_batches = 0
     while done < len(requests):
-        while nxt < len(requests) and arrivals[nxt] <= clock:
+        while nxt < len(requests) and arrivals[nxt] <= now():
             queue.append(nxt)
             nxt += 1
         take = []
@@ -95,7 +123,7 @@ def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_resul
             t0 = time.perf_counter()
             ids = sess.add([{"problem": requests[r]["problem"], "work_cap": requests[r]["work_cap"]} for r in take])
             dt = time.perf_counter() - t0
-            clock += dt
+            spend(dt)
             busy += dt
             admitted_batches += 1
             for i, r in zip(ids, take):
@@ -105,30 +133,31 @@ def serve(requests, rate, seed=1, max_active=256, mem_budget=1 << 30, keep_resul
             sess.step()
             dt = time.perf_counter() - t0
             peak = max(peak, sess.device_bytes)
-            clock += dt
+            spend(dt)
             busy += dt
             rounds += 1
         for i, res in sess.poll():
             r = owner.pop(i)
-            lat[r] = clock - arrivals[r]
+            lat[r] = now() - arrivals[r]
             proven[r] = bool(res.proven)
             results[r] = (res.distance, res.lower_bound, res.proven, res.levels)
             done += 1
         if not owner and not queue and nxt < len(requests):
-            clock = max(clock, arrivals[nxt])   # idle until the next arrival
+            idle_until(arrivals[nxt])   # nothing to do before the next arrival
+    end = now()
     sess.close()
-    makespan = clock - arrivals[0] if len(requests) else 0.0
+    makespan = end - arrivals[0] if len(requests) else 0.0
     out = dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
                mean_ms=float(lat.mean() * 1e3), throughput_rps=len(requests) / max(makespan, 1e-9),
                busy=busy / max(makespan, 1e-9), rounds=rounds, admissions=admitted_batches,
                proven=int(proven.sum()), requests=len(requests), rate_rps=rate,
-               device_bytes_peak=int(peak), mem_budget=int(budget), clock="virtual")
+               device_bytes_peak=int(peak), mem_budget=int(budget), clock="wall" if live else "virtual")
     if keep_results:
         out["results"] = results
     return out
 
 
-def calibrate_rate(requests, target_busy=0.7, seed=1, rounds=3):
+def calibrate_rate(requests, target_busy=0.7, seed=1, rounds=3, live=True):
     """Arrival rate for the target device load: start from the measured mean service time
     of single requests, then rescale by the observed load."""
     sample = requests[: min(16, len(requests))]
@@ -139,7 +168,7 @@ def calibrate_rate(requests, target_busy=0.7, seed=1, rounds=3):
     rate = target_busy / per
     best = None
     for _ in range(rounds):
-        stats = serve(requests, rate, seed)
+        stats = serve(requests, rate, seed, live=live)
         if best is None or abs(stats["busy"] - target_busy) < abs(best[1]["busy"] - target_busy):
             best = (rate, stats)
         if abs(stats["busy"] - target_busy) < 0.05:
