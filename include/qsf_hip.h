@@ -98,6 +98,17 @@ int qsf_hip_mem_info(size_t* free_bytes, size_t* total_bytes);
  * launch per kernel variant; qsf_hip_last_stats then sums codewords and device time. */
 int qsf_hip_enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out);
 
+/* Several levels of each of many problems in one batched run (no reference counterpart;
+ * the batched solver's lookahead rounds). Plans with equal groups[j] (0 <= groups[j] < n) are
+ * levels of one problem (same buffers_key): each enumerates on its own bound, starting at
+ * its current_best, so best_out[j] is what qsf_hip_enumerate_floor(&plans[j], floors[j], ...)
+ * returns; the group also keeps a shared bound (the minimum found by any of its plans), and
+ * a tile of plan j is skipped once that shared bound is <= floors[j] (the level cannot change
+ * the outcome: an earlier level of the same launch settled the problem). skipped_out[j]
+ * (may be NULL) counts plan j's tiles skipped that way; best_out[j] is exact when it is 0. */
+int qsf_hip_enumerate_levels(const qsf_enum_plan* plans, const int* floors, const int* groups, int n, int* best_out,
+                             uint64_t* skipped_out);
+
 /* Collect mode (replaces CudaBackend::collect, backend_cuda.cu:561-645): every codeword of
  * Hamming weight in [1, keep_weight] that is the XOR of d rows of one of the first
  * num_gamma generators, `stride` u64 words each, duplicates allowed. On success *hits is a

@@ -211,6 +211,16 @@ struct Side {
         return true;
     }
 
+    // The bound after level d (0 before level 1), as apply() computes it.
+    int outer_at(int d) const {
+        if (d <= 0) return 0;
+        const int counted = opt->sound ? active(d) : ng;
+        int o = 0;
+        for (int g = 0; g < counted; ++g) o += std::max(0, (d + 1) - (K - ranks[g]));
+        if (even) o += o & 1;
+        return o;
+    }
+
     u64 level_work(int d) const {
         const int act = active(d);
         long double c = 1;
@@ -701,53 +711,160 @@ std::vector<long long> BatchSession::add(std::vector<BatchProblem>&& probs) {
     return ids;
 }
 
+// One round. With lookahead (opt-in: QSA_LOOKAHEAD=1) every unfinished
+// problem enqueues every level it can still need: each side's levels from its next one up
+// to the first level whose bound reaches the current upper bound (the level where it would
+// stop or resolve if nothing lighter turns up), within the work cap. All of them run in one
+// batched device call (qsf_hip_enumerate_levels): each level on its own bound, so its
+// result is the one the level would give alone, and the levels of one problem share a
+// bound that skips a level once an earlier level has settled the problem. Then each
+// problem replays its levels in the order bz_distance / bz_min_interleaved visit them and
+// stops where they would stop. A level with skipped tiles is not replayed: it runs again,
+// exactly, in the next round (rare: it needs a light codeword found at a higher level).
+// Results therefore equal the level-by-level loop's. Without lookahead (the default) a
+// round is one level per problem. Lookahead is off by default because each level's kernel
+// is chosen on the bound at the start of the round: on random codes, whose upper bound
+// keeps falling from level to level, the later levels are priced on a looser bound than
+// the one they meet when they run, and fall to the VALU test where one level per round
+// takes the matrix test (256 codes n ~ U{64..160}: 96.5 ms of kernels against 83.3).
 int BatchSession::step() {
     Impl& I = *impl_;
+    const char* la_env = std::getenv("QSA_LOOKAHEAD");
+    const bool lookahead = la_env && std::atoi(la_env) != 0;
+    struct Enq {
+        Side* side;
+        int d;
+        int plan;      // index into plans, -1: no generator active at this level
+        u64 cw;
+    };
     std::vector<qsf_enum_plan> plans;
-    std::vector<int> floors, found;
-    std::vector<Stepper*> who;
-    std::vector<u64> cw;
-    for (auto& e : I.live) {
-        Stepper& S = *e.st;
-        while (!S.done && S.cur) {
-            Side* side = S.cur;
-            const u64 w = side->level_work(S.d);
-            const u64 spent = S.A.work + (S.pair ? S.B.work : 0);
-            const u64 cap = S.cap ? S.cap : I.work_cap;
-            if (cap && (w > cap || spent > cap - w)) { S.capped = S.done = true; break; }
-            qsf_enum_plan pl;
-            if (side->plan(S.d, pl)) {
-                plans.push_back(pl);
-                floors.push_back(side->outer);
-                who.push_back(&S);
-                cw.push_back(w);
-                break;
+    std::vector<int> floors, groups, offs;   // offs: level offset within its problem (for ordering)
+    std::vector<std::vector<Enq>> enq(I.live.size());
+    for (size_t k = 0; k < I.live.size(); ++k) {
+        Stepper& S = *I.live[k].st;
+        if (S.done || !S.cur) continue;
+        const u64 cap = S.cap ? S.cap : I.work_cap;
+        u64 spent = S.A.work + (S.pair ? S.B.work : 0);
+        // the visiting order of the level loop from the current position, without results:
+        // single side d, d+1, ...; a pair alternates sides at each level
+        const int best_now = S.pair ? S.best : S.A.inner;
+        auto horizon = [&](const Side& side) {   // last level this side can still need
+            for (int d = 1; d <= side.K; ++d)
+                if (side.outer_at(d) >= std::min(best_now, side.inner)) return d;
+            return side.K;
+        };
+        const int hA = S.A.has ? horizon(S.A) : 0, hB = (S.pair && S.B.has) ? horizon(S.B) : 0;
+        int d = S.d, turn = S.turn, off = 0;
+        const int max_levels = lookahead ? 64 : 1;
+        for (int steps = 0; (int)enq[k].size() < max_levels && steps < 4 * 256; ++steps) {
+            Side* side = !S.pair ? &S.A : (turn == 0 ? &S.A : &S.B);
+            const int h = side == &S.A ? hA : hB;
+            if (side->has && !side->resolved && d <= side->K && d <= h) {
+                const u64 w = side->level_work(d);
+                if (cap && (w > cap || spent > cap - w)) break;   // the level loop stops at the cap here
+                spent += w;
+                qsf_enum_plan pl;
+                if (side->plan(d, pl)) {
+                    enq[k].push_back(Enq{side, d, (int)plans.size(), w});
+                    plans.push_back(pl);
+                    floors.push_back(side->outer_at(d - 1));
+                    groups.push_back((int)k);
+                    offs.push_back(off++);
+                } else {
+                    enq[k].push_back(Enq{side, d, -1, w});
+                }
             }
-            S.after(kNoWeight, 0.0, qsf_hip_stats{});   // no generator active at this level
-            S.next();
+            if (!S.pair) {
+                if (++d > std::min(S.A.K, hA)) break;
+            } else {
+                if (++turn == 2) {
+                    turn = 0;
+                    if (++d > std::max(hA, hB)) break;
+                }
+            }
         }
     }
-    if (!plans.empty()) {
-        found.assign(plans.size(), kNoWeight);
+    // levels in ascending order across problems (the launch order of their tiles), so that
+    // low levels, which can settle a problem, run first
+    std::vector<int> order(plans.size());
+    for (size_t j = 0; j < order.size(); ++j) order[j] = (int)j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return offs[a] < offs[b]; });
+    std::vector<int> pos(plans.size());
+    std::vector<qsf_enum_plan> oplans(plans.size());
+    std::vector<int> ofloors(plans.size()), ogroups(plans.size());
+    for (size_t r = 0; r < order.size(); ++r) {
+        pos[order[r]] = (int)r;
+        oplans[r] = plans[order[r]];
+        ofloors[r] = floors[order[r]];
+        ogroups[r] = groups[order[r]];
+    }
+    std::vector<int> found(plans.size(), kNoWeight);
+    std::vector<uint64_t> skipped(plans.size(), 0);
+    double ms = 0;
+    qsf_hip_stats round{};
+    if (!oplans.empty()) {
         auto tr = clk::now();
-        const int rc = qsf_hip_enumerate_batch(plans.data(), floors.data(), (int)plans.size(), found.data());
+        const int rc = qsf_hip_enumerate_levels(oplans.data(), ofloors.data(), ogroups.data(), (int)oplans.size(),
+                                                found.data(), skipped.data());
         if (rc != 0) throw EngineError{rc, qsf_hip_last_error()};
-        qsf_hip_stats round{};
         qsf_hip_last_stats(&round);
-        const double ms = std::chrono::duration<double, std::milli>(clk::now() - tr).count();
+        ms = std::chrono::duration<double, std::milli>(clk::now() - tr).count();
         if (I.opt.profile)
             std::fprintf(stderr, "[prof round] %d: %zu levels, %llu codewords, call %.3f ms, device %.3f ms\n", stats.rounds,
-                         plans.size(), (unsigned long long)round.codewords, ms, round.kernel_ms);
+                         oplans.size(), (unsigned long long)round.codewords, ms, round.kernel_ms);
         ++stats.rounds;
         stats.codewords += round.codewords;
         stats.device_ms += round.kernel_ms;
-        for (size_t k = 0; k < plans.size(); ++k) {
+    }
+    // replay each problem's levels in the level loop's order
+    for (size_t k = 0; k < I.live.size(); ++k) {
+        Stepper& S = *I.live[k].st;
+        size_t fed = 0;
+        for (const Enq& q : enq[k]) {
+            if (S.done || !S.cur) break;
+            const u64 cap = S.cap ? S.cap : I.work_cap;
+            const u64 spent = S.A.work + (S.pair ? S.B.work : 0);
+            if (cap && (q.cw > cap || spent > cap - q.cw)) { S.capped = S.done = true; break; }
+            if (S.cur != q.side || S.d != q.d) {
+                // the loop resolved a side or moved on: skip enqueued levels it no longer visits
+                if (q.side == S.cur && q.d < S.d) continue;
+                if (q.side != S.cur) continue;
+                break;
+            }
+            if (q.plan < 0) {   // no generator active at this level
+                S.after(kNoWeight, 0.0, qsf_hip_stats{});
+                S.next();
+                continue;
+            }
+            const int r = pos[q.plan];
+            if (skipped[r] != 0) break;   // run this level again, exactly, next round
             qsf_hip_stats one{};
-            one.codewords = cw[k];
+            one.codewords = q.cw;
             one.kernel = round.kernel;
             one.kernel_ms = round.kernel_ms;
-            who[k]->after(found[k], ms, one);
-            who[k]->next();
+            S.after(found[r], ms, one);
+            S.next();
+            ++fed;
+        }
+        if (I.opt.profile && !enq[k].empty()) {
+            u64 waste = 0;
+            size_t nq = 0;
+            for (const Enq& q : enq[k]) if (q.plan >= 0) ++nq;
+            unsigned long long skip = 0;
+            for (size_t t = fed; t < enq[k].size(); ++t)
+                if (enq[k][t].plan >= 0) {
+                    waste += enq[k][t].cw;
+                    skip += skipped[pos[enq[k][t].plan]];
+                }
+            std::fprintf(stderr, "[prof lookahead] problem %zu: %zu levels enqueued, %zu replayed, %llu codewords not replayed, %llu of their tiles skipped%s\n",
+                         k, nq, fed, (unsigned long long)waste, skip, S.done ? (S.capped ? " (capped)" : " (done)") : "");
+        }
+        // the cap check for the next level the loop wants (as the level loop does before it)
+        if (!S.done && S.cur) {
+            const u64 cap = S.cap ? S.cap : I.work_cap;
+            const u64 w = S.cur->level_work(S.d);
+            const u64 spent = S.A.work + (S.pair ? S.B.work : 0);
+            if (cap && (w > cap || spent > cap - w)) S.capped = S.done = true;
         }
     }
     for (auto& e : I.live) stats.capped += (e.st->done && e.st->capped) ? 1 : 0;

@@ -136,6 +136,11 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 if (a.njobs == 1) break;
                 continue;   // this problem is settled; its remaining tiles are skipped
             }
+            // another level of this problem, in the same launch, settled it
+            if (J->shared != J->best && uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) <= floor) {
+                if (lane == 0) atomicAdd(J->skipped, 1ull);
+                continue;
+            }
         }
         struct {
             u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream;
@@ -317,6 +322,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                     if (!a.collect && wm < bound) {
                         bound = wm;
                         if (lane == 0) atomicMin(J->best, wm);
+                        if (lane == 0 && J->shared != J->best) atomicMin(J->shared, wm);
                     }
                     __syncthreads();
                 }
@@ -433,6 +439,13 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         if (__syncthreads_or(bound <= floor)) {
             if (a.njobs == 1) break;
             continue;
+        }
+        {   // another level of this problem, in the same launch, settled it
+            const int sb = uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (__syncthreads_or(J->shared != J->best && sb <= floor)) {
+                if (tid == 0) atomicAdd(J->skipped, 1ull);
+                continue;
+            }
         }
         const u64 tile_base = uni64(Rr.tile_base), stream_groups = uni64(Rr.stream_groups);
         const u64 stream_chunks = uni64(Rr.stream_chunks), n_lane = uni64(Rr.n_lane), n_stream = uni64(Rr.n_stream);
@@ -648,6 +661,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
                 if (wm < bound) {
                     bound = wm;
                     if (lane == 0) atomicMin(J->best, wm);
+                    if (lane == 0 && J->shared != J->best) atomicMin(J->shared, wm);
                     thrd = min(bound, 127);
                 }
                 if (bound <= floor) break;
@@ -807,6 +821,13 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         if (__syncthreads_or(bound <= floor)) {
             if (a.njobs == 1) break;
             continue;
+        }
+        {   // another level of this problem, in the same launch, settled it
+            const int sb = uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (__syncthreads_or(J->shared != J->best && sb <= floor)) {
+                if (tid == 0) atomicAdd(J->skipped, 1ull);
+                continue;
+            }
         }
         const u64 tile_base = uni64(Rr.tile_base), stream_groups = uni64(Rr.stream_groups);
         const u64 stream_chunks = uni64(Rr.stream_chunks), n_lane = uni64(Rr.n_lane), n_stream = uni64(Rr.n_stream);
@@ -1055,6 +1076,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 if (wm < bound) {
                     bound = wm;
                     if (lane == 0) atomicMin(J->best, wm);
+                    if (lane == 0 && J->shared != J->best) atomicMin(J->shared, wm);
                     thrf = (float)min(bound, 127);
                 }
                 if (bound <= floor) break;
@@ -1329,7 +1351,6 @@ struct Engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
     static constexpr int kVariants = 22;  // kernel variants of a tiled run (vtab in run_tiled)
-    hipEvent_t gev[2 * kVariants] = {};   // start/stop per kernel-variant launch of a tiled run
     u64* gbinom = nullptr;               // device address of tiled_binom (filled at probe)
     unsigned char* scratch_dev = nullptr;   // device address of scratch (no-allocation calls)
     DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
@@ -1361,8 +1382,11 @@ struct Engine {
             probe_msg = "stream/event creation failed";
             return ok = false;
         }
-        for (hipEvent_t& e : gev)
+        for (int k = 0; k < 2 * kVariants; ++k) {   // start/stop events of a tiled run's launch groups
+            hipEvent_t e = nullptr;
             if (hipEventCreate(&e) != hipSuccess) { probe_msg = "event creation failed"; return ok = false; }
+            gev_dyn.push_back(e);
+        }
         try {
             // the library's static device arrays on this device (the engine runs on the
             // device that is current here, and keeps using it)
@@ -1926,13 +1950,23 @@ struct Engine {
     // Many problems' levels at once: every tileable level's tiles share one persistent
     // launch per kernel variant; levels the tiled kernel cannot take run on the row-sweep
     // kernel one after another.
-    int enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out) {
+    // groups (optional): plans with the same group index are levels of one problem and share
+    // a running bound; skipped_out[j]: plan j's tiles skipped because that shared bound met
+    // its floor through another plan (0 when groups is null).
+    int enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n, int* best_out,
+                        const int* groups = nullptr, uint64_t* skipped_out = nullptr) {
         stats = qsf_hip_stats{};
+        if (skipped_out) std::fill(skipped_out, skipped_out + n, (uint64_t)0);
         cache_cap = std::max(cache_cap, (size_t)n + 6);
         std::vector<Solve*> solves = solves_for(plans, n);   // cache_cap > n: none evicted
         std::vector<TItem> items;
         std::vector<SItem> small;
         std::vector<int> item_of(n, -1), small_of(n, -1), sweep;
+        std::vector<int> phase(n, 0);   // a plan's position among its group's plans: its launch phase
+        if (groups) {
+            std::unordered_map<int, int> seen;
+            for (int j = 0; j < n; ++j) phase[j] = seen[groups[j]]++;
+        }
         for (int j = 0; j < n; ++j) {
             const qsf_enum_plan& p = plans[j];
             Solve& s = *solves[j];
@@ -1944,7 +1978,8 @@ struct Engine {
             stats.codewords = sat_add(stats.codewords, cw);
             if (tileable(s, p) && policy != 1) {
                 item_of[j] = (int)items.size();
-                items.push_back(TItem{&s, p, floors ? floors[j] : -1, choose_variant(s, p, true, matrix_min_work() >> 6)});
+                items.push_back(TItem{&s, p, floors ? floors[j] : -1, choose_variant(s, p, true, matrix_min_work() >> 6),
+                                      groups ? groups[j] : -1, phase[j]});
             } else if (small_sweep(p, cw)) {
                 small_of[j] = (int)small.size();
                 small.push_back(SItem{&s, &plans[j]});
@@ -1971,13 +2006,22 @@ struct Engine {
                         it.V = choose_variant(*it.s, it.p, false);
             }
         }
+        static const bool prof_items = std::getenv("QSA_PROFILE_ITEMS") != nullptr;   // diagnostics
+        if (prof_items)
+            for (const TItem& it : items)
+                std::fprintf(stderr, "[prof item] solve %016llx d %d best %d thr %ld cw %llu phase %d variant %d,%d,%d,%d sf %.3f\n",
+                             (unsigned long long)it.s->key, it.p.d, it.p.current_best, level_thr(*it.s, it.p),
+                             (unsigned long long)sat_mul(it.s->C(it.p.K, it.p.d), (u64)it.p.num_gamma), it.phase,
+                             it.V.fd, it.V.xd, it.V.rpl, it.V.mm, it.s->sf);
         double ms_total = 0;
         if (!items.empty()) {
             std::vector<int> res;
             run_tiled(items, false, 0, 0, res);
             ms_total += stats.kernel_ms;
             for (size_t i = 0; i < items.size(); ++i)
-                if (res[i] > items[i].floor) observe(*items[i].s, items[i].p, items[i].V, item_hits[i]);
+                if (res[i] > items[i].floor && item_skipped[i] == 0) observe(*items[i].s, items[i].p, items[i].V, item_hits[i]);
+            if (skipped_out)
+                for (int j = 0; j < n; ++j) skipped_out[j] = item_of[j] >= 0 ? item_skipped[item_of[j]] : 0;
             for (int j = 0; j < n; ++j)
                 if (item_of[j] >= 0) best_out[j] = std::min(res[item_of[j]], plans[j].current_best);
         }
@@ -2305,6 +2349,8 @@ struct Engine {
         qsf_enum_plan p;
         int floor;
         Variant V;
+        int group = -1;   // items (levels) of one problem share a bound slot; -1: alone
+        int phase = 0;    // launch phase: phases run in order, each after the previous one
     };
 
     static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -2319,10 +2365,12 @@ struct Engine {
     }
 
     // One tiled run: the levels of `items` (job i = items[i]) decomposed into rectangles,
-    // grouped by kernel variant. One upload of a block laid out as
-    //   [groups x 4 u64 counters | items x int bounds | items x TJob | rects | fixed rows],
-    // one persistent launch per variant (tiles pulled from the group's counter), one
-    // read-back of counters and bounds. res[i] = the bound job i ended with.
+    // grouped by launch phase and kernel variant. One upload of a block laid out as
+    //   [groups x 256-byte counter lines | items x int bounds | items x u64 passes |
+    //    items x u64 skipped tiles | shared bounds | items x TJob | rects | fixed rows]
+    // (each section from a 256-byte boundary), one persistent launch per group (tiles
+    // pulled from the group's counter), one read-back of counters and bounds.
+    // res[i] = the bound job i ended with.
     // shared_rows (collect mode): one device row counter for every launch of this run and
     // of earlier runs appending to the same output (levels 1..d of one collection);
     // nullptr: each variant counts its own rows from 0.
@@ -2346,37 +2394,59 @@ struct Engine {
             return v.mm == 0 ? 64ull * v.rpl : v.mm == 1 ? (u64)kMfmaLane : (u64)f4_lane_tile(v.mm);
         };
         const int ni = (int)items.size();
+        // Launch groups: the items of one (phase, variant) share a persistent launch; groups
+        // launch phase by phase (the levels a problem may not need come in later phases and
+        // are skipped once an earlier phase settles it), variants in order of first item.
+        std::vector<int> item_grp(ni), grp_v, grp_phase;
+        {
+            std::vector<int> key;   // phase * NV + variant per group
+            std::vector<int> idx(ni);
+            for (int i = 0; i < ni; ++i) idx[i] = i;
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return items[x].phase < items[y].phase; });
+            for (int i : idx) {
+                const int k = items[i].phase * NV + vidx(items[i].V);
+                int g = (int)(std::find(key.begin(), key.end(), k) - key.begin());
+                if (g == (int)key.size()) {
+                    key.push_back(k);
+                    grp_v.push_back(k % NV);
+                    grp_phase.push_back(items[i].phase);
+                }
+                item_grp[i] = g;
+            }
+        }
+        const int NG = (int)grp_v.size();
         // stream chunk length per VALU variant: halved (down to 64) while the launch would
         // have fewer tiles than half the resident waves, so that a small or medium level
         // (one code per call) spreads over the chip instead of a few hundred waves; the
         // [64..160] random codes one per call: 241 -> 187 ms of device time (a target of 4
         // tiles per wave gave 200 ms and slowed the gross d = 7 level by 15%). Tuning
         // overrides: QSA_TILED_S (a fixed length), QSA_TILED_S_TILES (the target).
-        std::vector<int> vS(NV);
-        for (int v = 0; v < NV; ++v) vS[v] = stream_len(vtab[v]);
+        std::vector<int> vS(NG);
+        for (int g = 0; g < NG; ++g) vS[g] = stream_len(vtab[grp_v[g]]);
         {
             const char* env = std::getenv("QSA_TILED_S");
-            std::vector<double> vcw(NV, 0.0);
-            for (const TItem& it : items)
-                vcw[vidx(it.V)] += (double)sat_mul(it.s->C(it.p.K, it.p.d), (u64)it.p.num_gamma);
+            std::vector<double> vcw(NG, 0.0);
+            for (int i = 0; i < ni; ++i)
+                vcw[item_grp[i]] += (double)sat_mul(items[i].s->C(items[i].p.K, items[i].p.d), (u64)items[i].p.num_gamma);
             const double resident = (double)std::max(1, cus) * 16;
             const char* per = std::getenv("QSA_TILED_S_TILES");   // tuning: tiles per resident wave
             const double want = per ? std::atof(per) : 0.5;
-            for (int v = 0; v < NV; ++v) {
-                if (vtab[v].mm != 0) continue;
+            for (int g = 0; g < NG; ++g) {
+                const Variant& V = vtab[grp_v[g]];
+                if (V.mm != 0) continue;
                 if (env) {
-                    vS[v] = std::max(64, std::min(kStreamMax, std::atoi(env)));
+                    vS[g] = std::max(64, std::min(kStreamMax, std::atoi(env)));
                 } else {
                     int S = kStreamMax;
-                    while (S > 64 && vcw[v] / (64.0 * vtab[v].rpl * S) < want * resident) S /= 2;
-                    vS[v] = S;
+                    while (S > 64 && vcw[g] / (64.0 * V.rpl * S) < want * resident) S /= 2;
+                    vS[g] = S;
                 }
             }
         }
-        std::vector<std::vector<Rect>> grect(NV);
+        std::vector<std::vector<Rect>> grect(NG);
         std::vector<u32> hfixed;
         std::vector<size_t> fixed_off(ni, 0);
-        std::vector<int> glds(NV, 0);
+        std::vector<int> glds(NG, 0);
         // each item's decomposition on its own (host threads for large batches), then
         // concatenated in item order
         std::vector<std::vector<Rect>> irect(ni);
@@ -2385,7 +2455,7 @@ struct Engine {
             const TItem& it = items[i];
             const Solve& s = *it.s;
             const int K = it.p.K, d = it.p.d, h = s.h;
-            RectBuilder B{s, 0, s.RW, vS[vidx(it.V)], lane_tile(it.V), it.V.mm > 0, {}, {}};
+            RectBuilder B{s, 0, s.RW, vS[item_grp[i]], lane_tile(it.V), it.V.mm > 0, {}, {}};
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
                 for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
@@ -2420,7 +2490,7 @@ struct Engine {
         for (int i = 0; i < ni; ++i) {
             const TItem& it = items[i];
             const Solve& s = *it.s;
-            const int vi = vidx(it.V);
+            const int vi = item_grp[i];
             grect[vi].insert(grect[vi].end(), irect[i].begin(), irect[i].end());
             fixed_off[i] = hfixed.size();
             hfixed.insert(hfixed.end(), ifixed[i].begin(), ifixed[i].end());
@@ -2431,13 +2501,13 @@ struct Engine {
         // (16 one-wave workgroups per CU for bz_tiled, 4 four-wave ones for bz_mfma)
         const u64 waves = (u64)std::max(1, cus) * 16;
         const u64 mfma_groups = (u64)std::max(1, cus) * 4;
-        std::vector<int> gG(NV, 1);
-        std::vector<u64> gtiles(NV, 0);
+        std::vector<int> gG(NG, 1);
+        std::vector<u64> gtiles(NG, 0);
         size_t nrect = 0;
-        for (int v = 0; v < NV; ++v) {
+        for (int v = 0; v < NG; ++v) {
             u64 chunks = 0;
             for (const Rect& r : grect[v]) chunks += r.lane_blocks * r.stream_chunks;
-            const u64 resident = vtab[v].mm > 0 ? mfma_groups : waves;
+            const u64 resident = vtab[grp_v[v]].mm > 0 ? mfma_groups : waves;
             gG[v] = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * resident)));
             u64 tiles = 0;
             for (Rect& r : grect[v]) {
@@ -2448,9 +2518,20 @@ struct Engine {
             gtiles[v] = tiles;
             nrect += grect[v].size();
         }
-        const size_t o_cnt = 0, o_best = NV * 4 * sizeof(u64);
-        const size_t o_hits = align16(o_best + (size_t)ni * sizeof(int));
-        const size_t o_jobs = align16(o_hits + (size_t)ni * sizeof(u64));
+        int ngroups = 0;
+        for (const TItem& it : items) ngroups = std::max(ngroups, it.group + 1);
+        // The sections that the kernels update with atomics sit on cache lines of their own:
+        // each group's tile counters (every workgroup increments next_tile at every tile),
+        // the bounds (read by every wave at every stream chunk), the pass and skip counts
+        // and the shared bounds. A tile counter on the line of the bounds it is polled beside
+        // slowed the gross d = 10 level from 31.5 to 38.8 ms.
+        constexpr size_t kLine = 256, kCntStride = kLine / sizeof(u64);
+        auto line_up = [](size_t x) { return (x + kLine - 1) & ~(kLine - 1); };
+        const size_t o_cnt = 0, o_best = (size_t)NG * kLine;
+        const size_t o_hits = line_up(o_best + (size_t)ni * sizeof(int));
+        const size_t o_skip = line_up(o_hits + (size_t)ni * sizeof(u64));
+        const size_t o_shared = line_up(o_skip + (size_t)ni * sizeof(u64));
+        const size_t o_jobs = line_up(o_shared + (size_t)ngroups * sizeof(int));
         const size_t o_rects = align16(o_jobs + (size_t)ni * sizeof(TJob));
         const size_t o_fixed = align16(o_rects + nrect * sizeof(Rect));
         const size_t total = align16(o_fixed + hfixed.size() * sizeof(u32) + 4);
@@ -2458,6 +2539,10 @@ struct Engine {
         stage_block(total, host, dev);
         std::memset(host, 0, o_jobs);
         int* hb = reinterpret_cast<int*>(host + o_best);
+        int* hs = reinterpret_cast<int*>(host + o_shared);
+        for (int g = 0; g < ngroups; ++g) hs[g] = QSF_WEIGHT_NONE;
+        for (const TItem& it : items)
+            if (it.group >= 0) hs[it.group] = std::min(hs[it.group], it.p.current_best);
         TJob* hj = reinterpret_cast<TJob*>(host + o_jobs);
         for (int i = 0; i < ni; ++i) {
             const TItem& it = items[i];
@@ -2470,14 +2555,16 @@ struct Engine {
             J.gamma = s.gamma.p;
             J.best = reinterpret_cast<int*>(dev + o_best) + i;
             J.hits = reinterpret_cast<unsigned long long*>(dev + o_hits) + i;
+            J.skipped = reinterpret_cast<unsigned long long*>(dev + o_skip) + i;
+            J.shared = it.group >= 0 ? reinterpret_cast<int*>(dev + o_shared) + it.group : J.best;
             J.K = it.p.K; J.RD = s.RD; J.SD = s.SD; J.RW = s.RW; J.stride = it.p.stride;
             J.floor = it.floor;
             hj[i] = J;
         }
-        std::vector<size_t> grect_off(NV, 0);
+        std::vector<size_t> grect_off(NG, 0);
         {
             size_t at = o_rects;
-            for (int v = 0; v < NV; ++v) {
+            for (int v = 0; v < NG; ++v) {
                 grect_off[v] = at;
                 if (!grect[v].empty()) std::memcpy(host + at, grect[v].data(), grect[v].size() * sizeof(Rect));
                 at += grect[v].size() * sizeof(Rect);
@@ -2489,33 +2576,39 @@ struct Engine {
         auto* cnt = reinterpret_cast<unsigned long long*>(dev + o_cnt);
         stats.kernel = 2;
         stats.tiles = 0;
-        int used[NV] = {0};
-        for (int v = 0; v < NV; ++v) {
-            if (gtiles[v] == 0) continue;
-            used[v] = 1;
+        std::vector<int> used(NG, 0);
+        while (gev_dyn.size() < (size_t)(2 * NG)) {   // start/stop events per launch group
+            hipEvent_t e = nullptr;
+            HIPCHK(hipEventCreate(&e));
+            gev_dyn.push_back(e);
+        }
+        for (int gi = 0; gi < NG; ++gi) {   // groups are numbered in launch order
+            const int v = grp_v[gi];
+            if (gtiles[gi] == 0) continue;
+            used[gi] = 1;
             const Variant& V = vtab[v];
             TiledArgs a{};
             a.jobs = reinterpret_cast<const TJob*>(dev + o_jobs);
             a.njobs = ni;
-            a.rects = reinterpret_cast<const Rect*>(dev + grect_off[v]);
-            a.nrects = (int)grect[v].size();
-            a.total_tiles = gtiles[v];
+            a.rects = reinterpret_cast<const Rect*>(dev + grect_off[gi]);
+            a.nrects = (int)grect[gi].size();
+            a.total_tiles = gtiles[gi];
             a.binom = gbinom;
             a.bk = kMaxD + 1;
-            a.S = vS[v];
-            a.G = gG[v];
-            a.lds_full = glds[v];
-            a.next_tile = cnt + 4 * v;
-            a.slow_hits = cnt + 4 * v + 1;
+            a.S = vS[gi];
+            a.G = gG[gi];
+            a.lds_full = glds[gi];
+            a.next_tile = cnt + kCntStride * gi;
+            a.slow_hits = cnt + kCntStride * gi + 1;
             a.collect = collect ? 1 : 0;
             a.keep = keep;
-            a.out_rows = shared_rows ? shared_rows : cnt + 4 * v + 3;
+            a.out_rows = shared_rows ? shared_rows : cnt + kCntStride * gi + 3;
             a.out = out.p;
             a.out_cap = cap;
-            const u64 grid = std::min<u64>(gtiles[v], V.mm > 0 ? mfma_groups : waves);
+            const u64 grid = std::min<u64>(gtiles[gi], V.mm > 0 ? mfma_groups : waves);
             const dim3 gd((unsigned)grid), bd(V.mm > 0 ? 64 * kMfmaWaves : 64);
-            const size_t lds = (size_t)glds[v] * sizeof(u32);
-            HIPCHK(hipEventRecord(gev[2 * v], stream));
+            const size_t lds = (size_t)glds[gi] * sizeof(u32);
+            HIPCHK(hipEventRecord(gev_dyn[2 * gi], stream));
             switch (v) {
                 case 0: hipLaunchKernelGGL((bz_tiled<1, 1, 32, 1>), gd, bd, lds, stream, a); break;
                 case 1: hipLaunchKernelGGL((bz_tiled<1, 2, 16, 1>), gd, bd, lds, stream, a); break;
@@ -2541,8 +2634,8 @@ struct Engine {
                 default: hipLaunchKernelGGL((bz_tiled<4, 4, 12, 2>), gd, bd, lds, stream, a); break;
             }
             HIPCHK(hipGetLastError());
-            HIPCHK(hipEventRecord(gev[2 * v + 1], stream));
-            stats.tiles = (int)std::min<u64>((u64)stats.tiles + gtiles[v], 0x7fffffff);
+            HIPCHK(hipEventRecord(gev_dyn[2 * gi + 1], stream));
+            stats.tiles = (int)std::min<u64>((u64)stats.tiles + gtiles[gi], 0x7fffffff);
             stats.fd = V.fd;
             stats.xd = V.xd;
             stats.rpl = V.rpl;
@@ -2557,22 +2650,33 @@ struct Engine {
         res.assign(hb, hb + ni);
         const unsigned long long* hh = reinterpret_cast<const unsigned long long*>(host + o_hits);
         item_hits.assign(hh, hh + ni);
+        const unsigned long long* hk = reinterpret_cast<const unsigned long long*>(host + o_skip);
+        item_skipped.assign(hk, hk + ni);
         const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(host + o_cnt);
         double ms_total = 0;
         stats.slow_hits = stats.medium_hits = 0;
         out_rows_last = 0;
-        for (int v = 0; v < NV; ++v) {
-            if (!used[v]) continue;
+        static const bool prof_groups = std::getenv("QSA_PROFILE_GROUPS") != nullptr;   // diagnostics
+        for (int gi = 0; gi < NG; ++gi) {
+            if (!used[gi]) continue;
             float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, gev[2 * v], gev[2 * v + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, gev_dyn[2 * gi], gev_dyn[2 * gi + 1]));
             ms_total += ms;
-            stats.slow_hits += hc[4 * v + 1];
-            stats.medium_hits += hc[4 * v + 2];
-            if (vtab[v].mm >= 2) {   // the fp4 kernels count launches that met another output layout here
-                if (hc[4 * v + 3] != 0)
+            if (prof_groups) {
+                u64 cw = 0;
+                int nitems = 0;
+                for (int i = 0; i < ni; ++i)
+                    if (item_grp[i] == gi) { cw += sat_mul(items[i].s->C(items[i].p.K, items[i].p.d), (u64)items[i].p.num_gamma); ++nitems; }
+                std::fprintf(stderr, "[prof group] phase %d variant %d: %d levels, %llu codewords, %llu tiles, S %d G %d, %.3f ms\n",
+                             grp_phase[gi], grp_v[gi], nitems, (unsigned long long)cw, (unsigned long long)gtiles[gi], vS[gi], gG[gi], ms);
+            }
+            stats.slow_hits += hc[kCntStride * gi + 1];
+            stats.medium_hits += hc[kCntStride * gi + 2];
+            if (vtab[grp_v[gi]].mm >= 2) {   // the fp4 kernels count launches that met another output layout here
+                if (hc[kCntStride * gi + 3] != 0)
                     throw HipError{3, "fp4 matrix products: output layout differs from the one the hot test assumes"};
             } else {
-                out_rows_last += hc[4 * v + 3];
+                out_rows_last += hc[kCntStride * gi + 3];
             }
         }
         stats.kernel_ms = ms_total;
@@ -2580,6 +2684,8 @@ struct Engine {
     unsigned long long out_rows_last = 0;   // collect mode: rows the last tiled run produced
     DevBuf<u64> sorted;                      // collect_upto: the deduplicated rows
     std::vector<u64> item_hits;             // the last tiled run's hot-test passes per item
+    std::vector<u64> item_skipped;          // its tiles skipped through a shared bound, per item
+    std::vector<hipEvent_t> gev_dyn;        // start/stop events of a tiled run's launch groups
 
     void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap, int* bp,
                       unsigned long long* cntp) {
@@ -2849,6 +2955,29 @@ int qsf_hip_enumerate_batch(const qsf_enum_plan* plans, const int* floors, int n
     }
     if (n == 0) return 0;
     return run_guarded("qsf_hip_enumerate_batch", [&](qsa::Engine& e) { return e.enumerate_batch(plans, floors, n, best_out); });
+}
+
+int qsf_hip_enumerate_levels(const qsf_enum_plan* plans, const int* floors, const int* groups, int n, int* best_out,
+                             uint64_t* skipped_out) {
+    if (n < 0 || (n > 0 && (!plans || !best_out || !groups))) {
+        engine().err = "qsf_hip_enumerate_levels: bad arguments";
+        return 1;
+    }
+    for (int j = 0; j < n; ++j) {
+        std::string why;
+        if (!qsa::plan_ok(plans + j, why)) {
+            engine().err = "qsf_hip_enumerate_levels: plan " + std::to_string(j) + ": " + why;
+            return 1;
+        }
+        if (groups[j] < 0 || groups[j] >= n) {
+            engine().err = "qsf_hip_enumerate_levels: group index out of range";
+            return 1;
+        }
+    }
+    if (n == 0) return 0;
+    return run_guarded("qsf_hip_enumerate_levels", [&](qsa::Engine& e) {
+        return e.enumerate_batch(plans, floors, n, best_out, groups, skipped_out);
+    });
 }
 
 int qsf_hip_enumerate(const qsf_enum_plan* plan, int* best_out) {

@@ -78,6 +78,10 @@ struct TJob {
     const u32* fixed;        // [f][RW + 1]: residual+syndrome of a fixed row set, then its flagged count
     const u64* gamma;        // [g][row][stride] full rows (collect mode)
     int* best;               // running bound; starts at current_best
+    // The bound shared by the levels of one problem run in the same launch (== best when
+    // the job is alone): a tile is skipped once it is <= floor, and counted in `skipped`.
+    int* shared;
+    unsigned long long* skipped;
     unsigned long long* hits;   // hot-test passes (medium-test entries) on this job, for the tuner
     int K, RD, SD, RW, stride;
     int floor;               // stop once best <= floor (floor < 0: never)
