@@ -210,7 +210,10 @@ def batched_learned(log, c4_cap=int(1e11)):
 
     c3 = codes.config3_batch(0, 256)
     out["batched_c3"] = compare("batched", c3, int(1e10), 8)
-    out["learned_c4"] = compare("learned", codes.config4_eval(0, 16, 20, 40), int(c4_cap), 16)
+    c4 = codes.config4_eval(0, 16, 20, 40)
+    out["learned_c4"] = compare("learned", c4, int(c4_cap), 16)
+    if scorer is not None:
+        out["learned_c4"]["proposal_ceiling"] = proposal_ceiling(c4, scorer, 16)
     out["elimination"] = elimination(c3)
     reqs = serving.make_requests(96, seed=0)
     rate, stats = serving.calibrate_rate(reqs, 0.7, rounds=3)   # wall clock
@@ -218,6 +221,36 @@ def batched_learned(log, c4_cap=int(1e11)):
     log(f"serving: {json.dumps(stats)}")
     out["serving_virtual_clock"] = serving.serve(reqs, rate, live=False)   # same rate, reproducible clock
     return out
+
+
+def proposal_ceiling(Hs, scorer, cands, optimum_orders=256, seed=5):
+    """Predicted BZ work (set ranks from the device peel, device upper bound) of the natural
+    peel, of the best of `cands` orders sampled from the trained scores, of the best of
+    `cands` uniform orders, and of the best of `optimum_orders` uniform orders (the
+    empirical optimum): geometric-mean ratios over the codes. When uniform proposals
+    already reach the optimum, no scorer can gain over them."""
+    import numpy as np
+    from qubitserf_amd import distfind as df
+    from qubitserf_amd.proposal.fit import family_work, sample_orders
+
+    ub = [r.distance for r in df.batch_distance([("classical", H) for H in Hs], work_cap=1)]
+    rng = np.random.default_rng(seed)
+    orders = []
+    for H, s in zip(Hs, scorer.scores(Hs)):
+        n = H.shape[1]
+        orders.append([np.arange(n, dtype=np.int32)] + sample_orders(s, cands, rng)
+                      + sample_orders(np.zeros(n, np.float32), optimum_orders, rng))
+    work = family_work(Hs, orders, ub)
+    nat = np.array([w[0] for w in work])
+    lrn = np.array([min([w[0]] + w[1:1 + cands]) for w in work])
+    uni = np.array([min([w[0]] + w[1 + cands:1 + 2 * cands]) for w in work])
+    opt = np.array([min(w) for w in work])
+    gm = lambda xs: float(np.exp(np.mean(np.log(xs))))
+    return {"candidates": cands, "optimum_orders": optimum_orders,
+            "gain_over_natural_learned": gm(nat / lrn), "gain_over_natural_uniform": gm(nat / uni),
+            "learned_over_optimum": gm(lrn / opt), "uniform_over_optimum": gm(uni / opt),
+            "codes_with_deficient_natural_peel": int((nat > opt).sum()),
+            "codes_uniform_at_optimum": int((uni == opt).sum()), "codes": len(Hs)}
 
 
 def collect_figure(p=8, keep=12, reps=3):

@@ -137,9 +137,17 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 continue;   // this problem is settled; its remaining tiles are skipped
             }
             // another level of this problem, in the same launch, settled it
-            if (J->shared != J->best && uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) <= floor) {
-                if (lane == 0) atomicAdd(J->skipped, 1ull);
-                continue;
+            if (J->shared != J->best) {
+                const int sb = uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                if (sb <= floor) {
+                    if (lane == 0) atomicAdd(J->skipped, 1ull);
+                    continue;
+                }
+                // The level loop's bound here is min(bound, shared): the levels before this
+                // one (earlier launch phases) are complete, and in this phase only this level
+                // moves the shared bound. Pruning on it keeps every level's result the one
+                // the level loop would get.
+                bound = min(bound, sb);
             }
         }
         struct {
@@ -446,6 +454,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
                 if (tid == 0) atomicAdd(J->skipped, 1ull);
                 continue;
             }
+            if (J->shared != J->best) bound = min(bound, sb);   // the level loop's bound (as bz_tiled)
         }
         const u64 tile_base = uni64(Rr.tile_base), stream_groups = uni64(Rr.stream_groups);
         const u64 stream_chunks = uni64(Rr.stream_chunks), n_lane = uni64(Rr.n_lane), n_stream = uni64(Rr.n_stream);
@@ -524,6 +533,8 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         for (int k = 0; k < nch; ++k) {
             if (k != 0) {
                 bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                if (J->shared != J->best)
+                    bound = min(bound, uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
                 if (__syncthreads_or(bound <= floor)) break;   // also: the previous chunk is consumed
             }
 #pragma unroll
@@ -828,6 +839,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 if (tid == 0) atomicAdd(J->skipped, 1ull);
                 continue;
             }
+            if (J->shared != J->best) bound = min(bound, sb);   // the level loop's bound (as bz_tiled)
         }
         const u64 tile_base = uni64(Rr.tile_base), stream_groups = uni64(Rr.stream_groups);
         const u64 stream_chunks = uni64(Rr.stream_chunks), n_lane = uni64(Rr.n_lane), n_stream = uni64(Rr.n_stream);
@@ -919,6 +931,8 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         for (int k = 0; k < nch; ++k) {
             if (k != 0) {
                 bound = min(bound, uni(__hip_atomic_load(J->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                if (J->shared != J->best)
+                    bound = min(bound, uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
                 if (__syncthreads_or(bound <= floor)) break;   // also: the previous chunk is consumed
             }
 #pragma unroll
@@ -2667,8 +2681,9 @@ struct Engine {
                 int nitems = 0;
                 for (int i = 0; i < ni; ++i)
                     if (item_grp[i] == gi) { cw += sat_mul(items[i].s->C(items[i].p.K, items[i].p.d), (u64)items[i].p.num_gamma); ++nitems; }
-                std::fprintf(stderr, "[prof group] phase %d variant %d: %d levels, %llu codewords, %llu tiles, S %d G %d, %.3f ms\n",
-                             grp_phase[gi], grp_v[gi], nitems, (unsigned long long)cw, (unsigned long long)gtiles[gi], vS[gi], gG[gi], ms);
+                std::fprintf(stderr, "[prof group] phase %d variant %d: %d levels, %llu codewords, %llu tiles, S %d G %d, %.3f ms, exact %llu medium %llu\n",
+                             grp_phase[gi], grp_v[gi], nitems, (unsigned long long)cw, (unsigned long long)gtiles[gi], vS[gi], gG[gi], ms,
+                             hc[kCntStride * gi + 1], hc[kCntStride * gi + 2]);
             }
             stats.slow_hits += hc[kCntStride * gi + 1];
             stats.medium_hits += hc[kCntStride * gi + 2];
