@@ -10,7 +10,9 @@ summed over levels and sides), with the time-to-certify alongside.
 Prints ONE JSON line. Fields beyond the common contract:
   time_to_certify_s   mean wall time of one solve
   roofline            of the enumeration kernel that holds most of the device time, with
-                      every kernel's figures under "kernels". bz_mfma (int8 matrix-core hot
+                      every kernel's figures under "kernels" (a matrix-test level's products
+                      with a list too short for the matrix tiles run on the VALU kernel and
+                      count there). bz_mfma (int8 matrix-core hot
                       test): achieved = 64 int8 ops (32 multiply-adds) x its codewords / its
                       device time, peak = 5.0 POP/s dense int8. bz_f4 (fp4 products,
                       32x32x64 / 16x16x128): 128 / 256 ops per codeword, peak 10.1 POP/s
@@ -352,14 +354,20 @@ def main():
                 tiled_cw += t["codewords"]
                 tiled_ms += t["kernel_ms"]
                 tiled_ops += t["ops_per_codeword"] * t["codewords"]
+                parts = []
                 if t["mfma"]:      # matrix-core hot test: its products are the work
                     name, ops, _ = MATRIX[t["mfma"]]
+                    cw, ms = t["codewords"] - t["side_codewords"], t["kernel_ms"] - t["side_ms"]
+                    parts.append((name, cw, ms, ops))
+                    if t["side_codewords"]:   # its thin products, on the VALU kernel
+                        parts.append(("bz_tiled", t["side_codewords"], t["side_ms"], t["side_ops_per_codeword"]))
                 else:              # VALU hot test: the variant's hot-loop lane-ops
-                    name, ops = "bz_tiled", t["ops_per_codeword"]
-                k = per.setdefault(name, [0, 0.0, 0.0])
-                k[0] += t["codewords"]
-                k[1] += t["kernel_ms"]
-                k[2] += ops * t["codewords"]
+                    parts.append(("bz_tiled", t["codewords"], t["kernel_ms"], t["ops_per_codeword"]))
+                for name, cw, ms, ops in parts:
+                    k = per.setdefault(name, [0, 0.0, 0.0])
+                    k[0] += cw
+                    k[1] += ms
+                    k[2] += ops * cw
     total_s = sum(walls)
     value = codewords / total_s
     achieved = tiled_ops / (tiled_ms * 1e-3) / 1e12 if tiled_ms > 0 else 0.0

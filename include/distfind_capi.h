@@ -99,6 +99,9 @@ typedef struct {
     double ms;             /* wall time of the level */
     double kernel_ms;      /* device time of the enumeration kernel (HIP events) */
     double ops_per_codeword; /* hot-loop int32 VALU lane-ops per codeword of the variant */
+    double side_ms;        /* of kernel_ms: the VALU launches beside a matrix test (its thin products) */
+    uint64_t side_codewords; /* of codewords: those thin products' */
+    double side_ops_per_codeword; /* hot-loop int32 VALU lane-ops per codeword of those launches */
 } DistfindLevel;
 
 /* Copies up to `cap` levels into `out` (may be NULL); returns the number recorded. */

@@ -73,6 +73,9 @@ class HipStats(ctypes.Structure):
         ("rpl", ctypes.c_int),
         ("ops_per_codeword", ctypes.c_double),
         ("mfma", ctypes.c_int),
+        ("side_ms", ctypes.c_double),
+        ("side_codewords", ctypes.c_uint64),
+        ("side_ops_per_codeword", ctypes.c_double),
     ]
 
 
@@ -122,7 +125,8 @@ class DistfindLevel(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("side", "d", "active", "ngamma", "K", "inner", "outer", "kernel",
                                               "fd", "xd", "rpl", "mfma")] + [
         ("codewords", ctypes.c_uint64), ("medium_hits", ctypes.c_uint64), ("slow_hits", ctypes.c_uint64),
-        ("ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("ops_per_codeword", ctypes.c_double)]
+        ("ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("ops_per_codeword", ctypes.c_double),
+        ("side_ms", ctypes.c_double), ("side_codewords", ctypes.c_uint64), ("side_ops_per_codeword", ctypes.c_double)]
 
 
 _lib = None

@@ -233,6 +233,8 @@ int distfind_last_trace(DistfindLevel* out, int cap) {
         o.fd = t.st.fd; o.xd = t.st.xd; o.rpl = t.st.rpl; o.mfma = t.st.mfma;
         o.codewords = t.st.codewords; o.medium_hits = t.st.medium_hits; o.slow_hits = t.st.slow_hits;
         o.ms = t.ms; o.kernel_ms = t.st.kernel_ms; o.ops_per_codeword = t.st.ops_per_codeword;
+        o.side_ms = t.st.side_ms; o.side_codewords = t.st.side_codewords;
+        o.side_ops_per_codeword = t.st.side_ops_per_codeword;
     }
     return n;
 }

@@ -96,8 +96,12 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
     __shared__ u32 sfull[kMaxRW];
     __shared__ Mask run_start[64];           // colex mask of the first stream element of each lane's run
     extern __shared__ u32 rows_full[];       // whole residual+syndrome table when it fits (dynamic size)
+    __shared__ u32 sbin[kSmallBinRows * kSmallBinCols];   // the unrank's binomials C(n < 64, k <= 8)
 
     const int lane = threadIdx.x;
+    for (int i = lane; i < kSmallBinRows * kSmallBinCols; i += 64)
+        sbin[i] = (u32)a.binom[(i / kSmallBinCols) * a.bk + i % kSmallBinCols];
+    __syncthreads();
     int cur_job = -1, cur_g = -1;
     int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
     const TJob* J = nullptr;
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         u32 xw[RPL][XD];
         int xoff[RPL];
         const u64 b0 = bx * (u64)(64 * RPL) + (u64)lane * RPL;
-        const Mask m0 = (b0 < R.n_lane) ? colex_unrank<Mask>(b0, lk, lm, a.binom, a.bk) : Mask{};
+        const Mask m0 = (b0 < R.n_lane) ? colex_unrank_any<Mask>(b0, lk, lm, sbin, a.binom, a.bk) : Mask{};
         {
             Mask m = m0;
 #pragma unroll
@@ -221,7 +225,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 const int i0 = lane * per;
                 const int i1 = min(len, i0 + per);
                 if (i0 < i1) {
-                    Mask m = colex_unrank<Mask>(c0 + (u64)i0, sk, sm, a.binom, a.bk);
+                    Mask m = colex_unrank_any<Mask>(c0 + (u64)i0, sk, sm, sbin, a.binom, a.bk);
                     run_start[lane] = m;
                     for (int i = i0; i < i1; ++i) {
                         SElem<XD> e;
@@ -398,6 +402,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
     __shared__ u64 lmk[kMfmaWaves * 256];                     // each wave's lane-element subsets
     __shared__ unsigned short lay[64 * 16];                   // product layout: (row, column) of lane l, value v
     __shared__ unsigned long long s_tile;
+    __shared__ u32 sbin[kSmallBinRows * kSmallBinCols];        // the unrank's binomials C(n < 64, k <= 8)
     extern __shared__ u32 rows_full[];
 
     const int tid = threadIdx.x;
@@ -415,6 +420,9 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) lay[lane * 16 + v] = (unsigned short)((pr[v] & 31) | ((pc[v] & 31) << 8));
     }
+    for (int i = tid; i < kSmallBinRows * kSmallBinCols; i += 64 * kMfmaWaves)
+        sbin[i] = (u32)a.binom[(i / kSmallBinCols) * a.bk + i % kSmallBinCols];
+    // (visible to every wave after the loop's first barrier)
     int cur_job = -1, cur_g = -1;
     int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
     const TJob* J = nullptr;
@@ -487,7 +495,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         u64* wlmk = lmk + 256 * wave;
         {
             const u64 first = bx * (u64)kMfmaLane + (u64)(256 * wave + kMfmaBlocks * row);
-            u64 mnext = (first < n_lane && lk) ? colex_mask(first, lk, lm, a.binom, a.bk) : 0ull;
+            u64 mnext = (first < n_lane && lk) ? colex_unrank_any<u64>(first, lk, lm, sbin, a.binom, a.bk) : 0ull;
 #pragma unroll
             for (int b = 0; b < kMfmaBlocks; ++b) {
                 const u64 idx = first + (u64)b;
@@ -526,7 +534,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         // padding (products >= 127), so the tile's groups past it are not tested
         const u64 t_real = min((u64)(64 * kMfmaWaves), (n_stream - s0 + (u64)(2 * nch) - 1) / (u64)(2 * nch));
         const int gmax = uni((int)min((u64)(kMfmaStream / 32), (2 * t_real + 31) / 32));
-        u64 smask = (sidx < n_stream && sk) ? colex_mask(sidx, sk, sm, a.binom, a.bk) : 0ull;
+        u64 smask = (sidx < n_stream && sk) ? colex_unrank_any<u64>(sidx, sk, sm, sbin, a.binom, a.bk) : 0ull;
         u32 sx0 = fw0;
         for (u64 mm = smask; mm; mm &= mm - 1) sx0 ^= rows_fast[(sbase + __builtin_ctzll(mm)) * XD];
 
@@ -771,6 +779,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     __shared__ unsigned short lay[64 * NV];       // product layout: (row, column) of lane l, value v
     __shared__ int s_bad;
     __shared__ unsigned long long s_tile;
+    __shared__ u32 sbin[kSmallBinRows * kSmallBinCols];        // the unrank's binomials C(n < 64, k <= 8)
     extern __shared__ u32 rows_full[];
 
     const int tid = threadIdx.x;
@@ -800,6 +809,9 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         return;
     }
 
+    for (int i = tid; i < kSmallBinRows * kSmallBinCols; i += 64 * kMfmaWaves)
+        sbin[i] = (u32)a.binom[(i / kSmallBinCols) * a.bk + i % kSmallBinCols];
+    // (visible to every wave after the loop's first barrier)
     int cur_job = -1, cur_g = -1;
     int K = 0, RW = 0, RD = 0, SD = 0, floor = -1, bound = 0;
     const TJob* J = nullptr;
@@ -871,7 +883,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         u64* wlmk = lmk + LW * wave;
         {
             const u64 first = bx * (u64)(kMfmaWaves * LW) + (u64)(LW * wave + kMfmaBlocks * row);
-            u64 mnext = (first < n_lane && lk) ? colex_mask(first, lk, lm, a.binom, a.bk) : 0ull;
+            u64 mnext = (first < n_lane && lk) ? colex_unrank_any<u64>(first, lk, lm, sbin, a.binom, a.bk) : 0ull;
 #pragma unroll
             for (int b = 0; b < kMfmaBlocks; ++b) {
                 const u64 idx = first + (u64)b;
@@ -918,7 +930,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         // padding, so the tile's groups past it are not tested
         const u64 t_real = min((u64)(64 * kMfmaWaves), (n_stream - s0 + (u64)(EPT * nch) - 1) / (u64)(EPT * nch));
         const int gmax = uni((int)min((u64)(SL / NR), (EPT * t_real + NR - 1) / NR));
-        u64 smask = (sidx < n_stream && sk) ? colex_mask(sidx, sk, sm, a.binom, a.bk) : 0ull;
+        u64 smask = (sidx < n_stream && sk) ? colex_unrank_any<u64>(sidx, sk, sm, sbin, a.binom, a.bk) : 0ull;
         u32 sx[NKB];
 #pragma unroll
         for (int w = 0; w < NKB; ++w) sx[w] = (fx && w < RD) ? fx[w] : 0u;
@@ -2411,23 +2423,43 @@ struct Engine {
         // Launch groups: the items of one (phase, variant) share a persistent launch; groups
         // launch phase by phase (the levels a problem may not need come in later phases and
         // are skipped once an earlier phase settles it), variants in order of first item.
-        std::vector<int> item_grp(ni), grp_v, grp_phase;
+        // A matrix-test level's thin products (a list shorter than thin_below, which would
+        // leave the 1024-lane x 512-element tiles nearly empty) run on the level's VALU
+        // variant instead, in a second group of the same phase: same job, same bound.
+        std::vector<int> item_grp(ni), item_grp2(ni, -1), grp_v, grp_phase, grp_side;
+        // (tuning override QSA_THIN_BELOW, 0: off; a variant forced by QSA_TILED_VARIANT
+        // runs the whole level)
+        const char* thin_env = std::getenv("QSA_THIN_BELOW");
+        const u64 thin_below = std::getenv("QSA_TILED_VARIANT") ? 0
+                               : thin_env ? (u64)std::strtoull(thin_env, nullptr, 10) : (u64)256;
         {
-            std::vector<int> key;   // phase * NV + variant per group
+            std::vector<int> key;   // (phase * NV + variant) * 2 + side per group
+            auto group_of = [&](int phase, int v, int side) {   // side: thin products only
+                const int k = (phase * NV + v) * 2 + side;
+                int g = (int)(std::find(key.begin(), key.end(), k) - key.begin());
+                if (g == (int)key.size()) {
+                    key.push_back(k);
+                    grp_v.push_back(v);
+                    grp_phase.push_back(phase);
+                    grp_side.push_back(side);
+                }
+                return g;
+            };
             std::vector<int> idx(ni);
             for (int i = 0; i < ni; ++i) idx[i] = i;
             std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return items[x].phase < items[y].phase; });
             for (int i : idx) {
-                const int k = items[i].phase * NV + vidx(items[i].V);
-                int g = (int)(std::find(key.begin(), key.end(), k) - key.begin());
-                if (g == (int)key.size()) {
-                    key.push_back(k);
-                    grp_v.push_back(k % NV);
-                    grp_phase.push_back(items[i].phase);
-                }
-                item_grp[i] = g;
+                item_grp[i] = group_of(items[i].phase, vidx(items[i].V), 0);
+                if (!collect && thin_below > 0 && items[i].V.mm > 0)
+                    item_grp2[i] = group_of(items[i].phase, vidx(choose_variant(*items[i].s, items[i].p, false)), 1);
             }
         }
+        // an item with a second group gets a second job entry: same bound, table and skip
+        // count, its own pass counter (the tuner reads the first group's passes only)
+        std::vector<int> job2(ni, -1);
+        int nj = ni;
+        for (int i = 0; i < ni; ++i)
+            if (item_grp2[i] >= 0) job2[i] = nj++;
         const int NG = (int)grp_v.size();
         // stream chunk length per VALU variant: halved (down to 64) while the launch would
         // have fewer tiles than half the resident waves, so that a small or medium level
@@ -2448,6 +2480,10 @@ struct Engine {
             for (int g = 0; g < NG; ++g) {
                 const Variant& V = vtab[grp_v[g]];
                 if (V.mm != 0) continue;
+                if (grp_side[g]) {   // thin products: each short list in one chunk
+                    vS[g] = (int)std::max<u64>(64, std::min<u64>(kStreamMax, thin_below));
+                    continue;
+                }
                 if (env) {
                     vS[g] = std::max(64, std::min(kStreamMax, std::atoi(env)));
                 } else {
@@ -2463,19 +2499,40 @@ struct Engine {
         std::vector<int> glds(NG, 0);
         // each item's decomposition on its own (host threads for large batches), then
         // concatenated in item order
-        std::vector<std::vector<Rect>> irect(ni);
+        std::vector<std::vector<Rect>> irect(ni), irect2(ni);
         std::vector<std::vector<u32>> ifixed(ni);
         auto decompose = [&](int i) {
             const TItem& it = items[i];
             const Solve& s = *it.s;
             const int K = it.p.K, d = it.p.d, h = s.h;
-            RectBuilder B{s, 0, s.RW, vS[item_grp[i]], lane_tile(it.V), it.V.mm > 0, {}, {}};
+            RectBuilder B{s, 0, s.RW, vS[item_grp[i]], lane_tile(it.V), it.V.mm > 0};
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
                 for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
                     B.split(RectBuilder::RowSet{}, 0, h, ka, h, K - h, d - ka, 0);
             }
             for (Rect& r : B.rv) r.job = i;
+            if (item_grp2[i] >= 0) {   // thin products: VALU geometry, longer list on the lanes
+                const u64 lt = lane_tile(vtab[grp_v[item_grp2[i]]]);
+                const u64 S2 = (u64)vS[item_grp2[i]];
+                std::vector<Rect> fat;
+                for (Rect& r : B.rv) {
+                    if (std::min(r.n_lane, r.n_stream) >= thin_below) {
+                        fat.push_back(r);
+                        continue;
+                    }
+                    Rect v = r;
+                    if (v.n_lane < v.n_stream) {
+                        std::swap(v.lbase, v.sbase); std::swap(v.lm, v.sm); std::swap(v.lk, v.sk);
+                        std::swap(v.n_lane, v.n_stream);
+                    }
+                    v.lane_blocks = (v.n_lane + lt - 1) / lt;
+                    v.stream_chunks = (v.n_stream + S2 - 1) / S2;
+                    v.job = job2[i];
+                    irect2[i].push_back(v);
+                }
+                B.rv.swap(fat);
+            }
             irect[i] = std::move(B.rv);
             ifixed[i] = std::move(B.fixed);
         };
@@ -2509,7 +2566,14 @@ struct Engine {
             fixed_off[i] = hfixed.size();
             hfixed.insert(hfixed.end(), ifixed[i].begin(), ifixed[i].end());
             if (it.p.K * s.RW <= kLdsFullRows) glds[vi] = std::max(glds[vi], it.p.K * s.RW);
+            if (!irect2[i].empty()) {
+                const int v2 = item_grp2[i];
+                grect[v2].insert(grect[v2].end(), irect2[i].begin(), irect2[i].end());
+                if (it.p.K * s.RW <= kLdsFullRows) glds[v2] = std::max(glds[v2], it.p.K * s.RW);
+            }
         }
+        item_thin.assign(ni, 0);
+        for (int i = 0; i < ni; ++i) item_thin[i] = irect2[i].empty() ? 0 : 1;
         // tiles: a lane block x G stream chunks; G large enough to amortise the register
         // tile, small enough to leave >= 8 tiles per resident workgroup for load balance
         // (16 one-wave workgroups per CU for bz_tiled, 4 four-wave ones for bz_mfma)
@@ -2532,6 +2596,24 @@ struct Engine {
             gtiles[v] = tiles;
             nrect += grect[v].size();
         }
+        static const bool prof_rects = std::getenv("QSA_PROFILE_RECTS") != nullptr;   // diagnostics
+        if (prof_rects)
+            for (int v = 0; v < NG; ++v) {
+                const Variant& V = vtab[grp_v[v]];
+                const double lt = (double)lane_tile(V), cap_tile = lt * vS[v] * gG[v];
+                u64 thin = 0, lane_thin = 0;
+                double real = 0, fill = 0;
+                for (const Rect& r : grect[v]) {
+                    const u64 t = r.lane_blocks * r.stream_groups;
+                    if (r.n_stream < 128) thin += t;
+                    if ((double)r.n_lane < lt) lane_thin += t;
+                    real += (double)r.n_lane * (double)r.n_stream;
+                    fill += (double)t * cap_tile;
+                }
+                std::fprintf(stderr, "[prof rects] phase %d variant %d: %zu rects, %llu tiles (%llu with < 128 stream, %llu with a partial lane tile), fill %.3f, %.4g codewords\n",
+                             grp_phase[v], grp_v[v], grect[v].size(), (unsigned long long)gtiles[v], (unsigned long long)thin,
+                             (unsigned long long)lane_thin, fill > 0 ? real / fill : 0.0, real);
+            }
         int ngroups = 0;
         for (const TItem& it : items) ngroups = std::max(ngroups, it.group + 1);
         // The sections that the kernels update with atomics sit on cache lines of their own:
@@ -2543,10 +2625,10 @@ struct Engine {
         auto line_up = [](size_t x) { return (x + kLine - 1) & ~(kLine - 1); };
         const size_t o_cnt = 0, o_best = (size_t)NG * kLine;
         const size_t o_hits = line_up(o_best + (size_t)ni * sizeof(int));
-        const size_t o_skip = line_up(o_hits + (size_t)ni * sizeof(u64));
+        const size_t o_skip = line_up(o_hits + (size_t)nj * sizeof(u64));
         const size_t o_shared = line_up(o_skip + (size_t)ni * sizeof(u64));
         const size_t o_jobs = line_up(o_shared + (size_t)ngroups * sizeof(int));
-        const size_t o_rects = align16(o_jobs + (size_t)ni * sizeof(TJob));
+        const size_t o_rects = align16(o_jobs + (size_t)nj * sizeof(TJob));
         const size_t o_fixed = align16(o_rects + nrect * sizeof(Rect));
         const size_t total = align16(o_fixed + hfixed.size() * sizeof(u32) + 4);
         unsigned char *host = nullptr, *dev = nullptr;
@@ -2574,6 +2656,10 @@ struct Engine {
             J.K = it.p.K; J.RD = s.RD; J.SD = s.SD; J.RW = s.RW; J.stride = it.p.stride;
             J.floor = it.floor;
             hj[i] = J;
+            if (job2[i] >= 0) {
+                J.hits = reinterpret_cast<unsigned long long*>(dev + o_hits) + job2[i];
+                hj[job2[i]] = J;
+            }
         }
         std::vector<size_t> grect_off(NG, 0);
         {
@@ -2591,11 +2677,16 @@ struct Engine {
         stats.kernel = 2;
         stats.tiles = 0;
         std::vector<int> used(NG, 0);
+        std::vector<u64> grp_side_cw(NG, 0);
+        for (int v = 0; v < NG; ++v)
+            if (grp_side[v])
+                for (const Rect& r : grect[v]) grp_side_cw[v] += r.n_lane * r.n_stream;
         while (gev_dyn.size() < (size_t)(2 * NG)) {   // start/stop events per launch group
             hipEvent_t e = nullptr;
             HIPCHK(hipEventCreate(&e));
             gev_dyn.push_back(e);
         }
+        double stats_cw = -1;
         for (int gi = 0; gi < NG; ++gi) {   // groups are numbered in launch order
             const int v = grp_v[gi];
             if (gtiles[gi] == 0) continue;
@@ -2603,7 +2694,7 @@ struct Engine {
             const Variant& V = vtab[v];
             TiledArgs a{};
             a.jobs = reinterpret_cast<const TJob*>(dev + o_jobs);
-            a.njobs = ni;
+            a.njobs = ni;   // levels (distinct bounds): a second job entry shares its level's bound
             a.rects = reinterpret_cast<const Rect*>(dev + grect_off[gi]);
             a.nrects = (int)grect[gi].size();
             a.total_tiles = gtiles[gi];
@@ -2650,14 +2741,22 @@ struct Engine {
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(gev_dyn[2 * gi + 1], stream));
             stats.tiles = (int)std::min<u64>((u64)stats.tiles + gtiles[gi], 0x7fffffff);
-            stats.fd = V.fd;
-            stats.xd = V.xd;
-            stats.rpl = V.rpl;
-            stats.mfma = V.mm;
-            // VALU lane-ops per codeword of the hot loop; the matrix tests spend 0.5 (v_min3
-            // over their products) besides the products themselves (32 int8 multiply-adds per
-            // codeword, or 64 / 128 fp4 ones)
-            stats.ops_per_codeword = V.mm > 0 ? 0.5 : (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
+            // the run's kernel in the stats: the variant chosen for the most codewords (a
+            // matrix-test level counts for its matrix variant, thin products included)
+            double gcw = 0;
+            for (int i = 0; i < ni; ++i)
+                if (item_grp[i] == gi) gcw += (double)sat_mul(items[i].s->C(items[i].p.K, items[i].p.d), (u64)items[i].p.num_gamma);
+            if (gcw > stats_cw) {
+                stats_cw = gcw;
+                stats.fd = V.fd;
+                stats.xd = V.xd;
+                stats.rpl = V.rpl;
+                stats.mfma = V.mm;
+                // VALU lane-ops per codeword of the hot loop; the matrix tests spend 0.5 (v_min3
+                // over their products) besides the products themselves (32 int8 multiply-adds per
+                // codeword, or 64 / 128 fp4 ones)
+                stats.ops_per_codeword = V.mm > 0 ? 0.5 : (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
+            }
         }
         HIPCHK(hipMemcpyAsync(host, dev, o_jobs, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -2669,6 +2768,10 @@ struct Engine {
         const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(host + o_cnt);
         double ms_total = 0;
         stats.slow_hits = stats.medium_hits = 0;
+        stats.side_ms = 0;
+        stats.side_codewords = 0;
+        stats.side_ops_per_codeword = 0;
+        double side_ops = 0;
         out_rows_last = 0;
         static const bool prof_groups = std::getenv("QSA_PROFILE_GROUPS") != nullptr;   // diagnostics
         for (int gi = 0; gi < NG; ++gi) {
@@ -2676,13 +2779,19 @@ struct Engine {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, gev_dyn[2 * gi], gev_dyn[2 * gi + 1]));
             ms_total += ms;
+            if (grp_side[gi]) {   // a launch of thin products only: time and codewords beside the matrix test
+                const Variant& V = vtab[grp_v[gi]];
+                stats.side_ms += ms;
+                stats.side_codewords += grp_side_cw[gi];
+                side_ops += (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl * (double)grp_side_cw[gi];
+            }
             if (prof_groups) {
                 u64 cw = 0;
                 int nitems = 0;
                 for (int i = 0; i < ni; ++i)
                     if (item_grp[i] == gi) { cw += sat_mul(items[i].s->C(items[i].p.K, items[i].p.d), (u64)items[i].p.num_gamma); ++nitems; }
-                std::fprintf(stderr, "[prof group] phase %d variant %d: %d levels, %llu codewords, %llu tiles, S %d G %d, %.3f ms, exact %llu medium %llu\n",
-                             grp_phase[gi], grp_v[gi], nitems, (unsigned long long)cw, (unsigned long long)gtiles[gi], vS[gi], gG[gi], ms,
+                std::fprintf(stderr, "[prof group] phase %d variant %d%s: %d levels, %llu codewords, %llu tiles, S %d G %d, %.3f ms, exact %llu medium %llu\n",
+                             grp_phase[gi], grp_v[gi], grp_side[gi] ? " (thin products)" : "", nitems, (unsigned long long)cw, (unsigned long long)gtiles[gi], vS[gi], gG[gi], ms,
                              hc[kCntStride * gi + 1], hc[kCntStride * gi + 2]);
             }
             stats.slow_hits += hc[kCntStride * gi + 1];
@@ -2694,12 +2803,14 @@ struct Engine {
                 out_rows_last += hc[kCntStride * gi + 3];
             }
         }
+        if (stats.side_codewords) stats.side_ops_per_codeword = side_ops / (double)stats.side_codewords;
         stats.kernel_ms = ms_total;
     }
     unsigned long long out_rows_last = 0;   // collect mode: rows the last tiled run produced
     DevBuf<u64> sorted;                      // collect_upto: the deduplicated rows
     std::vector<u64> item_hits;             // the last tiled run's hot-test passes per item
     std::vector<u64> item_skipped;          // its tiles skipped through a shared bound, per item
+    std::vector<char> item_thin;            // a matrix-test item whose thin products ran on the VALU kernel
     std::vector<hipEvent_t> gev_dyn;        // start/stop events of a tiled run's launch groups
 
     void launch_sweep(Solve& s, const qsf_enum_plan& p, bool collect, int keep, u64 cap, int* bp,

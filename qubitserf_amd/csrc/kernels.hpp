@@ -89,7 +89,7 @@ struct TJob {
 
 struct TiledArgs {
     const TJob* jobs;
-    int njobs;
+    int njobs;               // levels in the launch; with one, the launch ends once it is settled
     const Rect* rects;       // sorted by tile_base
     int nrects;
     u64 total_tiles;

@@ -67,6 +67,35 @@ QSA_HD u64 colex_mask(u64 idx, int k, int m, const u64* B, int bk) {
     return colex_unrank<u64>(idx, k, m, B, bk);
 }
 
+// The unrank's binomials for the common case, small enough for LDS: C(n, i) for n < 64
+// and i <= 8 all fit 32 bits (the largest, C(63, 8) = 3 872 894 697), stored row-major with
+// kSmallBinCols columns. The kernels copy them once per workgroup, so an unrank of a list of
+// at most 64 rows and 8 elements costs LDS reads instead of a chain of dependent global
+// loads; other lists use the global table.
+constexpr int kSmallBinRows = 64, kSmallBinK = 8, kSmallBinCols = kSmallBinK + 1;
+template <class Mask>
+QSA_HD Mask colex_unrank_small(u64 idx, int k, int m, const std::uint32_t* S) {
+    Mask mask{};
+    int cap = m;
+    for (int i = k; i >= 1; --i) {
+        int lo = i - 1, hi = cap - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if ((u64)S[mid * kSmallBinCols + i] <= idx) lo = mid; else hi = mid - 1;
+        }
+        set_bit(mask, lo);
+        idx -= (lo >= i) ? (u64)S[lo * kSmallBinCols + i] : 0ull;
+        cap = lo;
+    }
+    return mask;
+}
+// The small table when the list fits it, else the global one.
+template <class Mask>
+QSA_HD Mask colex_unrank_any(u64 idx, int k, int m, const std::uint32_t* S, const u64* B, int bk) {
+    return (m <= kSmallBinRows && k <= kSmallBinK) ? colex_unrank_small<Mask>(idx, k, m, S)
+                                                   : colex_unrank<Mask>(idx, k, m, B, bk);
+}
+
 // Next larger integer with the same popcount (colex successor); x != 0.
 QSA_HD u64 colex_next(u64 x) {
     u64 c = x & (~x + 1ull);

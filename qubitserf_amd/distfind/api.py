@@ -139,7 +139,8 @@ def version() -> str:
 def last_trace():
     """Per-level records of the last solve on this thread, as dicts with the fields of
     DistfindLevel (include/distfind_capi.h): side, d, active, ngamma, K, inner, outer,
-    kernel, fd, xd, rpl, mfma, codewords, medium_hits, slow_hits, ms, kernel_ms, ops_per_codeword."""
+    kernel, fd, xd, rpl, mfma, codewords, medium_hits, slow_hits, ms, kernel_ms, ops_per_codeword,
+    side_ms, side_codewords, side_ops_per_codeword."""
     L = _native.load()
     n = L.distfind_last_trace(None, 0)
     buf = (_native.DistfindLevel * max(n, 1))()

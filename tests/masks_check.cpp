@@ -1,6 +1,6 @@
 // Host-side check of the row-mask helpers the enumeration kernels use (masks.hpp):
 // colex unrank and successor on one- and two-word masks against brute force and against
-// each other, bit ranges of 256-bit masks against bit-by-bit extraction, and the set-bit
+// each other, the 32-bit small-table unrank against the global-table one, bit ranges of 256-bit masks against bit-by-bit extraction, and the set-bit
 // walk. Built and run by tests/test_cpu_masks.py; prints the first mismatch and exits 1.
 #include <cstdio>
 #include <cstdlib>
@@ -102,6 +102,41 @@ void check_boundary(const std::vector<u64>& B) {
         }
 }
 
+// The 32-bit LDS table: exact copies of the global binomials it holds, and the same
+// unrank as the global table for every list it serves (lists of up to 64 rows, up to 8
+// elements), at the first, last and random indices.
+void check_small_table(const std::vector<u64>& B) {
+    std::vector<std::uint32_t> S((size_t)qsa::kSmallBinRows * qsa::kSmallBinCols);
+    for (int n = 0; n < qsa::kSmallBinRows; ++n)
+        for (int i = 0; i < qsa::kSmallBinCols; ++i) {
+            const u64 v = B[(size_t)n * kBk + i];
+            if (v > 0xffffffffull) { fail("small table overflow", n, i, v); return; }
+            S[(size_t)n * qsa::kSmallBinCols + i] = (std::uint32_t)v;
+        }
+    std::mt19937_64 rng(11);
+    for (int m = 1; m <= 64; ++m)
+        for (int k = 0; k <= qsa::kSmallBinK && k <= m; ++k) {
+            const u64 total = B[(size_t)m * kBk + k];
+            for (int t = 0; t < 400; ++t) {
+                const u64 i = t == 0 ? 0 : t == 1 ? total - 1 : rng() % total;
+                const u64 a = qsa::colex_unrank<u64>(i, k, m, B.data(), kBk);
+                const u64 b = qsa::colex_unrank_small<u64>(i, k, m, S.data());
+                const u64 c = qsa::colex_unrank_any<u64>(i, k, m, S.data(), B.data(), kBk);
+                const M2 d = qsa::colex_unrank_small<M2>(i, k, m, S.data());
+                if (a != b || a != c || d.lo != a || d.hi != 0) { fail("small-table unrank", m, k, i); return; }
+            }
+        }
+    // outside the table: the global path
+    for (int m : {65, 100, 128})
+        for (int k : {2, 9}) {
+            const u64 total = B[(size_t)m * kBk + k];
+            const u64 i = total / 3;
+            const M2 a = qsa::colex_unrank<M2>(i, k, m, B.data(), kBk);
+            const M2 c = qsa::colex_unrank_any<M2>(i, k, m, S.data(), B.data(), kBk);
+            if (!eq(a, c)) fail("unrank_any fallback", m, k, i);
+        }
+}
+
 void check_ranges() {
     std::mt19937_64 rng(7);
     for (int t = 0; t < 200000; ++t) {
@@ -152,6 +187,7 @@ int main() {
     check_chain_m2(B, 70, 4);
     check_chain_m2(B, 128, 4);
     check_boundary(B);
+    check_small_table(B);
     check_ranges();
     if (fails) {
         std::printf("%d failures\n", fails);

@@ -339,3 +339,25 @@ def test_automatic_choice_runs_the_cost_models_pick(hip, monkeypatch):
         assert got == O.enumerate_plan(gamma, n, d, check=check, current_best=best), (K, d, best)
         kinds.add(pick[3])
     assert {0, 1, 2} <= kinds, kinds
+
+
+@pytest.mark.parametrize("thin", ["0", "64", "256", "4096", "100000000"])
+def test_matrix_levels_with_thin_products_on_the_valu_kernel(hip, thin, monkeypatch):
+    """A matrix-test level's products with a list shorter than QSA_THIN_BELOW run on the
+    VALU kernel in a second launch with the same job and bound (0: none; 1e8: every
+    product). Results equal the oracle's for the int8 and fp4 tests, with tight and loose
+    bounds; the run reports the matrix variant while its launch has products left."""
+    monkeypatch.setenv("QSA_MATRIX_MIN_WORK", "0")
+    monkeypatch.setenv("QSA_THIN_BELOW", thin)
+    rng = np.random.default_rng(77)
+    seen = set()
+    for K, d, best in [(80, 5, 7), (80, 5, 13), (72, 4, 16), (96, 4, 9), (60, 5, 9), (110, 4, 12)]:
+        gamma, check = rand_plan(rng, K, 3, 1 + K % 2, 2, systematic=K % 3 != 1)
+        n = 192
+        want = O.enumerate_plan(gamma, n, d, check=check, current_best=best)
+        got = hip.enumerate_level(gamma, n, d, check=check, current_best=best)
+        st = hip.last_stats()
+        assert got == want, (K, d, best, thin, got, want, st)
+        seen.add(st["mfma"])
+    if int(thin) <= 256:
+        assert seen & {1, 2}, seen
