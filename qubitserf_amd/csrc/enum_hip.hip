@@ -95,6 +95,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
     __shared__ SElem<XD> stream[kStreamMax];
     __shared__ u32 sfull[kMaxRW];
     __shared__ Mask run_start[64];           // colex mask of the first stream element of each lane's run
+    __shared__ int run_fix[64];              // and its fixed set (a rectangle with a list of them)
     extern __shared__ u32 rows_full[];       // whole residual+syndrome table when it fits (dynamic size)
     __shared__ u32 sbin[kSmallBinRows * kSmallBinCols];   // the unrank's binomials C(n < 64, k <= 8)
 
@@ -116,13 +117,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         const u64 t = uni64(t0);
         if (t >= a.total_tiles) break;
 
-        // locate the rectangle holding tile t (rects sorted by tile_base)
-        int lo = 0, hi = a.nrects - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (a.rects[mid].tile_base <= t) lo = mid; else hi = mid - 1;
-        }
-        const Rect& Rr = a.rects[uni(lo)];
+        const Rect& Rr = a.rects[uni(find_rect(a, t))];   // the rectangle holding tile t
         const int job = uni(Rr.job);
         if (job != cur_job) {   // the problem's parameters, and its running bound
             if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
@@ -141,6 +136,14 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 continue;   // this problem is settled; its remaining tiles are skipped
             }
             // another level of this problem, in the same launch, settled it
+            if (J->choice) {   // one of the level's candidate variants: run only the one picked
+                int sb = bound;
+                if (J->shared != J->best)
+                    sb = min(sb, uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                int c = 0;
+                if (lane == 0) c = pick_variant(J, sb);
+                if (uni(c) != J->variant) continue;
+            }
             if (J->shared != J->best) {
                 const int sb = uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                 if (sb <= floor) {
@@ -156,9 +159,12 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         }
         struct {
             u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream;
-            int fixed;
+            int fixed, nfix;
         } R = {uni64(Rr.tile_base), uni64(Rr.stream_groups), uni64(Rr.stream_chunks), uni64(Rr.n_lane),
-               uni64(Rr.n_stream), uni(Rr.fixed)};
+               uni64(Rr.n_stream), uni(Rr.fixed), uni(Rr.nfix)};
+        // a list of fixed sets: the stream is (fixed set, subset) pairs, ns subsets each
+        const bool flist = R.nfix > 1;
+        const u64 ns = flist ? R.n_stream / (u64)R.nfix : R.n_stream;
         const u64 local = t - R.tile_base;
         const u64 bx = local / R.stream_groups;
         const u64 bg = local - bx * R.stream_groups;
@@ -225,21 +231,46 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 const int i0 = lane * per;
                 const int i1 = min(len, i0 + per);
                 if (i0 < i1) {
-                    Mask m = colex_unrank_any<Mask>(c0 + (u64)i0, sk, sm, sbin, a.binom, a.bk);
+                    const u64 e0 = c0 + (u64)i0;
+                    int fi = flist ? (int)(e0 / ns) : 0;         // this run's fixed set
+                    u64 si = flist ? e0 - (u64)fi * ns : e0;     // and its subset's rank
+                    Mask m = colex_unrank_any<Mask>(si, sk, sm, sbin, a.binom, a.bk);
                     run_start[lane] = m;
+                    run_fix[lane] = fi;
+                    u32 fwe[XD];
+                    int foffe = foff;
+#pragma unroll
+                    for (int f = 0; f < XD; ++f) fwe[f] = fw[f];
+                    auto load_fix = [&](int k) {   // fixed set k of the rectangle's list
+                        const u32* fk = J->fixed + (size_t)(R.fixed + k) * (RW + 1);
+#pragma unroll
+                        for (int f = 0; f < XD; ++f) fwe[f] = f < RD ? fk[f] : 0u;
+                        foffe = (int)fk[RW];
+                    };
+                    if (flist) load_fix(fi);
                     for (int i = i0; i < i1; ++i) {
                         SElem<XD> e;
 #pragma unroll
-                        for (int f = 0; f < XD; ++f) e.w[f] = fw[f];
+                        for (int f = 0; f < XD; ++f) e.w[f] = fwe[f];
                         Mask mm = m;
                         while (nonzero(mm)) {
                             const int b = take_low(mm);
 #pragma unroll
                             for (int f = 0; f < XD; ++f) e.w[f] ^= rows_fast[(sbase + b) * XD + f];
                         }
-                        e.off = foff + pop_and(m, sflags);
+                        e.off = foffe + pop_and(m, sflags);
                         stream[i] = e;
-                        if (i + 1 < i1) m = colex_next(m);
+                        if (i + 1 < i1) {
+                            if (flist && si + 1 == ns) {   // next fixed set, first subset
+                                ++fi;
+                                si = 0;
+                                m = colex_first<Mask>(sk);
+                                load_fix(fi);
+                            } else {
+                                m = colex_next(m);
+                                ++si;
+                            }
+                        }
                     }
                 }
             }
@@ -278,9 +309,20 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                     ++slow_entries;
                     const int owner = s / per;
                     Mask smask = run_start[owner];
-                    for (int j = owner * per; j < s; ++j) smask = colex_next(smask);
+                    const u32* fxe = fx;   // the candidate stream element's fixed set
+                    if (flist) {
+                        int fi = run_fix[owner];
+                        u64 si = c0 + (u64)(owner * per) - (u64)fi * ns;
+                        for (int j = owner * per; j < s; ++j) {
+                            if (si + 1 == ns) { ++fi; si = 0; smask = colex_first<Mask>(sk); }
+                            else { smask = colex_next(smask); ++si; }
+                        }
+                        fxe = J->fixed + (size_t)(R.fixed + fi) * (RW + 1);
+                    } else {
+                        for (int j = owner * per; j < s; ++j) smask = colex_next(smask);
+                    }
                     if (lane < RW) {
-                        u32 v = fx ? fx[lane] : 0u;
+                        u32 v = fxe ? fxe[lane] : 0u;
                         Mask mm = smask;
                         while (nonzero(mm)) v ^= slow_tbl[(sbase + take_low(mm)) * RW + lane];
                         sfull[lane] = v;
@@ -402,6 +444,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
     __shared__ u64 lmk[kMfmaWaves * 256];                     // each wave's lane-element subsets
     __shared__ unsigned short lay[64 * 16];                   // product layout: (row, column) of lane l, value v
     __shared__ unsigned long long s_tile;
+    __shared__ int s_choice;                                   // the candidate picked for the tile's level
     __shared__ u32 sbin[kSmallBinRows * kSmallBinCols];        // the unrank's binomials C(n < 64, k <= 8)
     extern __shared__ u32 rows_full[];
 
@@ -435,12 +478,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         __syncthreads();
         const u64 t = uni64(s_tile);
         if (t >= a.total_tiles) break;
-        int lo = 0, hi = a.nrects - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (a.rects[mid].tile_base <= t) lo = mid; else hi = mid - 1;
-        }
-        const Rect& Rr = a.rects[uni(lo)];
+        const Rect& Rr = a.rects[uni(find_rect(a, t))];   // the rectangle holding tile t
         const int job = uni(Rr.job);
         if (job != cur_job) {
             if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
@@ -458,6 +496,11 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
         }
         {   // another level of this problem, in the same launch, settled it
             const int sb = uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (J->choice) {   // one of the level's candidate variants: run only the one picked
+                if (tid == 0) s_choice = pick_variant(J, J->shared != J->best ? min(bound, sb) : bound);
+                __syncthreads();
+                if (s_choice != J->variant) continue;
+            }
             if (__syncthreads_or(J->shared != J->best && sb <= floor)) {
                 if (tid == 0) atomicAdd(J->skipped, 1ull);
                 continue;
@@ -779,6 +822,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     __shared__ unsigned short lay[64 * NV];       // product layout: (row, column) of lane l, value v
     __shared__ int s_bad;
     __shared__ unsigned long long s_tile;
+    __shared__ int s_choice;                                   // the candidate picked for the tile's level
     __shared__ u32 sbin[kSmallBinRows * kSmallBinCols];        // the unrank's binomials C(n < 64, k <= 8)
     extern __shared__ u32 rows_full[];
 
@@ -824,12 +868,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         __syncthreads();
         const u64 t = uni64(s_tile);
         if (t >= a.total_tiles) break;
-        int lo = 0, hi = a.nrects - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (a.rects[mid].tile_base <= t) lo = mid; else hi = mid - 1;
-        }
-        const Rect& Rr = a.rects[uni(lo)];
+        const Rect& Rr = a.rects[uni(find_rect(a, t))];   // the rectangle holding tile t
         const int job = uni(Rr.job);
         if (job != cur_job) {
             if (lane == 0 && job_hits) atomicAdd(J->hits, (unsigned long long)job_hits);
@@ -847,6 +886,11 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         }
         {   // another level of this problem, in the same launch, settled it
             const int sb = uni(__hip_atomic_load(J->shared, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (J->choice) {   // one of the level's candidate variants: run only the one picked
+                if (tid == 0) s_choice = pick_variant(J, J->shared != J->best ? min(bound, sb) : bound);
+                __syncthreads();
+                if (s_choice != J->variant) continue;
+            }
             if (__syncthreads_or(J->shared != J->best && sb <= floor)) {
                 if (tid == 0) atomicAdd(J->skipped, 1ull);
                 continue;
@@ -2004,8 +2048,11 @@ struct Engine {
             stats.codewords = sat_add(stats.codewords, cw);
             if (tileable(s, p) && policy != 1) {
                 item_of[j] = (int)items.size();
-                items.push_back(TItem{&s, p, floors ? floors[j] : -1, choose_variant(s, p, true, matrix_min_work() >> 6),
-                                      groups ? groups[j] : -1, phase[j]});
+                TItem it{&s, p, floors ? floors[j] : -1, choose_variant(s, p, true, matrix_min_work() >> 6),
+                         groups ? groups[j] : -1, phase[j]};
+                // a level after the first of its problem meets a bound the call cannot know
+                if (groups && phase[j] > 0 && !std::getenv("QSA_TILED_VARIANT")) candidate_segments(it);
+                items.push_back(it);
             } else if (small_sweep(p, cw)) {
                 small_of[j] = (int)small.size();
                 small.push_back(SItem{&s, &plans[j]});
@@ -2014,22 +2061,43 @@ struct Engine {
             }
         }
         {   // a matrix variant whose levels together stay under the single-level minimum
-            // goes back to the VALU kernel
+            // goes back to the VALU kernel (a level counts once per variant among its
+            // candidates)
+            auto same_mm = [](const Variant& x, const Variant& y) { return x.mm == y.mm && x.xd == y.xd && x.rpl == y.rpl; };
             std::vector<std::pair<Variant, u64>> tot;
+            auto add = [&](const Variant& v, u64 cw) {
+                auto f = std::find_if(tot.begin(), tot.end(), [&](const std::pair<Variant, u64>& e) { return same_mm(e.first, v); });
+                if (f == tot.end()) tot.push_back({v, cw}); else f->second = sat_add(f->second, cw);
+            };
+            auto small_total = [&](const Variant& v) {
+                for (const auto& e : tot)
+                    if (same_mm(e.first, v)) return e.second < matrix_min_work();
+                return false;
+            };
             for (const TItem& it : items) {
-                if (it.V.mm == 0) continue;
                 const u64 cw = sat_mul(it.s->C(it.p.K, it.p.d), (u64)it.p.num_gamma);
-                auto f = std::find_if(tot.begin(), tot.end(), [&](const std::pair<Variant, u64>& e) {
-                    return e.first.mm == it.V.mm && e.first.xd == it.V.xd && e.first.rpl == it.V.rpl;
-                });
-                if (f == tot.end()) tot.push_back({it.V, cw}); else f->second = sat_add(f->second, cw);
+                if (it.nseg == 0) {
+                    if (it.V.mm != 0) add(it.V, cw);
+                    continue;
+                }
+                for (int k = 0; k < it.nseg; ++k) {
+                    bool seen = false;
+                    for (int q = 0; q < k; ++q) seen = seen || same_variant(it.seg_v[q], it.seg_v[k]);
+                    if (!seen && it.seg_v[k].mm != 0) add(it.seg_v[k], cw);
+                }
             }
             for (TItem& it : items) {
-                if (it.V.mm == 0) continue;
-                for (const auto& e : tot)
-                    if (e.first.mm == it.V.mm && e.first.xd == it.V.xd && e.first.rpl == it.V.rpl &&
-                        e.second < matrix_min_work())
-                        it.V = choose_variant(*it.s, it.p, false);
+                if (it.nseg == 0) {
+                    if (it.V.mm != 0 && small_total(it.V)) it.V = choose_variant(*it.s, it.p, false);
+                    continue;
+                }
+                for (int k = 0; k < it.nseg; ++k)
+                    if (it.seg_v[k].mm != 0 && small_total(it.seg_v[k])) {
+                        qsf_enum_plan pb = it.p;
+                        pb.current_best = it.seg_max[k];
+                        it.seg_v[k] = choose_variant(*it.s, pb, false);
+                    }
+                prune_segments(it);
             }
         }
         static const bool prof_items = std::getenv("QSA_PROFILE_ITEMS") != nullptr;   // diagnostics
@@ -2294,6 +2362,7 @@ struct Engine {
         int g, RW, S;
         u64 lane_tile;
         bool matrix;   // a matrix-core test: orient each product by its padded work
+        u64 flist_below = 0;   // > 0: products with a list this short take fixed-set lists
         std::vector<Rect> rv;
         std::vector<u32> fixed;
         static constexpr u64 kExpandBelow = 64;
@@ -2318,20 +2387,51 @@ struct Engine {
             r.lane_blocks = (r.n_lane + lane_tile - 1) / lane_tile;
             r.stream_chunks = (r.n_stream + S - 1) / S;
             r.fixed = -1;
+            r.nfix = 0;
             for (int q = 0; q < 4; ++q) r.fmask[q] = f.w[q];
             if (f.w[0] | f.w[1] | f.w[2] | f.w[3]) {
-                r.fixed = (int)(fixed.size() / (RW + 1));
-                const size_t at = fixed.size();
-                fixed.resize(at + RW + 1, 0);
-                int off = 0;
-                for (int row = 0; row < s.K; ++row) {
-                    if (!((f.w[row >> 6] >> (row & 63)) & 1)) continue;
-                    const u32* src = s.htbl.data() + ((size_t)g * s.K + row) * RW;
-                    for (int w = 0; w < RW; ++w) fixed[at + w] ^= src[w];
-                    off += (int)((s.hflags[4 * g + (row >> 6)] >> (row & 63)) & 1);
-                }
-                fixed[at + RW] = (u32)off;
+                r.fixed = add_fixed(f);
+                r.nfix = 1;
             }
+            rv.push_back(r);
+        }
+        // One fixed-set entry: the XOR of its rows' residual + syndrome words, then its
+        // flagged-row count. Returns its index.
+        int add_fixed(const RowSet& f) {
+            const int idx = (int)(fixed.size() / (RW + 1));
+            const size_t at = fixed.size();
+            fixed.resize(at + RW + 1, 0);
+            int off = 0;
+            for (int row = 0; row < s.K; ++row) {
+                if (!((f.w[row >> 6] >> (row & 63)) & 1)) continue;
+                const u32* src = s.htbl.data() + ((size_t)g * s.K + row) * RW;
+                for (int w = 0; w < RW; ++w) fixed[at + w] ^= src[w];
+                off += (int)((s.hflags[4 * g + (row >> 6)] >> (row & 63)) & 1);
+            }
+            fixed[at + RW] = (u32)off;
+            return idx;
+        }
+        // The products F_e + subsets(x) x subsets(y) of every fixed set F_e of a list as one
+        // rectangle for the VALU kernel: the longer list on the lanes, the stream every
+        // (F_e, shorter-list subset) pair. common: the rows all F_e share.
+        void emit_flist(const RowSet& common, const std::vector<RowSet>& fs, int xb, int xm, int xk, int yb, int ym,
+                        int yk, u64 nx, u64 ny) {
+            Rect r{};
+            r.g = g;
+            const bool x_lanes = nx >= ny;
+            r.lbase = x_lanes ? xb : yb; r.lm = x_lanes ? xm : ym; r.lk = x_lanes ? xk : yk;
+            r.sbase = x_lanes ? yb : xb; r.sm = x_lanes ? ym : xm; r.sk = x_lanes ? yk : xk;
+            r.n_lane = x_lanes ? nx : ny;
+            r.n_stream = (x_lanes ? ny : nx) * (u64)fs.size();
+            r.lane_blocks = (r.n_lane + lane_tile - 1) / lane_tile;
+            r.stream_chunks = (r.n_stream + S - 1) / S;
+            for (int q = 0; q < 4; ++q) r.fmask[q] = common.w[q];
+            r.fixed = -1;
+            for (const RowSet& fe : fs) {
+                const int idx = add_fixed(fe);
+                if (r.fixed < 0) r.fixed = idx;
+            }
+            r.nfix = (int)fs.size();
             rv.push_back(r);
         }
 
@@ -2353,6 +2453,31 @@ struct Engine {
             // rows (or the subset is the whole range, of sk <= 32 rows)
             const int sb = x_small ? xb : yb, sk = x_small ? xk : yk;
             const int mid = bm / 2;
+            if (flist_below > 0 && ns > 1 && depth + 1 >= max_depth) {
+                // the children are emitted as they are: those whose shorter list is short
+                // take every fixed set of the small side in one rectangle
+                std::vector<RowSet> fs(ns, f);
+                u64 m = sk == 0 ? 0 : ((1ull << sk) - 1);
+                for (u64 e = 0; e < ns; ++e) {
+                    for (u64 mm = m; mm; mm &= mm - 1) {
+                        const int row = sb + __builtin_ctzll(mm);
+                        fs[e].w[row >> 6] |= 1ull << (row & 63);
+                    }
+                    if (e + 1 < ns) {
+                        const u64 c = m & (~m + 1ull), r = m + c;
+                        m = (((r ^ m) >> 2) >> __builtin_ctzll(c)) | r;
+                    }
+                }
+                for (int j = std::max(0, bk - (bm - mid)); j <= std::min(bk, mid); ++j) {
+                    const u64 a2 = s.C(mid, j), b2 = s.C(bm - mid, bk - j);
+                    if (a2 == 0 || b2 == 0) continue;
+                    if (std::min(a2, b2) < flist_below)
+                        emit_flist(f, fs, bb, mid, j, bb + mid, bm - mid, bk - j, a2, b2);
+                    else
+                        for (const RowSet& fe : fs) emit(fe, bb, mid, j, bb + mid, bm - mid, bk - j, a2, b2);
+                }
+                return;
+            }
             u64 m = sk == 0 ? 0 : ((1ull << sk) - 1);
             for (u64 e = 0; e < ns; ++e) {
                 RowSet fe = f;
@@ -2377,7 +2502,59 @@ struct Engine {
         Variant V;
         int group = -1;   // items (levels) of one problem share a bound slot; -1: alone
         int phase = 0;    // launch phase: phases run in order, each after the previous one
+        // Candidate variants of a level queued ahead of its bound (nseg >= 2, at least one a
+        // matrix test): segment k's variant serves bounds <= seg_max[k]; V is the last.
+        int nseg = 0;
+        int seg_max[4] = {0, 0, 0, 0};
+        Variant seg_v[4];
     };
+
+    static bool same_variant(const Variant& x, const Variant& y) {
+        return x.fd == y.fd && x.xd == y.xd && x.rpl == y.rpl && x.mm == y.mm && x.w == y.w;
+    }
+    // A level queued ahead of the bound it will meet: the variant the cost model picks at
+    // each bound from the level's floor + 1 (at most kCandidateBounds below the current
+    // bound) up to the current bound, as ascending segments; a bound below the lowest one
+    // priced takes the lowest segment. Kept only when some segment is a matrix test: the
+    // VALU variants differ little in cost, and the matrix tests share one decomposition.
+    static constexpr int kCandidateBounds = 8;
+    void candidate_segments(TItem& it) const {
+        const int hi = it.p.current_best;
+        it.nseg = 0;
+        if (hi >= QSF_WEIGHT_NONE || hi > 127) return;
+        const int lo = std::max({1, it.floor + 1, hi - kCandidateBounds + 1});
+        for (int b = lo; b <= hi; ++b) {
+            qsf_enum_plan pb = it.p;
+            pb.current_best = b;
+            const Variant v = choose_variant(*it.s, pb, true, matrix_min_work() >> 6);
+            if (it.nseg > 0 && same_variant(it.seg_v[it.nseg - 1], v)) {
+                it.seg_max[it.nseg - 1] = b;
+            } else if (it.nseg < 4) {
+                it.seg_v[it.nseg] = v;
+                it.seg_max[it.nseg++] = b;
+            } else {   // more than four segments: the last one covers the rest
+                it.seg_max[3] = b;
+            }
+        }
+        prune_segments(it);
+    }
+    // Merges neighbouring segments of one variant; drops the candidates when none is a
+    // matrix test or only one remains (the level then runs its top segment's variant).
+    static void prune_segments(TItem& it) {
+        int m = 0;
+        bool matrix = false;
+        for (int k = 0; k < it.nseg; ++k) {
+            matrix = matrix || it.seg_v[k].mm > 0;
+            if (m > 0 && same_variant(it.seg_v[m - 1], it.seg_v[k])) {
+                it.seg_max[m - 1] = it.seg_max[k];
+                continue;
+            }
+            it.seg_v[m] = it.seg_v[k];
+            it.seg_max[m++] = it.seg_max[k];
+        }
+        if (m > 0) it.V = it.seg_v[m - 1];
+        it.nseg = (m >= 2 && matrix) ? m : 0;
+    }
 
     static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
@@ -2427,6 +2604,7 @@ struct Engine {
         // leave the 1024-lane x 512-element tiles nearly empty) run on the level's VALU
         // variant instead, in a second group of the same phase: same job, same bound.
         std::vector<int> item_grp(ni), item_grp2(ni, -1), grp_v, grp_phase, grp_side;
+        std::vector<std::vector<int>> cand_v(ni), cand_grp(ni);   // a level's candidate variants and their groups
         // (tuning override QSA_THIN_BELOW, 0: off; a variant forced by QSA_TILED_VARIANT
         // runs the whole level)
         const char* thin_env = std::getenv("QSA_THIN_BELOW");
@@ -2449,15 +2627,30 @@ struct Engine {
             for (int i = 0; i < ni; ++i) idx[i] = i;
             std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return items[x].phase < items[y].phase; });
             for (int i : idx) {
-                item_grp[i] = group_of(items[i].phase, vidx(items[i].V), 0);
-                if (!collect && thin_below > 0 && items[i].V.mm > 0)
-                    item_grp2[i] = group_of(items[i].phase, vidx(choose_variant(*items[i].s, items[i].p, false)), 1);
+                const TItem& it = items[i];
+                if (!collect && it.nseg >= 2) {   // candidates: distinct variants in segment order
+                    for (int k = 0; k < it.nseg; ++k) {
+                        const int v = vidx(it.seg_v[k]);
+                        if (std::find(cand_v[i].begin(), cand_v[i].end(), v) == cand_v[i].end()) cand_v[i].push_back(v);
+                    }
+                    for (int v : cand_v[i]) cand_grp[i].push_back(group_of(it.phase, v, 0));
+                    item_grp[i] = cand_grp[i][0];
+                } else {
+                    item_grp[i] = group_of(it.phase, vidx(it.V), 0);
+                }
+                const bool matrix = cand_v[i].empty() ? it.V.mm > 0 : true;
+                if (!collect && thin_below > 0 && matrix)
+                    item_grp2[i] = group_of(it.phase, vidx(choose_variant(*it.s, it.p, false)), 1);
             }
         }
-        // an item with a second group gets a second job entry: same bound, table and skip
-        // count, its own pass counter (the tuner reads the first group's passes only)
-        std::vector<int> job2(ni, -1);
+        // Job entries: item i's own (its only or first candidate's), one more per further
+        // candidate, and one for its thin products; all share the level's bound, table and
+        // skip count, each with its own pass counter.
+        std::vector<std::vector<int>> cand_job(ni);
         int nj = ni;
+        for (int i = 0; i < ni; ++i)
+            for (size_t k = 0; k < cand_v[i].size(); ++k) cand_job[i].push_back(k == 0 ? i : nj++);
+        std::vector<int> job2(ni, -1);
         for (int i = 0; i < ni; ++i)
             if (item_grp2[i] >= 0) job2[i] = nj++;
         const int NG = (int)grp_v.size();
@@ -2472,8 +2665,11 @@ struct Engine {
         {
             const char* env = std::getenv("QSA_TILED_S");
             std::vector<double> vcw(NG, 0.0);
-            for (int i = 0; i < ni; ++i)
-                vcw[item_grp[i]] += (double)sat_mul(items[i].s->C(items[i].p.K, items[i].p.d), (u64)items[i].p.num_gamma);
+            for (int i = 0; i < ni; ++i) {
+                const double cw = (double)sat_mul(items[i].s->C(items[i].p.K, items[i].p.d), (u64)items[i].p.num_gamma);
+                vcw[item_grp[i]] += cw;
+                for (size_t k = 1; k < cand_grp[i].size(); ++k) vcw[cand_grp[i][k]] += cw;
+            }
             const double resident = (double)std::max(1, cus) * 16;
             const char* per = std::getenv("QSA_TILED_S_TILES");   // tuning: tiles per resident wave
             const double want = per ? std::atof(per) : 0.5;
@@ -2500,12 +2696,32 @@ struct Engine {
         // each item's decomposition on its own (host threads for large batches), then
         // concatenated in item order
         std::vector<std::vector<Rect>> irect(ni), irect2(ni);
+        std::vector<std::vector<std::vector<Rect>>> irectc(ni);   // per further candidate
         std::vector<std::vector<u32>> ifixed(ni);
+        auto valu_geometry = [&](Rect v, int grp, int job) {   // a product on the VALU kernel
+            const u64 lt = lane_tile(vtab[grp_v[grp]]);
+            const u64 S2 = (u64)vS[grp];
+            if (v.nfix <= 1 && v.n_lane < v.n_stream) {   // a fixed-set list stays on the stream
+                std::swap(v.lbase, v.sbase); std::swap(v.lm, v.sm); std::swap(v.lk, v.sk);
+                std::swap(v.n_lane, v.n_stream);
+            }
+            v.lane_blocks = (v.n_lane + lt - 1) / lt;
+            v.stream_chunks = (v.n_stream + S2 - 1) / S2;
+            v.job = job;
+            return v;
+        };
         auto decompose = [&](int i) {
             const TItem& it = items[i];
             const Solve& s = *it.s;
             const int K = it.p.K, d = it.p.d, h = s.h;
-            RectBuilder B{s, 0, s.RW, vS[item_grp[i]], lane_tile(it.V), it.V.mm > 0};
+            // a level with candidates is decomposed once, for its first matrix candidate
+            int dk = 0;
+            for (size_t k = 0; k < cand_v[i].size(); ++k)
+                if (vtab[cand_v[i][k]].mm > 0) { dk = (int)k; break; }
+            const int dgrp = cand_v[i].empty() ? item_grp[i] : cand_grp[i][dk];
+            const Variant& DV = vtab[grp_v[dgrp]];
+            RectBuilder B{s, 0, s.RW, vS[dgrp], lane_tile(DV), DV.mm > 0};
+            B.flist_below = collect ? 0 : thin_below;   // fixed-set lists: VALU kernel, min mode
             for (int g = 0; g < it.p.num_gamma; ++g) {
                 B.g = g;
                 for (int ka = std::max(0, d - (K - h)); ka <= std::min(d, h); ++ka)
@@ -2513,27 +2729,30 @@ struct Engine {
             }
             for (Rect& r : B.rv) r.job = i;
             if (item_grp2[i] >= 0) {   // thin products: VALU geometry, longer list on the lanes
-                const u64 lt = lane_tile(vtab[grp_v[item_grp2[i]]]);
-                const u64 S2 = (u64)vS[item_grp2[i]];
                 std::vector<Rect> fat;
                 for (Rect& r : B.rv) {
-                    if (std::min(r.n_lane, r.n_stream) >= thin_below) {
-                        fat.push_back(r);
-                        continue;
-                    }
-                    Rect v = r;
-                    if (v.n_lane < v.n_stream) {
-                        std::swap(v.lbase, v.sbase); std::swap(v.lm, v.sm); std::swap(v.lk, v.sk);
-                        std::swap(v.n_lane, v.n_stream);
-                    }
-                    v.lane_blocks = (v.n_lane + lt - 1) / lt;
-                    v.stream_chunks = (v.n_stream + S2 - 1) / S2;
-                    v.job = job2[i];
-                    irect2[i].push_back(v);
+                    if (r.nfix <= 1 && std::min(r.n_lane, r.n_stream) >= thin_below) fat.push_back(r);
+                    else irect2[i].push_back(valu_geometry(r, item_grp2[i], job2[i]));
                 }
                 B.rv.swap(fat);
             }
-            irect[i] = std::move(B.rv);
+            // the full products once per candidate: the matrix tests share the matrix
+            // geometry, a VALU candidate takes them in its own
+            irectc[i].resize(cand_v[i].size());
+            for (size_t k = 0; k < cand_v[i].size(); ++k) {
+                std::vector<Rect>& out = irectc[i][k];
+                out.reserve(B.rv.size());
+                for (const Rect& r : B.rv) {
+                    if (vtab[cand_v[i][k]].mm > 0) {
+                        Rect c = r;
+                        c.job = cand_job[i][k];
+                        out.push_back(c);
+                    } else {
+                        out.push_back(valu_geometry(r, cand_grp[i][k], cand_job[i][k]));
+                    }
+                }
+            }
+            irect[i] = cand_v[i].empty() ? std::move(B.rv) : std::vector<Rect>{};
             ifixed[i] = std::move(B.fixed);
         };
         const int workers = ni >= 16 ? (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())) : 1;
@@ -2570,6 +2789,11 @@ struct Engine {
                 const int v2 = item_grp2[i];
                 grect[v2].insert(grect[v2].end(), irect2[i].begin(), irect2[i].end());
                 if (it.p.K * s.RW <= kLdsFullRows) glds[v2] = std::max(glds[v2], it.p.K * s.RW);
+            }
+            for (size_t k = 0; k < cand_v[i].size(); ++k) {
+                const int vk = cand_grp[i][k];
+                grect[vk].insert(grect[vk].end(), irectc[i][k].begin(), irectc[i][k].end());
+                if (it.p.K * s.RW <= kLdsFullRows) glds[vk] = std::max(glds[vk], it.p.K * s.RW);
             }
         }
         item_thin.assign(ni, 0);
@@ -2614,8 +2838,12 @@ struct Engine {
                              grp_phase[v], grp_v[v], grect[v].size(), (unsigned long long)gtiles[v], (unsigned long long)thin,
                              (unsigned long long)lane_thin, fill > 0 ? real / fill : 0.0, real);
             }
-        int ngroups = 0;
-        for (const TItem& it : items) ngroups = std::max(ngroups, it.group + 1);
+        int ngroups = 0, nchoice = 0;
+        std::vector<int> choice_of(ni, -1);   // a candidate level's choice slot
+        for (int i = 0; i < ni; ++i) {
+            ngroups = std::max(ngroups, items[i].group + 1);
+            if (!cand_v[i].empty()) choice_of[i] = nchoice++;
+        }
         // The sections that the kernels update with atomics sit on cache lines of their own:
         // each group's tile counters (every workgroup increments next_tile at every tile),
         // the bounds (read by every wave at every stream chunk), the pass and skip counts
@@ -2627,10 +2855,27 @@ struct Engine {
         const size_t o_hits = line_up(o_best + (size_t)ni * sizeof(int));
         const size_t o_skip = line_up(o_hits + (size_t)nj * sizeof(u64));
         const size_t o_shared = line_up(o_skip + (size_t)ni * sizeof(u64));
-        const size_t o_jobs = line_up(o_shared + (size_t)ngroups * sizeof(int));
+        const size_t o_choice = line_up(o_shared + (size_t)ngroups * sizeof(int));
+        const size_t o_jobs = line_up(o_choice + (size_t)nchoice * sizeof(int));
         const size_t o_rects = align16(o_jobs + (size_t)nj * sizeof(TJob));
         const size_t o_fixed = align16(o_rects + nrect * sizeof(Rect));
-        const size_t total = align16(o_fixed + hfixed.size() * sizeof(u32) + 4);
+        // per group: the rectangles' first tiles (u64 each) and the tile index (u32 per
+        // 2^shift tiles, the shift keeping it under kTileIndex entries)
+        constexpr u64 kTileIndex = 1u << 16;
+        std::vector<int> gshift(NG, 0);
+        std::vector<u64> gnix(NG, 0), gtb_off(NG, 0), gix_off(NG, 0);
+        size_t ntb = 0, nix = 0;
+        for (int v = 0; v < NG; ++v) {
+            while ((gtiles[v] >> gshift[v]) > kTileIndex) ++gshift[v];
+            gnix[v] = gtiles[v] ? ((gtiles[v] - 1) >> gshift[v]) + 1 : 0;
+            gtb_off[v] = ntb;
+            gix_off[v] = nix;
+            ntb += grect[v].size();
+            nix += gnix[v];
+        }
+        const size_t o_tb = align16(o_fixed + hfixed.size() * sizeof(u32) + 4);
+        const size_t o_tix = align16(o_tb + ntb * sizeof(u64));
+        const size_t total = align16(o_tix + nix * sizeof(u32) + 4);
         unsigned char *host = nullptr, *dev = nullptr;
         stage_block(total, host, dev);
         std::memset(host, 0, o_jobs);
@@ -2639,6 +2884,8 @@ struct Engine {
         for (int g = 0; g < ngroups; ++g) hs[g] = QSF_WEIGHT_NONE;
         for (const TItem& it : items)
             if (it.group >= 0) hs[it.group] = std::min(hs[it.group], it.p.current_best);
+        int* hch = reinterpret_cast<int*>(host + o_choice);
+        for (int c = 0; c < nchoice; ++c) hch[c] = -1;
         TJob* hj = reinterpret_cast<TJob*>(host + o_jobs);
         for (int i = 0; i < ni; ++i) {
             const TItem& it = items[i];
@@ -2656,9 +2903,22 @@ struct Engine {
             J.K = it.p.K; J.RD = s.RD; J.SD = s.SD; J.RW = s.RW; J.stride = it.p.stride;
             J.floor = it.floor;
             hj[i] = J;
-            if (job2[i] >= 0) {
-                J.hits = reinterpret_cast<unsigned long long*>(dev + o_hits) + job2[i];
-                hj[job2[i]] = J;
+            if (job2[i] >= 0) {   // thin products: always run
+                TJob J2 = J;
+                J2.hits = reinterpret_cast<unsigned long long*>(dev + o_hits) + job2[i];
+                hj[job2[i]] = J2;
+            }
+            for (size_t k = 0; k < cand_v[i].size(); ++k) {
+                TJob Jc = J;
+                Jc.hits = reinterpret_cast<unsigned long long*>(dev + o_hits) + cand_job[i][k];
+                Jc.choice = reinterpret_cast<int*>(dev + o_choice) + choice_of[i];
+                Jc.variant = cand_v[i][k];
+                Jc.nseg = it.nseg;
+                for (int q = 0; q < 4; ++q) {
+                    Jc.seg_max[q] = q < it.nseg ? it.seg_max[q] : 0;
+                    Jc.seg_var[q] = q < it.nseg ? vidx(it.seg_v[q]) : cand_v[i][k];
+                }
+                hj[cand_job[i][k]] = Jc;
             }
         }
         std::vector<size_t> grect_off(NG, 0);
@@ -2671,6 +2931,20 @@ struct Engine {
             }
         }
         if (!hfixed.empty()) std::memcpy(host + o_fixed, hfixed.data(), hfixed.size() * sizeof(u32));
+        {
+            u64* tb = reinterpret_cast<u64*>(host + o_tb);
+            u32* tix = reinterpret_cast<u32*>(host + o_tix);
+            for (int v = 0; v < NG; ++v) {
+                const std::vector<Rect>& R = grect[v];
+                for (size_t r = 0; r < R.size(); ++r) tb[gtb_off[v] + r] = R[r].tile_base;
+                size_t r = 0;   // the rectangle holding tile q << shift, by a forward sweep
+                for (u64 q = 0; q < gnix[v]; ++q) {
+                    const u64 t = q << gshift[v];
+                    while (r + 1 < R.size() && R[r + 1].tile_base <= t) ++r;
+                    tix[gix_off[v] + q] = (u32)r;
+                }
+            }
+        }
         HIPCHK(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, stream));
 
         auto* cnt = reinterpret_cast<unsigned long long*>(dev + o_cnt);
@@ -2698,6 +2972,10 @@ struct Engine {
             a.rects = reinterpret_cast<const Rect*>(dev + grect_off[gi]);
             a.nrects = (int)grect[gi].size();
             a.total_tiles = gtiles[gi];
+            a.tile_base = reinterpret_cast<const u64*>(dev + o_tb) + gtb_off[gi];
+            a.tile_ix = reinterpret_cast<const unsigned*>(dev + o_tix) + gix_off[gi];
+            a.n_ix = (unsigned)gnix[gi];
+            a.ix_shift = gshift[gi];
             a.binom = gbinom;
             a.bk = kMaxD + 1;
             a.S = vS[gi];
@@ -2765,6 +3043,16 @@ struct Engine {
         item_hits.assign(hh, hh + ni);
         const unsigned long long* hk = reinterpret_cast<const unsigned long long*>(host + o_skip);
         item_skipped.assign(hk, hk + ni);
+        for (int i = 0; i < ni; ++i) {   // a candidate level: the variant picked and its passes
+            if (cand_v[i].empty()) continue;
+            const int c = hch[choice_of[i]];
+            for (size_t k = 0; k < cand_v[i].size(); ++k)
+                if (cand_v[i][k] == c) {
+                    items[i].V = vtab[c];
+                    item_hits[i] = hh[cand_job[i][k]];
+                }
+            items[i].nseg = 0;
+        }
         const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(host + o_cnt);
         double ms_total = 0;
         stats.slow_hits = stats.medium_hits = 0;

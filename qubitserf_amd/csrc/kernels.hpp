@@ -63,8 +63,12 @@ struct Rect {
     int lbase, lm, lk;     // lane list: lk-subsets of rows [lbase, lbase + lm)
     int sbase, sm, sk;     // stream list: sk-subsets of rows [sbase, sbase + sm)
     int fixed;             // index into TiledArgs::fixed, or -1 for no fixed rows
+    // nfix > 1 (VALU kernel, min mode): a list of nfix fixed row sets, entries fixed ..
+    // fixed + nfix - 1; the stream is then every (fixed set, sk-subset) pair, fixed set
+    // major: element e = f C(sm, sk) + s. fmask holds the rows all of them share.
+    int nfix;
     u64 fmask[4];          // the fixed rows as a 256-bit row mask
-    u64 n_lane, n_stream;  // list lengths C(lm, lk), C(sm, sk)
+    u64 n_lane, n_stream;  // list lengths C(lm, lk), C(sm, sk) (x nfix on the stream side)
     u64 lane_blocks;       // ceil(n_lane / (64 * RPL))
     u64 stream_chunks;     // ceil(n_stream / S)
     u64 stream_groups;     // ceil(stream_chunks / G): a tile is one lane block x G chunks
@@ -85,7 +89,30 @@ struct TJob {
     unsigned long long* hits;   // hot-test passes (medium-test entries) on this job, for the tuner
     int K, RD, SD, RW, stride;
     int floor;               // stop once best <= floor (floor < 0: never)
+    // A level queued ahead of the bound it will meet (several levels of a problem in one
+    // call) may carry candidate kernel variants for its full products: one job entry and
+    // launch per candidate over the same products, and the first wave that reaches the
+    // level picks the candidate for the bound it meets there, once, into *choice (-1: not
+    // yet); the other candidates' tiles are passed over. choice == nullptr: no choice.
+    // Candidate k serves bounds <= seg_max[k] (ascending); the last one any bound.
+    int* choice;
+    int variant;             // this job's candidate (the launch's kernel variant)
+    int nseg;
+    int seg_max[4];
+    int seg_var[4];
 };
+
+// The candidate of a multi-variant level for the bound the level meets: picked by the
+// first caller, the same for every later one.
+__device__ inline int pick_variant(const TJob* J, int bound) {
+    int c = __hip_atomic_load(J->choice, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (c >= 0) return c;
+    int want = J->seg_var[J->nseg - 1];
+    for (int k = 0; k + 1 < J->nseg; ++k)
+        if (bound <= J->seg_max[k]) { want = J->seg_var[k]; break; }
+    const int old = atomicCAS(J->choice, -1, want);
+    return old < 0 ? want : old;
+}
 
 struct TiledArgs {
     const TJob* jobs;
@@ -93,6 +120,13 @@ struct TiledArgs {
     const Rect* rects;       // sorted by tile_base
     int nrects;
     u64 total_tiles;
+    // Tile -> rectangle lookup: tile_base[r] = rects[r].tile_base (a compact copy), and
+    // tile_ix[q] = the rectangle holding tile q << ix_shift (n_ix entries), so the
+    // rectangle of tile t lies in [tile_ix[t >> ix_shift], tile_ix[(t >> ix_shift) + 1]].
+    const u64* tile_base;
+    const unsigned* tile_ix;
+    unsigned n_ix;
+    int ix_shift;
     const u64* binom;        // C(n, k) at binom[n * bk + k], n <= kMaxTiledK, k < bk, saturating
     int bk;
     int S;                   // stream chunk length (<= kStreamMax)
@@ -106,6 +140,19 @@ struct TiledArgs {
     u64* out;
     u64 out_cap;             // rows
 };
+
+// The rectangle holding tile t (see TiledArgs::tile_ix).
+__device__ inline int find_rect(const TiledArgs& a, u64 t) {
+    const u64 q = t >> a.ix_shift;
+    int lo = (int)a.tile_ix[q];
+    int hi = q + 1 < a.n_ix ? (int)a.tile_ix[q + 1] : a.nrects - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.tile_base[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 
 struct SweepArgs {
     const u64* gamma;        // [g][row][stride] full rows

@@ -96,6 +96,14 @@ QSA_HD Mask colex_unrank_any(u64 idx, int k, int m, const std::uint32_t* S, cons
                                                    : colex_unrank<Mask>(idx, k, m, B, bk);
 }
 
+// The first k-subset in colex order: rows 0 .. k - 1.
+template <class Mask>
+QSA_HD Mask colex_first(int k) {
+    Mask m{};
+    for (int b = 0; b < k; ++b) set_bit(m, b);
+    return m;
+}
+
 // Next larger integer with the same popcount (colex successor); x != 0.
 QSA_HD u64 colex_next(u64 x) {
     u64 c = x & (~x + 1ull);
