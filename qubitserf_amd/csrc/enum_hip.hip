@@ -2515,8 +2515,9 @@ struct Engine {
     // A level queued ahead of the bound it will meet: the variant the cost model picks at
     // each bound from the level's floor + 1 (at most kCandidateBounds below the current
     // bound) up to the current bound, as ascending segments; a bound below the lowest one
-    // priced takes the lowest segment. Kept only when some segment is a matrix test: the
-    // VALU variants differ little in cost, and the matrix tests share one decomposition.
+    // priced takes the lowest segment. Kept only when some segment is a matrix test (VALU
+    // candidates alone measured slower: their variants differ little in cost, and each
+    // candidate launch walks its passed-over tiles).
     static constexpr int kCandidateBounds = 8;
     void candidate_segments(TItem& it) const {
         const int hi = it.p.current_best;
@@ -2579,6 +2580,7 @@ struct Engine {
     // nullptr: each variant counts its own rows from 0.
     void run_tiled(std::vector<TItem>& items, bool collect, int keep, u64 cap, std::vector<int>& res,
                    unsigned long long* shared_rows = nullptr) {
+        const auto prof_t0 = std::chrono::steady_clock::now();   // host-time diagnostics (QSA_PROFILE_GROUPS)
         constexpr int NV = kVariants;
         static const Variant vtab[NV] = {
             {1, 1, 32, 0},    {1, 2, 16, 0},    {1, 2, 24, 0},    {1, 2, 32, 0},    {2, 2, 16, 0},    {2, 4, 12, 0},
@@ -2638,7 +2640,8 @@ struct Engine {
                 } else {
                     item_grp[i] = group_of(it.phase, vidx(it.V), 0);
                 }
-                const bool matrix = cand_v[i].empty() ? it.V.mm > 0 : true;
+                bool matrix = it.V.mm > 0;   // thin products leave a matrix test's launch
+                for (int v : cand_v[i]) matrix = matrix || vtab[v].mm > 0;
                 if (!collect && thin_below > 0 && matrix)
                     item_grp2[i] = group_of(it.phase, vidx(choose_variant(*it.s, it.p, false)), 1);
             }
@@ -2755,6 +2758,7 @@ struct Engine {
             irect[i] = cand_v[i].empty() ? std::move(B.rv) : std::vector<Rect>{};
             ifixed[i] = std::move(B.fixed);
         };
+        const auto prof_t1 = std::chrono::steady_clock::now();
         const int workers = ni >= 16 ? (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())) : 1;
         if (workers > 1) {
             std::atomic<int> next{0};
@@ -2945,6 +2949,7 @@ struct Engine {
                 }
             }
         }
+        const auto prof_t2 = std::chrono::steady_clock::now();
         HIPCHK(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, stream));
 
         auto* cnt = reinterpret_cast<unsigned long long*>(dev + o_cnt);
@@ -3036,8 +3041,17 @@ struct Engine {
                 stats.ops_per_codeword = V.mm > 0 ? 0.5 : (2.0 * V.fd * V.rpl + V.rpl / 2.0 + 3.0) / V.rpl;
             }
         }
+        const auto prof_t3 = std::chrono::steady_clock::now();
         HIPCHK(hipMemcpyAsync(host, dev, o_jobs, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
+        if (std::getenv("QSA_PROFILE_GROUPS")) {
+            auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+                return std::chrono::duration<double, std::milli>(b - a).count();
+            };
+            std::fprintf(stderr, "[prof run] %d items, %zu rects, %zu bytes: prepare %.3f ms, decompose+block %.3f ms, launch %.3f ms, wait %.3f ms\n",
+                         ni, nrect, total, ms(prof_t0, prof_t1), ms(prof_t1, prof_t2), ms(prof_t2, prof_t3),
+                         ms(prof_t3, std::chrono::steady_clock::now()));
+        }
         res.assign(hb, hb + ni);
         const unsigned long long* hh = reinterpret_cast<const unsigned long long*>(host + o_hits);
         item_hits.assign(hh, hh + ni);

@@ -126,6 +126,17 @@ void check_small_table(const std::vector<u64>& B) {
                 if (a != b || a != c || d.lo != a || d.hi != 0) { fail("small-table unrank", m, k, i); return; }
             }
         }
+    // the first subset of each size: rank 0 of the unrank
+    for (int k = 0; k <= 64; ++k) {
+        const u64 want = k == 64 ? ~0ull : ((1ull << k) - 1);
+        if (qsa::colex_first<u64>(k) != want) fail("colex_first<u64>", 64, k, 0);
+        if (k <= 8 && qsa::colex_unrank_small<u64>(0, k, 64, S.data()) != want) fail("first vs unrank", 64, k, 0);
+    }
+    for (int k = 0; k <= 128; ++k) {
+        const M2 f = qsa::colex_first<M2>(k);
+        M2 want{k >= 64 ? ~0ull : ((1ull << k) - 1), k >= 128 ? ~0ull : k > 64 ? ((1ull << (k - 64)) - 1) : 0ull};
+        if (!eq(f, want)) fail("colex_first<M2>", 128, k, 0);
+    }
     // outside the table: the global path
     for (int m : {65, 100, 128})
         for (int k : {2, 9}) {
