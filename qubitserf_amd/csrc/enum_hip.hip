@@ -2050,8 +2050,10 @@ struct Engine {
                 item_of[j] = (int)items.size();
                 TItem it{&s, p, floors ? floors[j] : -1, choose_variant(s, p, true, matrix_min_work() >> 6),
                          groups ? groups[j] : -1, phase[j]};
-                // a level after the first of its problem meets a bound the call cannot know
-                if (groups && phase[j] > 0 && !std::getenv("QSA_TILED_VARIANT")) candidate_segments(it);
+                // a level after the first of its problem meets a bound the call cannot know;
+                // below 2^26 codewords the choice costs more host time than it saves
+                if (groups && phase[j] > 0 && cw >= (1ull << 26) && !std::getenv("QSA_TILED_VARIANT"))
+                    candidate_segments(it);
                 items.push_back(it);
             } else if (small_sweep(p, cw)) {
                 small_of[j] = (int)small.size();
@@ -2642,8 +2644,14 @@ struct Engine {
                 }
                 bool matrix = it.V.mm > 0;   // thin products leave a matrix test's launch
                 for (int v : cand_v[i]) matrix = matrix || vtab[v].mm > 0;
-                if (!collect && thin_below > 0 && matrix)
-                    item_grp2[i] = group_of(it.phase, vidx(choose_variant(*it.s, it.p, false)), 1);
+                if (!collect && thin_below > 0 && matrix) {
+                    // a candidate level's thin products: the VALU variant at its matrix
+                    // candidates' highest bound (the bound at which it would leave them)
+                    qsf_enum_plan pv = it.p;
+                    for (int k = 0; k < it.nseg; ++k)
+                        if (it.seg_v[k].mm > 0) pv.current_best = it.seg_max[k];
+                    item_grp2[i] = group_of(it.phase, vidx(choose_variant(*it.s, pv, false)), 1);
+                }
             }
         }
         // Job entries: item i's own (its only or first candidate's), one more per further
