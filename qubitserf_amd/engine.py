@@ -236,7 +236,9 @@ def last_stats() -> dict:
     _native.load().qsf_hip_last_stats(ctypes.byref(s))
     return dict(codewords=int(s.codewords), kernel_ms=float(s.kernel_ms), kernel=int(s.kernel), tiles=int(s.tiles),
                 fd=int(s.fd), xd=int(s.xd), rpl=int(s.rpl), slow_hits=int(s.slow_hits),
-                medium_hits=int(s.medium_hits), ops_per_codeword=float(s.ops_per_codeword), mfma=int(s.mfma))
+                medium_hits=int(s.medium_hits), ops_per_codeword=float(s.ops_per_codeword), mfma=int(s.mfma),
+                side_ms=float(s.side_ms), side_codewords=int(s.side_codewords),
+                side_ops_per_codeword=float(s.side_ops_per_codeword))
 
 
 def pack_rows(M: np.ndarray) -> np.ndarray:

@@ -361,3 +361,52 @@ def test_matrix_levels_with_thin_products_on_the_valu_kernel(hip, thin, monkeypa
         seen.add(st["mfma"])
     if int(thin) <= 256:
         assert seen & {1, 2}, seen
+
+
+def sparse_plan(rng, K, n, ng, p, kcheck):
+    """Systematic generators with sparse redundancy (density p), so that light codewords
+    exist and the matrix tests are the cost model's pick at a bound just above them."""
+    stride = (n + 63) // 64
+    gamma = np.zeros((ng, K, stride), np.uint64)
+    for g in range(ng):
+        cols = rng.permutation(n)
+        M = np.zeros((K, n), np.uint8)
+        M[np.arange(K), cols[:K]] = 1
+        M[:, cols[K:]] = rng.random((K, n - K)) < p
+        for w in range(stride):
+            blk = M[:, 64 * w:64 * w + 64].astype(np.uint64)
+            gamma[g, :, w] = (blk << np.arange(blk.shape[1], dtype=np.uint64)).sum(1)
+    check = rng.integers(0, 2**64, size=(kcheck, stride), dtype=np.uint64) if kcheck else None
+    return gamma, check
+
+
+@pytest.mark.parametrize("thin", ["256", "100000000"])
+def test_fixed_set_lists_on_expanded_levels(hip, thin, monkeypatch):
+    """Levels large enough that a short list (fewer than 64 subsets) is expanded into fixed
+    row sets, on a matrix test (sparse generators: the cost model picks the int8 or fp4
+    test at a bound just above the level's minimum): the children with a short list run
+    as fixed-set-list rectangles on the VALU kernel (every (fixed set, subset) pair on the
+    stream). With the bound one above the minimum, the minimum itself must be found: a
+    missed or doubly counted pair, a wrong fixed set or its syndrome, shows as a wrong
+    result."""
+    monkeypatch.setenv("QSA_MATRIX_MIN_WORK", "0")
+    monkeypatch.setenv("QSA_THIN_BELOW", thin)
+    rng = np.random.default_rng(4242)
+    n = 192
+    seen_side = seen_matrix = False
+    for K, d, ng, p, kc in [(80, 6, 1, 0.03, 0), (96, 5, 1, 0.03, 2), (72, 6, 2, 0.04, 0), (88, 5, 2, 0.03, 1),
+                            (100, 5, 1, 0.03, 0)]:
+        gamma, check = sparse_plan(rng, K, n, ng, p, kc)
+        low = O.enumerate_plan(gamma, n, d, check=check)
+        for best in (low + 1, low + 4, NONE):
+            want = O.enumerate_plan(gamma, n, d, check=check, current_best=best)
+            pick = hip.plan_variant(gamma, n, d, check=check, current_best=best)
+            got = hip.enumerate_level(gamma, n, d, check=check, current_best=best)
+            st = hip.last_stats()
+            assert got == want, (K, d, ng, best, thin, got, want, st)
+            assert st["side_codewords"] <= st["codewords"], st
+            if pick[3] > 0:
+                seen_matrix = True
+                seen_side = seen_side or st["side_codewords"] > 0
+    assert seen_matrix
+    assert seen_side, "no matrix-test level ran products on the VALU kernel"
