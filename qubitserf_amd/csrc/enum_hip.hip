@@ -198,21 +198,33 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         const u64 b0 = bx * (u64)(64 * RPL) + (u64)lane * RPL;
         const Mask m0 = (b0 < R.n_lane) ? colex_unrank_any<Mask>(b0, lk, lm, sbin, a.binom, a.bk) : Mask{};
         {
+            // the first element's rows XORed in full, each next one by the rows that leave
+            // or enter between colex neighbours (usually two)
             Mask m = m0;
+            u32 acc[XD];
+#pragma unroll
+            for (int f = 0; f < XD; ++f) acc[f] = 0;
+            for (Mask mm = m; nonzero(mm);) {
+                const int b = take_low(mm);
+#pragma unroll
+                for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(lbase + b) * XD + f];
+            }
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
                 const bool valid = b0 + k < R.n_lane;
 #pragma unroll
-                for (int f = 0; f < XD; ++f) xw[k][f] = 0;
+                for (int f = 0; f < XD; ++f) xw[k][f] = valid ? acc[f] : 0u;
                 if (valid) {
-                    Mask mm = m;
-                    while (nonzero(mm)) {
-                        const int b = take_low(mm);
-#pragma unroll
-                        for (int f = 0; f < XD; ++f) xw[k][f] ^= rows_fast[(lbase + b) * XD + f];
-                    }
                     xoff[k] = pop_and(m, lflags);
-                    if (b0 + k + 1 < R.n_lane && lk > 0) m = colex_next(m);
+                    if (b0 + k + 1 < R.n_lane && lk > 0) {
+                        const Mask nm = colex_next(m);
+                        for (Mask mm = mask_xor(nm, m); nonzero(mm);) {
+                            const int b = take_low(mm);
+#pragma unroll
+                            for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(lbase + b) * XD + f];
+                        }
+                        m = nm;
+                    }
                 } else {
                     xoff[k] = kBigOffset;
                 }
@@ -248,26 +260,35 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                         foffe = (int)fk[RW];
                     };
                     if (flist) load_fix(fi);
-                    for (int i = i0; i < i1; ++i) {
-                        SElem<XD> e;
+                    u32 acc[XD];   // the XOR of the current subset's rows, kept by successor steps
 #pragma unroll
-                        for (int f = 0; f < XD; ++f) e.w[f] = fwe[f];
-                        Mask mm = m;
+                    for (int f = 0; f < XD; ++f) acc[f] = 0;
+                    auto xor_rows = [&](Mask mm) {
                         while (nonzero(mm)) {
                             const int b = take_low(mm);
 #pragma unroll
-                            for (int f = 0; f < XD; ++f) e.w[f] ^= rows_fast[(sbase + b) * XD + f];
+                            for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(sbase + b) * XD + f];
                         }
+                    };
+                    xor_rows(m);
+                    for (int i = i0; i < i1; ++i) {
+                        SElem<XD> e;
+#pragma unroll
+                        for (int f = 0; f < XD; ++f) e.w[f] = fwe[f] ^ acc[f];
                         e.off = foffe + pop_and(m, sflags);
                         stream[i] = e;
                         if (i + 1 < i1) {
                             if (flist && si + 1 == ns) {   // next fixed set, first subset
                                 ++fi;
                                 si = 0;
-                                m = colex_first<Mask>(sk);
+                                const Mask nm = colex_first<Mask>(sk);
+                                xor_rows(mask_xor(nm, m));
+                                m = nm;
                                 load_fix(fi);
                             } else {
-                                m = colex_next(m);
+                                const Mask nm = colex_next(m);
+                                xor_rows(mask_xor(nm, m));
+                                m = nm;
                                 ++si;
                             }
                         }

@@ -19,6 +19,8 @@ using u64 = std::uint64_t;
 struct M2 {
     u64 lo, hi;
 };
+QSA_HD u64 mask_xor(u64 a, u64 b) { return a ^ b; }
+QSA_HD M2 mask_xor(M2 a, M2 b) { return M2{a.lo ^ b.lo, a.hi ^ b.hi}; }
 QSA_HD bool nonzero(u64 m) { return m != 0; }
 QSA_HD bool nonzero(M2 m) { return (m.lo | m.hi) != 0; }
 QSA_HD int pop_and(u64 m, u64 f) { return __builtin_popcountll(m & f); }
