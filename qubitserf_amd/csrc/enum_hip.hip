@@ -2474,12 +2474,15 @@ struct Engine {
             // level (K = 94, d = 9: 1.2e3 rectangles at 15% padding instead of 7.4e4 at 14%),
             // while every rectangle costs host time, upload and a longer tile search
             const int max_depth = matrix ? 1 : 3;
-            // A product with a very short list (fewer than 16 subsets: a level's ka = 0 or
-            // ka = d term, one list of C(h, d) against the empty subset) is split down to
-            // 2^16 codewords: unsplit, the VALU kernel would test each lane block against
-            // a stream of a few elements, building 768 lane elements per few thousand
-            // codewords (C(28, 8) x 1: 3.1e6 codewords in 4048 such blocks).
-            const u64 floor_cw = ns < 16 ? (1ull << 16) : (1ull << 22);
+            // A product against a single subset (a level's ka = 0 or ka = d term: one list of
+            // C(h, d) against the empty subset) is split down to 2^16 codewords: unsplit, the
+            // VALU kernel tests each lane block against a one-element stream, building 768
+            // lane elements per 768 codewords (C(28, 8) x 1: 3.1e6 codewords in 4048 such
+            // blocks). Its halves are products of two lists, and splitting it makes no more
+            // rectangles than halves. A short list of several subsets is left whole: its
+            // split multiplies the rectangles by its length, each with a few hundred
+            // codewords.
+            const u64 floor_cw = ns == 1 ? (1ull << 16) : (1ull << 22);
             if (ns >= kExpandBelow || bk == 0 || bm < 2 || depth >= max_depth || sat_mul(nx, ny) < floor_cw) {
                 emit(f, xb, xm, xk, yb, ym, yk, nx, ny);
                 return;
