@@ -158,17 +158,16 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
             }
         }
         struct {
-            u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream, lane_blocks, lane_step;
+            u64 tile_base, stream_groups, stream_chunks, n_lane, n_stream;
             int fixed, nfix;
         } R = {uni64(Rr.tile_base), uni64(Rr.stream_groups), uni64(Rr.stream_chunks), uni64(Rr.n_lane),
-               uni64(Rr.n_stream), uni64(Rr.lane_blocks), uni64(Rr.lane_step), uni(Rr.fixed), uni(Rr.nfix)};
+               uni64(Rr.n_stream), uni(Rr.fixed), uni(Rr.nfix)};
         // a list of fixed sets: the stream is (fixed set, subset) pairs, ns subsets each
         const bool flist = R.nfix > 1;
         const u64 ns = flist ? R.n_stream / (u64)R.nfix : R.n_stream;
         const u64 local = t - R.tile_base;
-        const u64 bt = local / R.stream_groups;
-        const u64 bg = local - bt * R.stream_groups;
-        const u64 bx0 = bt * R.lane_step, bx1 = min(bx0 + R.lane_step, R.lane_blocks);
+        const u64 bx = local / R.stream_groups;
+        const u64 bg = local - bx * R.stream_groups;
         const int g = uni(Rr.g);
 
         const int lk = uni(Rr.lk), lbase = uni(Rr.lbase), lm = uni(Rr.lm);
@@ -193,240 +192,235 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
         }
         __syncthreads();   // rows_fast visible
 
-        // ---- the tile's lane blocks (several only when the stream is one short chunk,
-        // built by the first block and kept for the rest)
-        for (u64 bx = bx0; bx < bx1; ++bx) {
-            // ---- register tile: RPL consecutive list elements per lane, built once per block
-            u32 xw[RPL][XD];
-            int xoff[RPL];
-            const u64 b0 = bx * (u64)(64 * RPL) + (u64)lane * RPL;
-            const Mask m0 = (b0 < R.n_lane) ? colex_unrank_any<Mask>(b0, lk, lm, sbin, a.binom, a.bk) : Mask{};
-            {
-                // the first element's rows XORed in full, each next one by the rows that leave
-                // or enter between colex neighbours (usually two)
-                Mask m = m0;
-                u32 acc[XD];
+        // ---- register tile: RPL consecutive list elements per lane, built once per tile
+        u32 xw[RPL][XD];
+        int xoff[RPL];
+        const u64 b0 = bx * (u64)(64 * RPL) + (u64)lane * RPL;
+        const Mask m0 = (b0 < R.n_lane) ? colex_unrank_any<Mask>(b0, lk, lm, sbin, a.binom, a.bk) : Mask{};
+        {
+            // the first element's rows XORed in full, each next one by the rows that leave
+            // or enter between colex neighbours (usually two)
+            Mask m = m0;
+            u32 acc[XD];
 #pragma unroll
-                for (int f = 0; f < XD; ++f) acc[f] = 0;
-                for (Mask mm = m; nonzero(mm);) {
-                    const int b = take_low(mm);
+            for (int f = 0; f < XD; ++f) acc[f] = 0;
+            for (Mask mm = m; nonzero(mm);) {
+                const int b = take_low(mm);
 #pragma unroll
-                    for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(lbase + b) * XD + f];
-                }
+                for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(lbase + b) * XD + f];
+            }
 #pragma unroll
-                for (int k = 0; k < RPL; ++k) {
-                    const bool valid = b0 + k < R.n_lane;
+            for (int k = 0; k < RPL; ++k) {
+                const bool valid = b0 + k < R.n_lane;
 #pragma unroll
-                    for (int f = 0; f < XD; ++f) xw[k][f] = valid ? acc[f] : 0u;
-                    if (valid) {
-                        xoff[k] = pop_and(m, lflags);
-                        if (b0 + k + 1 < R.n_lane && lk > 0) {
-                            const Mask nm = colex_next(m);
-                            for (Mask mm = mask_xor(nm, m); nonzero(mm);) {
-                                const int b = take_low(mm);
+                for (int f = 0; f < XD; ++f) xw[k][f] = valid ? acc[f] : 0u;
+                if (valid) {
+                    xoff[k] = pop_and(m, lflags);
+                    if (b0 + k + 1 < R.n_lane && lk > 0) {
+                        const Mask nm = colex_next(m);
+                        for (Mask mm = mask_xor(nm, m); nonzero(mm);) {
+                            const int b = take_low(mm);
 #pragma unroll
-                                for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(lbase + b) * XD + f];
-                            }
-                            m = nm;
+                            for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(lbase + b) * XD + f];
                         }
-                    } else {
-                        xoff[k] = kBigOffset;
+                        m = nm;
+                    }
+                } else {
+                    xoff[k] = kBigOffset;
+                }
+            }
+        }
+
+        const u64 ch0 = bg * (u64)a.G;
+        const u64 ch1 = min(ch0 + (u64)a.G, R.stream_chunks);
+        for (u64 ch = ch0; ch < ch1; ++ch) {
+            // ---- stream chunk -> LDS (each lane a contiguous run, colex successor steps)
+            const u64 c0 = ch * (u64)a.S;
+            const int len = (int)min((u64)a.S, R.n_stream - c0);
+            if (ch != ch0) __syncthreads();   // previous chunk's readers are done
+            const int per = (len + 63) >> 6;
+            {
+                const int i0 = lane * per;
+                const int i1 = min(len, i0 + per);
+                if (i0 < i1) {
+                    const u64 e0 = c0 + (u64)i0;
+                    int fi = flist ? (int)(e0 / ns) : 0;         // this run's fixed set
+                    u64 si = flist ? e0 - (u64)fi * ns : e0;     // and its subset's rank
+                    Mask m = colex_unrank_any<Mask>(si, sk, sm, sbin, a.binom, a.bk);
+                    run_start[lane] = m;
+                    run_fix[lane] = fi;
+                    u32 fwe[XD];
+                    int foffe = foff;
+#pragma unroll
+                    for (int f = 0; f < XD; ++f) fwe[f] = fw[f];
+                    auto load_fix = [&](int k) {   // fixed set k of the rectangle's list
+                        const u32* fk = J->fixed + (size_t)(R.fixed + k) * (RW + 1);
+#pragma unroll
+                        for (int f = 0; f < XD; ++f) fwe[f] = f < RD ? fk[f] : 0u;
+                        foffe = (int)fk[RW];
+                    };
+                    if (flist) load_fix(fi);
+                    u32 acc[XD];   // the XOR of the current subset's rows, kept by successor steps
+#pragma unroll
+                    for (int f = 0; f < XD; ++f) acc[f] = 0;
+                    auto xor_rows = [&](Mask mm) {
+                        while (nonzero(mm)) {
+                            const int b = take_low(mm);
+#pragma unroll
+                            for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(sbase + b) * XD + f];
+                        }
+                    };
+                    xor_rows(m);
+                    for (int i = i0; i < i1; ++i) {
+                        SElem<XD> e;
+#pragma unroll
+                        for (int f = 0; f < XD; ++f) e.w[f] = fwe[f] ^ acc[f];
+                        e.off = foffe + pop_and(m, sflags);
+                        stream[i] = e;
+                        if (i + 1 < i1) {
+                            if (flist && si + 1 == ns) {   // next fixed set, first subset
+                                ++fi;
+                                si = 0;
+                                const Mask nm = colex_first<Mask>(sk);
+                                xor_rows(mask_xor(nm, m));
+                                m = nm;
+                                load_fix(fi);
+                            } else {
+                                const Mask nm = colex_next(m);
+                                xor_rows(mask_xor(nm, m));
+                                m = nm;
+                                ++si;
+                            }
+                        }
                     }
                 }
             }
+            __syncthreads();   // stream chunk visible
 
-            const u64 ch0 = bg * (u64)a.G;
-            const u64 ch1 = min(ch0 + (u64)a.G, R.stream_chunks);
-            for (u64 ch = ch0; ch < ch1; ++ch) {
-                // ---- stream chunk -> LDS (each lane a contiguous run, colex successor steps)
-                const u64 c0 = ch * (u64)a.S;
-                const int len = (int)min((u64)a.S, R.n_stream - c0);
-                const int per = (len + 63) >> 6;
-                if (bx == bx0 || ch1 - ch0 > 1) {   // else the one chunk is in LDS already
-                    if (ch != ch0 || bx != bx0) __syncthreads();   // previous chunk's readers are done
-                    const int i0 = lane * per;
-                    const int i1 = min(len, i0 + per);
-                    if (i0 < i1) {
-                        const u64 e0 = c0 + (u64)i0;
-                        int fi = flist ? (int)(e0 / ns) : 0;         // this run's fixed set
-                        u64 si = flist ? e0 - (u64)fi * ns : e0;     // and its subset's rank
-                        Mask m = colex_unrank_any<Mask>(si, sk, sm, sbin, a.binom, a.bk);
-                        run_start[lane] = m;
-                        run_fix[lane] = fi;
-                        u32 fwe[XD];
-                        int foffe = foff;
+            // ---- hot loop: every (lane element, stream element) pair. Two steps per trip
+            // with two element registers: the LDS read of the next element is in flight
+            // while the current one is tested, without register copies.
+            auto step = [&](const u32 (&ew)[FD], const int eoff, const int s) {
+                int acc = kBigOffset;
 #pragma unroll
-                        for (int f = 0; f < XD; ++f) fwe[f] = fw[f];
-                        auto load_fix = [&](int k) {   // fixed set k of the rectangle's list
-                            const u32* fk = J->fixed + (size_t)(R.fixed + k) * (RW + 1);
+                for (int k = 0; k < RPL; ++k) {
+                    int tk = xoff[k];
 #pragma unroll
-                            for (int f = 0; f < XD; ++f) fwe[f] = f < RD ? fk[f] : 0u;
-                            foffe = (int)fk[RW];
-                        };
-                        if (flist) load_fix(fi);
-                        u32 acc[XD];   // the XOR of the current subset's rows, kept by successor steps
-#pragma unroll
-                        for (int f = 0; f < XD; ++f) acc[f] = 0;
-                        auto xor_rows = [&](Mask mm) {
-                            while (nonzero(mm)) {
-                                const int b = take_low(mm);
-#pragma unroll
-                                for (int f = 0; f < XD; ++f) acc[f] ^= rows_fast[(sbase + b) * XD + f];
-                            }
-                        };
-                        xor_rows(m);
-                        for (int i = i0; i < i1; ++i) {
-                            SElem<XD> e;
-#pragma unroll
-                            for (int f = 0; f < XD; ++f) e.w[f] = fwe[f] ^ acc[f];
-                            e.off = foffe + pop_and(m, sflags);
-                            stream[i] = e;
-                            if (i + 1 < i1) {
-                                if (flist && si + 1 == ns) {   // next fixed set, first subset
-                                    ++fi;
-                                    si = 0;
-                                    const Mask nm = colex_first<Mask>(sk);
-                                    xor_rows(mask_xor(nm, m));
-                                    m = nm;
-                                    load_fix(fi);
-                                } else {
-                                    const Mask nm = colex_next(m);
-                                    xor_rows(mask_xor(nm, m));
-                                    m = nm;
-                                    ++si;
-                                }
-                            }
-                        }
-                    }
+                    for (int f = 0; f < FD; ++f) tk += __popc(xw[k][f] ^ ew[f]);
+                    acc = min(acc, tk);
                 }
-                __syncthreads();   // stream chunk visible
-
-                // ---- hot loop: every (lane element, stream element) pair. Two steps per trip
-                // with two element registers: the LDS read of the next element is in flight
-                // while the current one is tested, without register copies.
-                auto step = [&](const u32 (&ew)[FD], const int eoff, const int s) {
-                    int acc = kBigOffset;
+                const int thr = bound - eoff;
+                if (__builtin_expect(__ballot(acc < thr) != 0ull, 0)) {
+                    // ---- medium test: all XD register dwords
+                    u32 ex[XD];
+#pragma unroll
+                    for (int f = 0; f < XD; ++f) ex[f] = stream[s].w[f];
+                    u32 cand = 0;
 #pragma unroll
                     for (int k = 0; k < RPL; ++k) {
                         int tk = xoff[k];
 #pragma unroll
-                        for (int f = 0; f < FD; ++f) tk += __popc(xw[k][f] ^ ew[f]);
-                        acc = min(acc, tk);
+                        for (int f = 0; f < XD; ++f) tk += __popc(xw[k][f] ^ ex[f]);
+                        if (tk < thr) cand |= 1u << k;
                     }
-                    const int thr = bound - eoff;
-                    if (__builtin_expect(__ballot(acc < thr) != 0ull, 0)) {
-                        // ---- medium test: all XD register dwords
-                        u32 ex[XD];
-#pragma unroll
-                        for (int f = 0; f < XD; ++f) ex[f] = stream[s].w[f];
-                        u32 cand = 0;
-#pragma unroll
-                        for (int k = 0; k < RPL; ++k) {
-                            int tk = xoff[k];
-#pragma unroll
-                            for (int f = 0; f < XD; ++f) tk += __popc(xw[k][f] ^ ex[f]);
-                            if (tk < thr) cand |= 1u << k;
+                    ++medium_entries;
+                    ++job_hits;
+                    if (__ballot(cand != 0) == 0ull) return;
+                    // ---- slow path: exact weight and logical test of the light candidates
+                    ++slow_entries;
+                    const int owner = s / per;
+                    Mask smask = run_start[owner];
+                    const u32* fxe = fx;   // the candidate stream element's fixed set
+                    if (flist) {
+                        int fi = run_fix[owner];
+                        u64 si = c0 + (u64)(owner * per) - (u64)fi * ns;
+                        for (int j = owner * per; j < s; ++j) {
+                            if (si + 1 == ns) { ++fi; si = 0; smask = colex_first<Mask>(sk); }
+                            else { smask = colex_next(smask); ++si; }
                         }
-                        ++medium_entries;
-                        ++job_hits;
-                        if (__ballot(cand != 0) == 0ull) return;
-                        // ---- slow path: exact weight and logical test of the light candidates
-                        ++slow_entries;
-                        const int owner = s / per;
-                        Mask smask = run_start[owner];
-                        const u32* fxe = fx;   // the candidate stream element's fixed set
-                        if (flist) {
-                            int fi = run_fix[owner];
-                            u64 si = c0 + (u64)(owner * per) - (u64)fi * ns;
-                            for (int j = owner * per; j < s; ++j) {
-                                if (si + 1 == ns) { ++fi; si = 0; smask = colex_first<Mask>(sk); }
-                                else { smask = colex_next(smask); ++si; }
-                            }
-                            fxe = J->fixed + (size_t)(R.fixed + fi) * (RW + 1);
-                        } else {
-                            for (int j = owner * per; j < s; ++j) smask = colex_next(smask);
+                        fxe = J->fixed + (size_t)(R.fixed + fi) * (RW + 1);
+                    } else {
+                        for (int j = owner * per; j < s; ++j) smask = colex_next(smask);
+                    }
+                    if (lane < RW) {
+                        u32 v = fxe ? fxe[lane] : 0u;
+                        Mask mm = smask;
+                        while (nonzero(mm)) v ^= slow_tbl[(sbase + take_low(mm)) * RW + lane];
+                        sfull[lane] = v;
+                    }
+                    __syncthreads();
+                    int mine = QSF_WEIGHT_NONE;
+                    while (cand) {
+                        const int k = __builtin_ctz(cand);
+                        cand &= cand - 1;
+                        Mask lmask = m0;
+                        for (int j = 0; j < k; ++j) lmask = colex_next(lmask);
+                        int w = pop_and(lmask, lflags) + eoff;
+                        for (int f = 0; f < RD && w < bound; ++f) {
+                            u32 v = sfull[f];
+                            Mask mm = lmask;
+                            while (nonzero(mm)) v ^= slow_tbl[(lbase + take_low(mm)) * RW + f];
+                            w += __popc(v);
                         }
-                        if (lane < RW) {
-                            u32 v = fxe ? fxe[lane] : 0u;
-                            Mask mm = smask;
-                            while (nonzero(mm)) v ^= slow_tbl[(sbase + take_low(mm)) * RW + lane];
-                            sfull[lane] = v;
-                        }
-                        __syncthreads();
-                        int mine = QSF_WEIGHT_NONE;
-                        while (cand) {
-                            const int k = __builtin_ctz(cand);
-                            cand &= cand - 1;
-                            Mask lmask = m0;
-                            for (int j = 0; j < k; ++j) lmask = colex_next(lmask);
-                            int w = pop_and(lmask, lflags) + eoff;
-                            for (int f = 0; f < RD && w < bound; ++f) {
-                                u32 v = sfull[f];
-                                Mask mm = lmask;
-                                while (nonzero(mm)) v ^= slow_tbl[(lbase + take_low(mm)) * RW + f];
-                                w += __popc(v);
-                            }
-                            if (a.collect) {
-                                // bound = keep + 1; the lanes of this candidate round append together
-                                const unsigned long long slot = wave_append(a.out_rows, w >= 1 && w < bound);
-                                if (slot != ~0ull) {
-                                    if (slot < a.out_cap) {
-                                        const int stride = J->stride;
-                                        const u64* G = J->gamma + (size_t)g * K * stride;
-                                        for (int wd = 0; wd < stride; ++wd) {
-                                            u64 v = 0;
-                                            for (int q = 0; q < 4; ++q)
-                                                for (u64 mm = Rr.fmask[q]; mm; mm &= mm - 1)
-                                                    v ^= G[(size_t)(64 * q + __builtin_ctzll(mm)) * stride + wd];
-                                            for (Mask mm = smask; nonzero(mm);) v ^= G[(size_t)(sbase + take_low(mm)) * stride + wd];
-                                            for (Mask mm = lmask; nonzero(mm);) v ^= G[(size_t)(lbase + take_low(mm)) * stride + wd];
-                                            a.out[slot * stride + wd] = v;
-                                        }
+                        if (a.collect) {
+                            // bound = keep + 1; the lanes of this candidate round append together
+                            const unsigned long long slot = wave_append(a.out_rows, w >= 1 && w < bound);
+                            if (slot != ~0ull) {
+                                if (slot < a.out_cap) {
+                                    const int stride = J->stride;
+                                    const u64* G = J->gamma + (size_t)g * K * stride;
+                                    for (int wd = 0; wd < stride; ++wd) {
+                                        u64 v = 0;
+                                        for (int q = 0; q < 4; ++q)
+                                            for (u64 mm = Rr.fmask[q]; mm; mm &= mm - 1)
+                                                v ^= G[(size_t)(64 * q + __builtin_ctzll(mm)) * stride + wd];
+                                        for (Mask mm = smask; nonzero(mm);) v ^= G[(size_t)(sbase + take_low(mm)) * stride + wd];
+                                        for (Mask mm = lmask; nonzero(mm);) v ^= G[(size_t)(lbase + take_low(mm)) * stride + wd];
+                                        a.out[slot * stride + wd] = v;
                                     }
                                 }
-                                continue;   // next candidate of this lane
                             }
-                            if (w < bound && w < mine) {
-                                bool logical = (SD == 0);
-                                for (int f = 0; f < SD && !logical; ++f) {
-                                    u32 v = sfull[RD + f];
-                                    Mask mm = lmask;
-                                    while (nonzero(mm)) v ^= slow_tbl[(lbase + take_low(mm)) * RW + RD + f];
-                                    logical = v != 0;
-                                }
-                                if (logical) mine = w;
+                            continue;   // next candidate of this lane
+                        }
+                        if (w < bound && w < mine) {
+                            bool logical = (SD == 0);
+                            for (int f = 0; f < SD && !logical; ++f) {
+                                u32 v = sfull[RD + f];
+                                Mask mm = lmask;
+                                while (nonzero(mm)) v ^= slow_tbl[(lbase + take_low(mm)) * RW + RD + f];
+                                logical = v != 0;
                             }
+                            if (logical) mine = w;
                         }
-                        const int wm = __builtin_amdgcn_readfirstlane(wave_min(mine));
-                        if (!a.collect && wm < bound) {
-                            bound = wm;
-                            if (lane == 0) atomicMin(J->best, wm);
-                            if (lane == 0 && J->shared != J->best) atomicMin(J->shared, wm);
-                        }
-                        __syncthreads();
                     }
-                };
-                u32 aw[FD], bw[FD];
-                int aoff = 0, boff = 0;
-#pragma unroll
-                for (int f = 0; f < FD; ++f) aw[f] = stream[0].w[f];
-                aoff = stream[0].off;
-                for (int s = 0; s < len; s += 2) {
-                    if (s + 1 < len) {
-#pragma unroll
-                        for (int f = 0; f < FD; ++f) bw[f] = stream[s + 1].w[f];
-                        boff = stream[s + 1].off;
+                    const int wm = __builtin_amdgcn_readfirstlane(wave_min(mine));
+                    if (!a.collect && wm < bound) {
+                        bound = wm;
+                        if (lane == 0) atomicMin(J->best, wm);
+                        if (lane == 0 && J->shared != J->best) atomicMin(J->shared, wm);
                     }
-                    step(aw, aoff, s);
-                    if (s + 1 >= len) break;
-                    if (s + 2 < len) {
-#pragma unroll
-                        for (int f = 0; f < FD; ++f) aw[f] = stream[s + 2].w[f];
-                        aoff = stream[s + 2].off;
-                    }
-                    step(bw, boff, s + 1);
+                    __syncthreads();
                 }
-                if (bound <= floor) break;
+            };
+            u32 aw[FD], bw[FD];
+            int aoff = 0, boff = 0;
+#pragma unroll
+            for (int f = 0; f < FD; ++f) aw[f] = stream[0].w[f];
+            aoff = stream[0].off;
+            for (int s = 0; s < len; s += 2) {
+                if (s + 1 < len) {
+#pragma unroll
+                    for (int f = 0; f < FD; ++f) bw[f] = stream[s + 1].w[f];
+                    boff = stream[s + 1].off;
+                }
+                step(aw, aoff, s);
+                if (s + 1 >= len) break;
+                if (s + 2 < len) {
+#pragma unroll
+                    for (int f = 0; f < FD; ++f) aw[f] = stream[s + 2].w[f];
+                    aoff = stream[s + 2].off;
+                }
+                step(bw, boff, s + 1);
             }
             if (bound <= floor) break;
         }
@@ -2850,37 +2844,20 @@ struct Engine {
         // tile, small enough to leave >= 8 tiles per resident workgroup for load balance
         // (16 one-wave workgroups per CU for bz_tiled, 4 four-wave ones for bz_mfma)
         const u64 waves = (u64)std::max(1, cus) * 16;
-        static const bool lane_step_on = std::getenv("QSA_LANE_STEP_OFF") == nullptr;   // tuning: one block per tile
         const u64 mfma_groups = (u64)std::max(1, cus) * 4;
         std::vector<int> gG(NG, 1);
         std::vector<u64> gtiles(NG, 0);
         size_t nrect = 0;
-        // A VALU rectangle whose stream is one short chunk (the thin products' lists, tens of
-        // elements) takes up to 4 lane blocks per tile (lane_step), as many as fill about
-        // one chunk: the per-tile work (claim, rectangle lookup, job state, stream build) is
-        // about a third of such a tile's time, and the tile builds its stream once. Only
-        // while the group keeps >= 8 tiles per resident wave: a few long tiles at the end
-        // of a small group would cost more than they save.
         for (int v = 0; v < NG; ++v) {
-            const bool valu = vtab[grp_v[v]].mm == 0;
             u64 chunks = 0;
             for (const Rect& r : grect[v]) chunks += r.lane_blocks * r.stream_chunks;
-            const u64 resident = valu ? waves : mfma_groups;
+            const u64 resident = vtab[grp_v[v]].mm > 0 ? mfma_groups : waves;
             gG[v] = (int)std::max<u64>(1, std::min<u64>(16, chunks / (8 * resident)));
-            u64 step_cap = 1;
-            if (valu && lane_step_on) {
-                u64 tiles1 = 0;
-                for (const Rect& r : grect[v]) tiles1 += r.lane_blocks * ((r.stream_chunks + gG[v] - 1) / gG[v]);
-                step_cap = std::max<u64>(1, std::min<u64>(4, tiles1 / (8 * resident)));
-            }
             u64 tiles = 0;
             for (Rect& r : grect[v]) {
                 r.stream_groups = (r.stream_chunks + gG[v] - 1) / gG[v];
-                r.lane_step = 1;
-                if (step_cap > 1 && r.stream_chunks == 1 && 2 * r.n_stream <= (u64)vS[v])
-                    r.lane_step = std::min<u64>({step_cap, r.lane_blocks, (u64)vS[v] / std::max<u64>(1, r.n_stream)});
                 r.tile_base = tiles;
-                tiles += (r.lane_blocks + r.lane_step - 1) / r.lane_step * r.stream_groups;
+                tiles += r.lane_blocks * r.stream_groups;
             }
             gtiles[v] = tiles;
             nrect += grect[v].size();
@@ -2893,7 +2870,7 @@ struct Engine {
                 u64 thin = 0, lane_thin = 0;
                 double real = 0, fill = 0;
                 for (const Rect& r : grect[v]) {
-                    const u64 t = (r.lane_blocks + r.lane_step - 1) / r.lane_step * r.stream_groups;
+                    const u64 t = r.lane_blocks * r.stream_groups;
                     if (r.n_stream < 128) thin += t;
                     if ((double)r.n_lane < lt) lane_thin += t;
                     real += (double)r.n_lane * (double)r.n_stream;

@@ -71,10 +71,7 @@ struct Rect {
     u64 n_lane, n_stream;  // list lengths C(lm, lk), C(sm, sk) (x nfix on the stream side)
     u64 lane_blocks;       // ceil(n_lane / (64 * RPL))
     u64 stream_chunks;     // ceil(n_stream / S)
-    u64 stream_groups;     // ceil(stream_chunks / G): a tile is lane_step lane blocks x G chunks
-    // lane blocks per tile: 1, or more (VALU kernel) when the stream is one short chunk,
-    // which the tile then builds once for all its lane blocks
-    u64 lane_step;
+    u64 stream_groups;     // ceil(stream_chunks / G): a tile is one lane block x G chunks
     u64 tile_base;         // index of this rectangle's first tile
 };
 
