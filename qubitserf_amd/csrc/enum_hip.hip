@@ -2458,12 +2458,12 @@ struct Engine {
             rv.push_back(r);
         }
 
-        void split(const RowSet& f, int xb, int xm, int xk, int yb, int ym, int yk, int depth) {
-            const u64 nx = s.C(xm, xk), ny = s.C(ym, yk);
-            if (nx == 0 || ny == 0) return;
+        // Whether split() emits the product of xk-subsets of xm rows and yk-subsets of ym
+        // rows (nx and ny subsets) whole at this depth, rather than expanding it.
+        bool whole(u64 nx, u64 ny, int xm, int xk, int ym, int yk, int depth) const {
             const bool x_small = nx <= ny;
             const u64 ns = x_small ? nx : ny;
-            const int bb = x_small ? yb : xb, bm = x_small ? ym : xm, bk = x_small ? yk : xk;
+            const int bm = x_small ? ym : xm, bk = x_small ? yk : xk;
             // the matrix tests' 1024 x 512 tiles gain little from splitting below the first
             // level (K = 94, d = 9: 1.2e3 rectangles at 15% padding instead of 7.4e4 at 14%),
             // while every rectangle costs host time, upload and a longer tile search
@@ -2477,18 +2477,26 @@ struct Engine {
             // split multiplies the rectangles by its length, each with a few hundred
             // codewords.
             const u64 floor_cw = ns == 1 ? (1ull << 16) : (1ull << 22);
-            if (ns >= kExpandBelow || bk == 0 || bm < 2 || depth >= max_depth || sat_mul(nx, ny) < floor_cw) {
+            return ns >= kExpandBelow || bk == 0 || bm < 2 || depth >= max_depth || sat_mul(nx, ny) < floor_cw;
+        }
+
+        void split(const RowSet& f, int xb, int xm, int xk, int yb, int ym, int yk, int depth) {
+            const u64 nx = s.C(xm, xk), ny = s.C(ym, yk);
+            if (nx == 0 || ny == 0) return;
+            if (whole(nx, ny, xm, xk, ym, yk, depth)) {
                 emit(f, xb, xm, xk, yb, ym, yk, nx, ny);
                 return;
             }
             // small side, enumerated: fewer than 64 subsets, so its range has fewer than 64
-            // rows (or the subset is the whole range, of sk <= 32 rows)
+            // rows (or the subset is the whole range, of sk <= 32 rows); the big side split
+            // in halves
+            const bool x_small = nx <= ny;
+            const u64 ns = x_small ? nx : ny;
+            const int bb = x_small ? yb : xb, bm = x_small ? ym : xm, bk = x_small ? yk : xk;
             const int sb = x_small ? xb : yb, sk = x_small ? xk : yk;
             const int mid = bm / 2;
-            if (flist_below > 0 && ns > 1 && depth + 1 >= max_depth) {
-                // the children are emitted as they are: those whose shorter list is short
-                // take every fixed set of the small side in one rectangle
-                std::vector<RowSet> fs(ns, f);
+            std::vector<RowSet> fs(ns, f);   // f plus each subset of the small side, colex order
+            {
                 u64 m = sk == 0 ? 0 : ((1ull << sk) - 1);
                 for (u64 e = 0; e < ns; ++e) {
                     for (u64 mm = m; mm; mm &= mm - 1) {
@@ -2500,28 +2508,18 @@ struct Engine {
                         m = (((r ^ m) >> 2) >> __builtin_ctzll(c)) | r;
                     }
                 }
-                for (int j = std::max(0, bk - (bm - mid)); j <= std::min(bk, mid); ++j) {
-                    const u64 a2 = s.C(mid, j), b2 = s.C(bm - mid, bk - j);
-                    if (a2 == 0 || b2 == 0) continue;
-                    if (std::min(a2, b2) < flist_below)
-                        emit_flist(f, fs, bb, mid, j, bb + mid, bm - mid, bk - j, a2, b2);
-                    else
-                        for (const RowSet& fe : fs) emit(fe, bb, mid, j, bb + mid, bm - mid, bk - j, a2, b2);
-                }
-                return;
             }
-            u64 m = sk == 0 ? 0 : ((1ull << sk) - 1);
-            for (u64 e = 0; e < ns; ++e) {
-                RowSet fe = f;
-                for (u64 mm = m; mm; mm &= mm - 1) {
-                    const int row = sb + __builtin_ctzll(mm);
-                    fe.w[row >> 6] |= 1ull << (row & 63);
-                }
-                for (int j = std::max(0, bk - (bm - mid)); j <= std::min(bk, mid); ++j)
-                    split(fe, bb, mid, j, bb + mid, bm - mid, bk - j, depth + 1);
-                if (e + 1 < ns) {   // next sk-subset of [sb, sb + sm) in colex order
-                    const u64 c = m & (~m + 1ull), r = m + c;
-                    m = (((r ^ m) >> 2) >> __builtin_ctzll(c)) | r;
+            for (int j = std::max(0, bk - (bm - mid)); j <= std::min(bk, mid); ++j) {
+                const u64 a2 = s.C(mid, j), b2 = s.C(bm - mid, bk - j);
+                if (a2 == 0 || b2 == 0) continue;
+                if (!whole(a2, b2, mid, j, bm - mid, bk - j, depth + 1)) {
+                    for (const RowSet& fe : fs) split(fe, bb, mid, j, bb + mid, bm - mid, bk - j, depth + 1);
+                } else if (flist_below > 0 && ns > 1 && std::min(a2, b2) < flist_below) {
+                    // a child emitted whole whose shorter list is short: every fixed set of
+                    // the small side in one rectangle
+                    emit_flist(f, fs, bb, mid, j, bb + mid, bm - mid, bk - j, a2, b2);
+                } else {
+                    for (const RowSet& fe : fs) emit(fe, bb, mid, j, bb + mid, bm - mid, bk - j, a2, b2);
                 }
             }
         }
