@@ -74,6 +74,15 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long* ro
     return hit ? base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull)) : ~0ull;
 }
 
+// popcount(x) + acc in one instruction. The compiler forms popcount(a) + popcount(b) + c
+// as two popcounts into 0 and a three-input add; chained through the popcount's own
+// addend, the hot test's distance over FD dwords is FD XORs and FD popcounts.
+__device__ __forceinline__ int popc_acc(u32 x, int acc) {
+    int r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
 template <int FD>
 struct SElem {
     u32 w[FD];
@@ -306,7 +315,7 @@ __global__ __launch_bounds__(64) void bz_tiled(TiledArgs a) {
                 for (int k = 0; k < RPL; ++k) {
                     int tk = xoff[k];
 #pragma unroll
-                    for (int f = 0; f < FD; ++f) tk += __popc(xw[k][f] ^ ew[f]);
+                    for (int f = 0; f < FD; ++f) tk = popc_acc(xw[k][f] ^ ew[f], tk);
                     acc = min(acc, tk);
                 }
                 const int thr = bound - eoff;
