@@ -15,8 +15,8 @@ Prints ONE JSON line. Fields beyond the common contract:
                       count there). bz_mfma (int8 matrix-core hot
                       test): achieved = 64 int8 ops (32 multiply-adds) x its codewords / its
                       device time, peak = 5.0 POP/s dense int8. bz_f4 (fp4 products,
-                      32x32x64 / 16x16x128): 128 / 256 ops per codeword, peak 10.1 POP/s
-                      dense fp4. bz_tiled (VALU hot test):
+                      32x32x64 / 16x16x128 / two chained 32x32x64): 128 / 256 / 256 ops per
+                      codeword, peak 10.1 POP/s dense fp4. bz_tiled (VALU hot test):
                       achieved = sum over levels of (the variant's hot-loop lane-ops per
                       codeword x codewords) / device time (HIP events on its stream), peak =
                       256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-ops/s.
@@ -52,7 +52,8 @@ VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12     # int32 lane-ops/s, gfx950
 OPS_PER_CODEWORD = 5                                # 2 xor + 2 bcnt + 1 compare (64-bit test)
 # Matrix-core hot tests by the trace's `mfma` field: kernel, ops per codeword (2 per
 # multiply-add: 32 int8 dimensions, or 64 / 128 fp4 ones), dense MI355X peak in TOP/s.
-MATRIX = {1: ("bz_mfma", 64, 5000.0), 2: ("bz_f4_32x32x64", 128, 10100.0), 3: ("bz_f4_16x16x128", 256, 10100.0)}
+MATRIX = {1: ("bz_mfma", 64, 5000.0), 2: ("bz_f4_32x32x64", 128, 10100.0), 3: ("bz_f4_16x16x128", 256, 10100.0),
+          4: ("bz_f4_2x32x32x64", 256, 10100.0)}
 TRAFFIC_FILES = {"bz_mfma": ROOT / "profiles" / "bz_mfma_gross_traffic.json",
                  "bz_tiled": ROOT / "profiles" / "bz_tiled_gross_traffic.json"}
 

@@ -772,8 +772,11 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
 // As bz_mfma, on the fp4 matrix instructions with unit block scales:
 //   MM = 2  v_mfma_scale_f32_32x32x64_f8f6f4   32 x 32 codewords per product, 64 dimensions
 //   MM = 3  v_mfma_scale_f32_16x16x128_f8f6f4  16 x 16 codewords per product, 128 dimensions
+//   MM = 4  two 32x32x64 products chained through the accumulator, 128 dimensions
 // gfx950 runs fp4 products at twice the int8 rate, so a product tests 60 (MM = 2) or 124
-// (MM = 3) residual bits for about the time the int8 test spends on 30. The lane side holds
+// (MM = 3) residual bits for about the time the int8 test spends on 30. MM = 4 tests 124
+// bits on 32 x 32 outputs: twice the products of MM = 2 for the same reduction, with four
+// lane blocks per wave (the second product's operands take the other four's registers). The lane side holds
 // +1 / -1 (codes 0x2 / 0xA) per test bit, the stream side 1 / 0 (0x2 / 0x0). Dimensions
 // 28..31 of the last 32-dimension block carry the lane constant L = off_lane + |a| as four
 // fp4 digits (d0, d1, d2, d3) against the fixed stream values (6, 6, 6, 1), so the f32
@@ -783,9 +786,10 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
 // is the instructions' output layout, read off index products at start-up and checked (a
 // launch that finds another layout reports it and the host raises). The stream constant
 // off_stream then enters as a per-lane threshold, min(bound, 127) - off_stream.
-// Block kb of the 32-dimension blocks (lanes kb NR .. kb NR + NR - 1) holds residual word
-// kb of both elements: nibble i of dword j is bit j + 4 i, i.e. (w >> j) & 0x11111111
-// moved into place (two VALU per dword); the last word keeps its bits 0..27.
+// Block kb of the 32-dimension blocks (lanes kb NR .. kb NR + NR - 1) of product p holds
+// residual word p NKB + kb of both elements: nibble i of dword j is bit j + 4 i, i.e.
+// (w >> j) & 0x11111111 moved into place (two VALU per dword); the last word keeps its bits
+// 0..27.
 // Padding: a lane element past its list holds only the constant 114 (digits 6, 6, 6, 6), a
 // stream element past its list only zeros with off_stream = 127; the host uses these
 // kernels only while the bound is at most 100, so neither can pass.
@@ -795,24 +799,29 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// One product (NP = 1) or NP chained products (the second accumulates onto the first);
+// NB lane blocks per wave.
 template <int MM> struct F4Prod;
 template <> struct F4Prod<2> {
     typedef f32x16 acc;
-    static constexpr int NR = 32;
-    __device__ static acc mm(i32x4 a, i32x4 b) {
+    static constexpr int NR = 32, NP = 1, NB = kMfmaBlocks;
+    __device__ static acc mm(i32x4 a, i32x4 b, acc c = acc{}) {
         return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(i32x8{a[0], a[1], a[2], a[3], 0, 0, 0, 0},
-                                                               i32x8{b[0], b[1], b[2], b[3], 0, 0, 0, 0}, acc{},
+                                                               i32x8{b[0], b[1], b[2], b[3], 0, 0, 0, 0}, c,
                                                                4, 4, 0, 127, 0, 127);
     }
 };
 template <> struct F4Prod<3> {
     typedef f32x4 acc;
-    static constexpr int NR = 16;
-    __device__ static acc mm(i32x4 a, i32x4 b) {
+    static constexpr int NR = 16, NP = 1, NB = kMfmaBlocks;
+    __device__ static acc mm(i32x4 a, i32x4 b, acc c = acc{}) {
         return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(i32x8{a[0], a[1], a[2], a[3], 0, 0, 0, 0},
-                                                                i32x8{b[0], b[1], b[2], b[3], 0, 0, 0, 0}, acc{},
+                                                                i32x8{b[0], b[1], b[2], b[3], 0, 0, 0, 0}, c,
                                                                 4, 4, 0, 127, 0, 127);
     }
+};
+template <> struct F4Prod<4> : F4Prod<2> {
+    static constexpr int NP = 2, NB = f4_blocks(4);
 };
 
 // fp4 e2m1 code of v in {-1, 0, 1, 2, 3, 4, 6}
@@ -839,13 +848,17 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     typedef typename P::acc acc;
     constexpr int NR = P::NR;                     // rows (lane elements) = columns (stream elements) per product
     constexpr int NKB = 64 / NR;                  // 32-dimension blocks across the lanes
+    constexpr int NP = P::NP;                     // chained products per lane block
+    constexpr int NW = NKB * NP;                  // residual words tested
+    constexpr int NB = P::NB;                     // lane blocks per wave
     constexpr int NV = NR * NR / 64;              // product values per lane
-    constexpr int LW = kMfmaBlocks * NR;          // lane elements per wave
+    constexpr int LW = NB * NR;                   // lane elements per wave
     constexpr int SL = f4_stream(MM);             // stream chunk length
     constexpr int EPT = SL / (64 * kMfmaWaves);   // stream elements each thread makes per chunk
-    static_assert(XD >= NKB, "the held residual dwords must cover the test bits");
+    static_assert(XD >= NW, "the held residual dwords must cover the test bits");
+    static_assert(NB * NV <= 128, "candidate bits of a group");
     __shared__ u32 rows_fast[kMaxMatrixK * XD];
-    __shared__ i32x4 sbytes[NKB * SL];            // block kb of chunk position e at [kb * SL + e]
+    __shared__ i32x4 sbytes[NW * SL];             // word w of chunk position e at [w * SL + e]
     __shared__ float soff[SL];                    // off_stream of each chunk position (127: padding)
     __shared__ u64 smk[SL];                       // the chunk's stream-element subsets
     __shared__ u64 lmk[kMfmaWaves * LW];          // each wave's lane-element subsets
@@ -952,46 +965,49 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         __syncthreads();
 
         // ---- this wave's LW lane elements: row r of block b is element
-        // bx * (4 LW) + LW wave + 8 r + b (the NKB lanes of a row share it)
-        i32x4 av[kMfmaBlocks];
+        // bx * (4 LW) + LW wave + NB r + b (the NKB lanes of a row share it)
+        i32x4 av[NB][NP];
         u64* wlmk = lmk + LW * wave;
         {
-            const u64 first = bx * (u64)(kMfmaWaves * LW) + (u64)(LW * wave + kMfmaBlocks * row);
+            const u64 first = bx * (u64)(kMfmaWaves * LW) + (u64)(LW * wave + NB * row);
             u64 mnext = (first < n_lane && lk) ? colex_unrank_any<u64>(first, lk, lm, sbin, a.binom, a.bk) : 0ull;
 #pragma unroll
-            for (int b = 0; b < kMfmaBlocks; ++b) {
+            for (int b = 0; b < NB; ++b) {
                 const u64 idx = first + (u64)b;
                 const bool valid = idx < n_lane;
                 const u64 m = valid ? mnext : 0ull;
                 if (valid && lk && idx + 1 < n_lane) mnext = colex_next(mnext);
-                if (kb == 0) wlmk[kMfmaBlocks * row + b] = m;
-                u32 x[NKB];
+                if (kb == 0) wlmk[NB * row + b] = m;
+                u32 x[NW];
 #pragma unroll
-                for (int w = 0; w < NKB; ++w) x[w] = 0u;
+                for (int w = 0; w < NW; ++w) x[w] = 0u;
                 for (u64 mm = m; mm; mm &= mm - 1) {
                     const int r = lbase + __builtin_ctzll(mm);
 #pragma unroll
-                    for (int w = 0; w < NKB; ++w) x[w] ^= rows_fast[r * XD + w];
+                    for (int w = 0; w < NW; ++w) x[w] ^= rows_fast[r * XD + w];
                 }
-                x[NKB - 1] &= 0x0fffffffu;
+                x[NW - 1] &= 0x0fffffffu;
                 int L = __popcll(m & lflags);
-                u32 xk = x[0];
 #pragma unroll
-                for (int w = 0; w < NKB; ++w) {
-                    L += __popc(x[w]);
-                    if (kb == w) xk = x[w];
+                for (int w = 0; w < NW; ++w) L += __popc(x[w]);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    u32 xk = x[p * NKB];
+#pragma unroll
+                    for (int w = 1; w < NKB; ++w)
+                        if (kb == w) xk = x[p * NKB + w];
+                    u32 q[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) q[j] = 0x22222222u | (((xk >> j) & 0x11111111u) << 3);
+                    if (p == NP - 1 && kb == NKB - 1) {
+                        const u32 c = valid ? f4_lane_const(L) : 0x7777u;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) q[j] = (valid ? (q[j] & 0x0fffffffu) : 0u) | (((c >> (4 * j)) & 15u) << 28);
+                    } else if (!valid) {
+                        q[0] = q[1] = q[2] = q[3] = 0u;
+                    }
+                    av[b][p] = i32x4{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
                 }
-                u32 q[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) q[j] = 0x22222222u | (((xk >> j) & 0x11111111u) << 3);
-                if (kb == NKB - 1) {
-                    const u32 c = valid ? f4_lane_const(L) : 0x7777u;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) q[j] = (valid ? (q[j] & 0x0fffffffu) : 0u) | (((c >> (4 * j)) & 15u) << 28);
-                } else if (!valid) {
-                    q[0] = q[1] = q[2] = q[3] = 0u;
-                }
-                av[b] = i32x4{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
             }
         }
 
@@ -1005,13 +1021,13 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         const u64 t_real = min((u64)(64 * kMfmaWaves), (n_stream - s0 + (u64)(EPT * nch) - 1) / (u64)(EPT * nch));
         const int gmax = uni((int)min((u64)(SL / NR), (EPT * t_real + NR - 1) / NR));
         u64 smask = (sidx < n_stream && sk) ? colex_unrank_any<u64>(sidx, sk, sm, sbin, a.binom, a.bk) : 0ull;
-        u32 sx[NKB];
+        u32 sx[NW];
 #pragma unroll
-        for (int w = 0; w < NKB; ++w) sx[w] = (fx && w < RD) ? fx[w] : 0u;
+        for (int w = 0; w < NW; ++w) sx[w] = (fx && w < RD) ? fx[w] : 0u;
         for (u64 mm = smask; mm; mm &= mm - 1) {
             const int r = sbase + __builtin_ctzll(mm);
 #pragma unroll
-            for (int w = 0; w < NKB; ++w) sx[w] ^= rows_fast[r * XD + w];
+            for (int w = 0; w < NW; ++w) sx[w] ^= rows_fast[r * XD + w];
         }
 
         for (int k = 0; k < nch; ++k) {
@@ -1026,12 +1042,12 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 const int pos = EPT * tid + e;
                 if (sidx < n_stream) {
 #pragma unroll
-                    for (int w = 0; w < NKB; ++w) {
-                        const u32 sw = w == NKB - 1 ? (sx[w] & 0x0fffffffu) : sx[w];
+                    for (int w = 0; w < NW; ++w) {
+                        const u32 sw = w == NW - 1 ? (sx[w] & 0x0fffffffu) : sx[w];
                         u32 q[4];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) q[j] = ((sw >> j) & 0x11111111u) << 1;
-                        if (w == NKB - 1) {   // the fixed values (6, 6, 6, 1) of the lane-constant dimensions
+                        if (w == NW - 1) {   // the fixed values (6, 6, 6, 1) of the lane-constant dimensions
                             q[0] |= 7u << 28; q[1] |= 7u << 28; q[2] |= 7u << 28; q[3] |= 2u << 28;
                         }
                         sbytes[w * SL + pos] = i32x4{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
@@ -1043,14 +1059,14 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                         for (u64 mm = nm ^ smask; mm; mm &= mm - 1) {
                             const int r = sbase + __builtin_ctzll(mm);
 #pragma unroll
-                            for (int w = 0; w < NKB; ++w) sx[w] ^= rows_fast[r * XD + w];
+                            for (int w = 0; w < NW; ++w) sx[w] ^= rows_fast[r * XD + w];
                         }
                         smask = nm;
                     }
                     ++sidx;
                 } else {   // past the list: zeros, and a threshold no product reaches
 #pragma unroll
-                    for (int w = 0; w < NKB; ++w) sbytes[w * SL + pos] = i32x4{0, 0, 0, 0};
+                    for (int w = 0; w < NW; ++w) sbytes[w * SL + pos] = i32x4{0, 0, 0, 0};
                     soff[pos] = 127.0f;
                     smk[pos] = 0ull;
                 }
@@ -1058,12 +1074,23 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
             __syncthreads();
 
             float thrf = (float)min(bound, 127);
-            // B operand and stream constant one group ahead
+            // B operands and stream constant one group ahead (product p: word p NKB + kb; the
+            // chained test loads its second product's operand in the group, for registers)
             const i32x4* sb = sbytes + kb * SL + row;
             i32x4 bnext = sb[0];
             float onext = soff[row];
+            // lane block b's product with the group's B operands
+            auto prod = [&](int b, const i32x4 (&bv)[NP]) {
+                acc c = P::mm(av[b][0], bv[0]);
+#pragma unroll
+                for (int p = 1; p < NP; ++p) c = P::mm(av[b][p], bv[p], c);
+                return c;
+            };
             for (int gi = 0; gi < gmax; ++gi) {
-                const i32x4 bv = bnext;
+                i32x4 bv[NP];
+                bv[0] = bnext;
+#pragma unroll
+                for (int p = 1; p < NP; ++p) bv[p] = sb[p * NKB * SL + gi * NR];
                 const float thr = thrf - onext;   // this lane's column: L + sum < bound - off_stream
                 if (gi + 1 < gmax) {
                     bnext = sb[(gi + 1) * NR];
@@ -1081,20 +1108,29 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                         for (int v = 3; v < 15; v += 2) m = min(min(m, __float_as_int(c[v])), __float_as_int(c[v + 1]));
                         return min(m, __float_as_int(c[15]));
                     };
+                    if constexpr (NP == 1) {
 #pragma unroll
-                    for (int b0 = 0; b0 < kMfmaBlocks; b0 += 4) {
-                        const acc p0 = P::mm(av[b0 + 0], bv);
-                        const acc p1 = P::mm(av[b0 + 1], bv);
-                        const acc p2 = P::mm(av[b0 + 2], bv);
-                        const acc p3 = P::mm(av[b0 + 3], bv);
-                        mn = min(mn, min(min(red(p0), red(p1)), min(red(p2), red(p3))));
+                        for (int b0 = 0; b0 < NB; b0 += 4) {
+                            const acc p0 = prod(b0 + 0, bv);
+                            const acc p1 = prod(b0 + 1, bv);
+                            const acc p2 = prod(b0 + 2, bv);
+                            const acc p3 = prod(b0 + 3, bv);
+                            mn = min(mn, min(min(red(p0), red(p1)), min(red(p2), red(p3))));
+                        }
+                    } else {   // chained products in passes of two (two accumulators live)
+#pragma unroll
+                        for (int b0 = 0; b0 < NB; b0 += 2) {
+                            const acc p0 = prod(b0 + 0, bv);
+                            const acc p1 = prod(b0 + 1, bv);
+                            mn = min(mn, min(red(p0), red(p1)));
+                        }
                     }
                 } else {
-                    acc p[kMfmaBlocks];
+                    acc p[NB];
 #pragma unroll
-                    for (int b = 0; b < kMfmaBlocks; ++b) p[b] = P::mm(av[b], bv);
+                    for (int b = 0; b < NB; ++b) p[b] = prod(b, bv);
 #pragma unroll
-                    for (int b = 0; b < kMfmaBlocks; ++b)
+                    for (int b = 0; b < NB; ++b)
                         mn = min(min(mn, min(__float_as_int(p[b][0]), __float_as_int(p[b][1]))),
                                  min(__float_as_int(p[b][2]), __float_as_int(p[b][3])));
                 }
@@ -1107,13 +1143,17 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 ++job_hits;
                 int mine = QSF_WEIGHT_NONE;
                 u64 cand[2] = {0ull, 0ull};
-                // an opaque copy of the B operand: the products are formed again here instead
+                // an opaque copy of the B operands: the products are formed again here instead
                 // of all eight being kept live from the test above (which would spill)
-                i32x4 bx = bv;
-                asm volatile("" : "+v"(bx));
+                i32x4 bx[NP];
 #pragma unroll
-                for (int b = 0; b < kMfmaBlocks; ++b) {
-                    const acc p = P::mm(av[b], bx);
+                for (int p = 0; p < NP; ++p) {
+                    bx[p] = bv[p];
+                    asm volatile("" : "+v"(bx[p]));
+                }
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const acc p = prod(b, bx);
                     float pm = p[0];
 #pragma unroll
                     for (int v = 1; v < NV; ++v) pm = fminf(pm, p[v]);
@@ -1129,7 +1169,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                         const int q = 64 * part + __builtin_ctzll(cm);
                         const int b = q / NV;
                         const u32 ij = lay[lane * NV + (q % NV)];
-                        const u64 lmask = wlmk[kMfmaBlocks * (ij & 31) + b];
+                        const u64 lmask = wlmk[NB * (ij & 31) + b];
                         const int pos = gi * NR + (int)(ij >> 8);
                         const u64 sm_ = smk[pos];
                         const int base = __popcll(lmask & lflags) + foff + __popcll(sm_ & sflags);
@@ -1450,7 +1490,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
-    static constexpr int kVariants = 22;  // kernel variants of a tiled run (vtab in run_tiled)
+    static constexpr int kVariants = 23;  // kernel variants of a tiled run (vtab in run_tiled)
     u64* gbinom = nullptr;               // device address of tiled_binom (filled at probe)
     unsigned char* scratch_dev = nullptr;   // device address of scratch (no-allocation calls)
     DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
@@ -2242,7 +2282,8 @@ struct Engine {
     // elements), which would otherwise reach the slow path. thr = bound - (flagged rows
     // that any d-subset must contain).
     // mm: 0 VALU hot test (bz_tiled<fd, xd, rpl>), 1 int8 products (bz_mfma<xd>), 2 / 3 fp4
-    // products of 32 x 32 / 16 x 16 codewords (bz_f4<xd, mm>); the matrix tests have fd = 0.
+    // products of 32 x 32 / 16 x 16 codewords, 4 two chained 32 x 32 fp4 products
+    // (bz_f4<xd, mm>); the matrix tests have fd = 0.
     // w: words per list subset mask (2 for K > 128, VALU kernel only).
     struct Variant {
         int fd, xd, rpl, mm;
@@ -2274,12 +2315,14 @@ struct Engine {
         v.w = 2;
         return v;
     }
-    // The hot test of a variant: its unit (the matrix tests count 8192-codeword groups
-    // holding a value below the threshold, the VALU test stream steps of 64 x RPL
-    // codewords) and its test bits (30 for int8, 60 for fp4 32x32x64, 32 FD for VALU).
-    static double test_unit(const Variant& V) { return V.mm > 0 ? 8192.0 : 64.0 * V.rpl; }
+    // The hot test of a variant: its unit (the matrix tests count groups of one wave's lane
+    // blocks x 32 stream elements holding a value below the threshold: 8192 codewords, 4096
+    // for the chained fp4 test; the VALU test stream steps of 64 x RPL codewords) and its
+    // test bits (30 for int8, 60 for fp4 32x32x64, 124 for the 128-dimension fp4 tests,
+    // 32 FD for VALU).
+    static double test_unit(const Variant& V) { return V.mm == 4 ? 4096.0 : V.mm > 0 ? 8192.0 : 64.0 * V.rpl; }
     static int test_bits(const Variant& V, int res) {
-        return std::min(V.mm == 1 ? 30 : V.mm == 2 ? 60 : V.mm == 3 ? 124 : 32 * V.fd, res);
+        return std::min(V.mm == 1 ? 30 : V.mm == 2 ? 60 : V.mm >= 3 ? 124 : 32 * V.fd, res);
     }
     // Hot-test passes on a level of cw codewords with threshold thr if the residual bits
     // were fair coins (per-codeword pass probability p = P(Binomial(bits, 1/2) < thr)).
@@ -2304,6 +2347,7 @@ struct Engine {
         const double f = s.factor_at(p);
         const double hits = std::max(fair_hits(V, cw, p) * f, (double)cw * std::min(1.0, p * f));
         if (V.mm == 1) return tuning::kMatrixLaunch + (double)cw * tuning::kInt8Cost + hits * tuning::kInt8Hit;
+        if (V.mm == 4) return tuning::kMatrixLaunch + (double)cw * tuning::kF4x2Cost + hits * tuning::kF4x2Hit;
         if (V.mm >= 2) return tuning::kMatrixLaunch + (double)cw * tuning::kF4Cost + hits * tuning::kF4Hit;
         const int k = valu_index(V);
         return tuning::kValuLaunch + (double)cw * tuning::kValuCost[k] + hits * tuning::kValuHit[k];
@@ -2330,7 +2374,7 @@ struct Engine {
     static Variant choose_variant_k128(const Solve& s, const qsf_enum_plan& p, bool allow_mfma, u64 min_work) {
         static const Variant table[] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {2, 2, 16, 0}, {2, 4, 12, 0}, {3, 4, 12, 0},
                                         {4, 4, 12, 0}, {1, 2, 24, 0}, {1, 2, 32, 0}, {0, 1, 4, 1},  {0, 2, 4, 1},
-                                        {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3}};
+                                        {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3},  {0, 4, 4, 4}};
         // the matrix tests compare their products with min(bound, 127); their padding
         // elements stay above any bound <= 100
         allow_mfma = allow_mfma && p.current_best <= 100;
@@ -2371,13 +2415,14 @@ struct Engine {
             if (s.RD >= 3) { consider(table[4]); consider(table[5]); }
         }
         if (allow_mfma) {
-            // int8 32x32x32 products over 30 test bits; fp4 32x32x64 products over 60. The
-            // 16x16x128 fp4 form (124 bits, table[13]) is not chosen automatically: its
-            // 256-element stream chunks pay the per-chunk bound reload and barrier four
-            // times as often, and on the quasi-cyclic evaluation codes it ran at half the
-            // VALU kernel's rate.
+            // int8 32x32x32 products over 30 test bits; fp4 32x32x64 products over 60, or two
+            // of them chained over 124 (table[14]). The 16x16x128 fp4 form (124 bits,
+            // table[13]) is not chosen automatically: its 256-element stream chunks pay the
+            // per-chunk bound reload and barrier four times as often, and on the
+            // quasi-cyclic evaluation codes it ran at half the VALU kernel's rate.
             consider(s.RD >= 3 ? table[10] : s.RD == 2 ? table[9] : table[8]);
             if (!std::getenv("QSA_NO_F4") && s.RD >= 2) consider(s.RD >= 3 ? table[12] : table[11]);
+            if (!std::getenv("QSA_NO_F4") && s.RD >= 3) consider(table[14]);
         }
         return best;
     }
@@ -2625,7 +2670,7 @@ struct Engine {
             {1, 1, 32, 0},    {1, 2, 16, 0},    {1, 2, 24, 0},    {1, 2, 32, 0},    {2, 2, 16, 0},    {2, 4, 12, 0},
             {3, 4, 12, 0},    {4, 4, 12, 0},    {0, 1, 4, 1},     {0, 2, 4, 1},     {0, 4, 4, 1},     {0, 2, 4, 2},
             {0, 4, 4, 2},     {0, 4, 2, 3},     {1, 1, 32, 0, 2}, {1, 2, 16, 0, 2}, {1, 2, 24, 0, 2}, {1, 2, 32, 0, 2},
-            {2, 2, 16, 0, 2}, {2, 4, 12, 0, 2}, {3, 4, 12, 0, 2}, {4, 4, 12, 0, 2}};
+            {2, 2, 16, 0, 2}, {2, 4, 12, 0, 2}, {3, 4, 12, 0, 2}, {4, 4, 12, 0, 2}, {0, 4, 4, 4}};
         auto vidx = [&](const Variant& v) {
             for (int k = 0; k < NV; ++k)
                 if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl && vtab[k].mm == v.mm &&
@@ -3064,7 +3109,8 @@ struct Engine {
                 case 18: hipLaunchKernelGGL((bz_tiled<2, 2, 16, 2>), gd, bd, lds, stream, a); break;
                 case 19: hipLaunchKernelGGL((bz_tiled<2, 4, 12, 2>), gd, bd, lds, stream, a); break;
                 case 20: hipLaunchKernelGGL((bz_tiled<3, 4, 12, 2>), gd, bd, lds, stream, a); break;
-                default: hipLaunchKernelGGL((bz_tiled<4, 4, 12, 2>), gd, bd, lds, stream, a); break;
+                case 21: hipLaunchKernelGGL((bz_tiled<4, 4, 12, 2>), gd, bd, lds, stream, a); break;
+                default: hipLaunchKernelGGL((bz_f4<4, 4>), gd, bd, lds, stream, a); break;
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(gev_dyn[2 * gi + 1], stream));

@@ -45,9 +45,11 @@ constexpr int kMfmaBlocks = 8;      // bz_mfma: 32-element lane blocks per wave
 constexpr int kMfmaWaves = 4;       // bz_mfma: waves per workgroup, sharing each stream chunk
 constexpr int kMfmaLane = 32 * kMfmaBlocks * kMfmaWaves;   // bz_mfma: lane elements per tile
 constexpr int kMfmaStream = 512;    // bz_mfma: stream chunk length (two elements per thread)
-// bz_f4<XD, MM>: 32 x 32 products (MM = 2) or 16 x 16 products (MM = 3) per lane block
-constexpr int f4_rows(int mm) { return mm == 2 ? 32 : 16; }
-constexpr int f4_lane_tile(int mm) { return f4_rows(mm) * kMfmaBlocks * kMfmaWaves; }   // lane elements per tile
+// bz_f4<XD, MM>: 32 x 32 products (MM = 2), 16 x 16 products (MM = 3), or two chained
+// 32 x 32 products (MM = 4) per lane block
+constexpr int f4_rows(int mm) { return mm == 3 ? 16 : 32; }
+constexpr int f4_blocks(int mm) { return mm == 4 ? 4 : kMfmaBlocks; }                  // lane blocks per wave
+constexpr int f4_lane_tile(int mm) { return f4_rows(mm) * f4_blocks(mm) * kMfmaWaves; }   // lane elements per tile
 constexpr int f4_stream(int mm) { return mm == 2 ? 512 : 256; }                         // stream chunk length
 constexpr int kMaxRW = 64;          // residual + syndrome dwords per row in the tiled kernel
 constexpr int kLdsFullRows = 1024;  // dwords: the whole K x RW table is kept in LDS up to this size

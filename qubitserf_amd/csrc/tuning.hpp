@@ -18,6 +18,9 @@ constexpr double kInt8Cost = 4.033e-14;
 constexpr double kInt8Hit = 7.963e-10;
 constexpr double kF4Cost = 4.560e-14;
 constexpr double kF4Hit = 8.303e-10;
+// Two chained fp4 32x32x64 products (124 test bits): per codeword, and per group of 4096.
+constexpr double kF4x2Cost = 7.000e-14;
+constexpr double kF4x2Hit = 5.000e-10;
 // Fixed seconds per level launch (set-up, tail) of the VALU and matrix-core kernels.
 constexpr double kValuLaunch = 2.946e-05;
 constexpr double kMatrixLaunch = 2.460e-05;

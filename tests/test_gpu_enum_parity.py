@@ -278,13 +278,14 @@ def test_mfma_chosen_on_large_levels(hip, monkeypatch):
 
 
 
-@pytest.mark.parametrize("mm,xd,stride", [(2, 2, 2), (2, 4, 3), (3, 4, 3)])
+@pytest.mark.parametrize("mm,xd,stride", [(2, 2, 2), (2, 4, 3), (3, 4, 3), (4, 4, 3)])
 def test_f4_hot_test(hip, mm, xd, stride, monkeypatch):
     """The fp4 matrix-core hot tests (bz_f4: 32x32x64 products with 60 test bits, 16x16x128
-    with 124) on random and systematic generators, loose and tight bounds, against the
-    oracle. Systematic generators flag every row (the lane and stream constants are the
-    subset sizes); random ones flag few (the constants vary per element)."""
-    rpl = 4 if mm == 2 else 2
+    with 124, two chained 32x32x64 with 124) on random and systematic generators, loose and
+    tight bounds, against the oracle. Systematic generators flag every row (the lane and
+    stream constants are the subset sizes); random ones flag few (the constants vary per
+    element)."""
+    rpl = 2 if mm == 3 else 4
     monkeypatch.setenv("QSA_TILED_VARIANT", f"0,{xd},{rpl},{mm}")
     rng = np.random.default_rng(1300 + 10 * mm + xd)
     kmax = 64 if stride == 2 else 120
@@ -304,9 +305,9 @@ def test_f4_hot_test(hip, mm, xd, stride, monkeypatch):
 
 def test_f4_chosen_on_high_thresholds(hip, monkeypatch):
     """Automatic choice: levels whose bound leaves too many light codewords on 30 test bits
-    run the fp4 32x32x64 test (60 bits), looser ones the VALU kernel; results equal the
-    oracle's. The size rule (>= 2^33 codewords) is lowered so that the oracle can check the
-    levels."""
+    run the fp4 32x32x64 test (60 bits), looser ones the two chained products (124 bits);
+    each level runs the cost model's pick and its result equals the oracle's. The size rule
+    (>= 2^33 codewords) is lowered so that the oracle can check the levels."""
     monkeypatch.setenv("QSA_MATRIX_MIN_WORK", "0")
     rng = np.random.default_rng(31)
     seen = set()
@@ -314,21 +315,24 @@ def test_f4_chosen_on_high_thresholds(hip, monkeypatch):
         gamma, check = rand_plan(rng, K, 3, 1, 2, systematic=True)
         n = 192
         want = O.enumerate_plan(gamma, n, d, check=check, current_best=best)
+        pick = hip.plan_variant(gamma, n, d, check=check, current_best=best)
         got = hip.enumerate_level(gamma, n, d, check=check, current_best=best)
         st = hip.last_stats()
         assert got == want, (K, d, best, got, want, st)
+        assert (st["fd"], st["xd"], st["rpl"], st["mfma"]) == pick, (K, d, best, pick, st)
         seen.add(st["mfma"])
-    assert seen == {0, 2}, seen
+    assert {2, 4} <= seen, seen
 
 
 def test_automatic_choice_runs_the_cost_models_pick(hip, monkeypatch):
     """The engine runs, on a fresh solve, exactly the variant its measured cost model picks
     (qsf_hip_plan_variant, host only), across tight and loose bounds, with results equal to
-    the oracle's. The size rule is lowered so that the oracle can check the levels."""
+    the oracle's. The size rule is lowered so that the oracle can check the levels; a bound
+    above 100 leaves the VALU kernel only."""
     monkeypatch.setenv("QSA_MATRIX_MIN_WORK", "0")
     rng = np.random.default_rng(41)
     kinds = set()
-    for K, d, best in [(80, 5, 7), (80, 5, 13), (72, 4, 16), (64, 4, 30), (96, 4, 9), (60, 5, 22)]:
+    for K, d, best in [(80, 5, 7), (80, 5, 13), (72, 4, 16), (64, 4, 30), (96, 4, 9), (60, 5, 22), (70, 4, 120)]:
         gamma, check = rand_plan(rng, K, 3, 1, 2, systematic=True)
         n = 192
         pick = hip.plan_variant(gamma, n, d, check=check, current_best=best)
