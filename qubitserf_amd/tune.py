@@ -6,7 +6,8 @@ constants of the cost model that picks a hot-test variant for every weight level
 
 Measured trials, all through the plugin-level C ABI with the variant forced
 (QSA_TILED_VARIANT):
-  * cost per codeword of each VALU variant and of the int8 and fp4 matrix tests, on one
+  * cost per codeword of each VALU variant and of the int8, fp4 and chained fp4 matrix
+    tests, on one
     large level (K = 80, d = 8, C(80, 8) = 2.9e10 codewords) of a random systematic
     generator whose threshold lets almost nothing pass the hot test;
   * cost per hot-test pass: the same level with a looser bound, so that ~20% of the test
@@ -51,6 +52,8 @@ def unit_and_bits(v):
         return 8192.0, 30
     if mm == 2:
         return 8192.0, 60
+    if mm == 4:
+        return 4096.0, 124
     return 64.0 * rpl, 32 * fd
 
 
@@ -143,7 +146,7 @@ def measure(log=print):
     run = Runner(systematic(rng, K, stride), n)
     cw = math.comb(K, d)
     tight = d + 2
-    variants = [(f, x, r, 0) for f, x, r in VALU] + [(0, 4, 4, 1), (0, 4, 4, 2)]
+    variants = [(f, x, r, 0) for f, x, r in VALU] + [(0, 4, 4, 1), (0, 4, 4, 2), (0, 4, 4, 4)]
     out = {"level": f"K={K} n={n} d={d} random systematic, {cw:.3e} codewords", "variants": {}}
     small_d = 4
     small_cw = math.comb(K, small_d)
@@ -168,7 +171,7 @@ def measure(log=print):
 def header(m, device):
     V = m["variants"]
     valu = [V[f"{f},{x},{r},0"] for f, x, r in VALU]
-    i8, f4 = V["0,4,4,1"], V["0,4,4,2"]
+    i8, f4, f4x2 = V["0,4,4,1"], V["0,4,4,2"], V["0,4,4,4"]
     hits = [r["hit"] for r in valu if r["hit"]]
     valu_hit_default = sorted(hits)[len(hits) // 2] if hits else 1.5e-9
 
@@ -176,7 +179,7 @@ def header(m, device):
         return f"{x:.3e}"
 
     launch_valu = sorted(r["launch"] for r in valu)[len(valu) // 2]
-    launch_mm = max(i8["launch"], f4["launch"])
+    launch_mm = max(i8["launch"], f4["launch"], f4x2["launch"])
     lines = [
         "// Kernel-choice constants of the enumeration engine, measured on an MI355X.",
         "// Regenerate with:  python -m qubitserf_amd.tune --write   (on a gfx950 device)",
@@ -198,6 +201,9 @@ def header(m, device):
         f"constexpr double kInt8Hit = {fmt(i8['hit'] or 1.8e-10)};",
         f"constexpr double kF4Cost = {fmt(f4['cost'])};",
         f"constexpr double kF4Hit = {fmt(f4['hit'] or i8['hit'] or 1.8e-10)};",
+        "// Two chained fp4 32x32x64 products (124 test bits): per codeword, and per group of 4096.",
+        f"constexpr double kF4x2Cost = {fmt(f4x2['cost'])};",
+        f"constexpr double kF4x2Hit = {fmt(f4x2['hit'] or f4['hit'] or 1.8e-10)};",
         "// Fixed seconds per level launch (set-up, tail) of the VALU and matrix-core kernels.",
         f"constexpr double kValuLaunch = {fmt(launch_valu)};",
         f"constexpr double kMatrixLaunch = {fmt(launch_mm)};",
