@@ -153,8 +153,8 @@ def pack_batch_bits(Hs, seeds=None):
         Hb[b, : H.shape[0], : H.shape[1]] = H.astype(np.uint8, copy=False) & 1
         hms[b], hns[b] = H.shape
     mask = (np.arange(n)[None, :] < hns[:, None]).astype(np.float32)
-    dc_raw = Hb.sum(2, dtype=np.int32).astype(np.float32)           # (B, m)
-    dv_raw = Hb.sum(1, dtype=np.int32).astype(np.float32)           # (B, n)
+    dc_raw = Hb.sum(2, dtype=np.uint16).astype(np.float32)          # (B, m)
+    dv_raw = Hb.sum(1, dtype=np.uint16).astype(np.float32)          # (B, n)
     inv = bf16_bits(np.float32(1.0) / np.arange(1, max(m, n) + 1, dtype=np.float32))
     pc = inv[np.maximum(dc_raw, 1).astype(np.int64) - 1]          # (B, m) pattern of 1 / deg_c
     pv = inv[np.maximum(dv_raw, 1).astype(np.int64) - 1]          # (B, n) pattern of 1 / deg_v
@@ -164,12 +164,17 @@ def pack_batch_bits(Hs, seeds=None):
     X = np.zeros((B, n, F), np.float32)
     X[:, :, 0] = mask
     X[:, :, 1] = dv_raw / np.maximum(np.float32(1.0), dv_raw.max(1, keepdims=True))
-    mean_dc = np.einsum("bm,bmn->bn", dc_raw, Hb, dtype=np.float32) / np.maximum(dv_raw, np.float32(1.0))
+    mean_dc = np.matmul(dc_raw[:, None, :], Hb, dtype=np.float32)[:, 0, :] / np.maximum(dv_raw, np.float32(1.0))
     X[:, :, 2] = mean_dc / np.maximum(np.float32(1.0), dc_raw.max(1, keepdims=True))
     X[:, :, 3] = (hms / hns).astype(np.float32)[:, None] * mask
-    for b in range(B):
-        X[b, : hns[b], 4:8] = np.random.default_rng(0 if seeds is None else seeds[b]).standard_normal(
-            (int(hns[b]), 4)).astype(np.float32) * 0.5
+    if seeds is None:   # every code draws from seed 0: the first n_b rows of one draw
+        noise = np.random.default_rng(0).standard_normal((n, 4)).astype(np.float32) * 0.5
+        for b in range(B):
+            X[b, : hns[b], 4:8] = noise[: hns[b]]
+    else:
+        for b in range(B):
+            X[b, : hns[b], 4:8] = np.random.default_rng(seeds[b]).standard_normal(
+                (int(hns[b]), 4)).astype(np.float32) * 0.5
     return X, Ac, Av, mask
 
 

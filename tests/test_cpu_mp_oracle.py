@@ -84,3 +84,8 @@ def test_pack_batch_bits_equals_pack_batch():
     X2, A2, V2, m2 = pack_batch_bits(Hs, seeds)
     assert np.array_equal(X, X2) and np.array_equal(mask, m2)
     assert A2.dtype == np.uint16 and np.array_equal(bf16_bits(Ac), A2) and np.array_equal(bf16_bits(Av), V2)
+    # without seeds every code draws from seed 0 (one draw, cut per code, for the batch)
+    X, Ac, Av, mask = pack_batch(Hs)
+    X2, A2, V2, m2 = pack_batch_bits(Hs)
+    assert np.array_equal(X, X2) and np.array_equal(mask, m2)
+    assert np.array_equal(bf16_bits(Ac), A2) and np.array_equal(bf16_bits(Av), V2)
