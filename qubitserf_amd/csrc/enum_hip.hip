@@ -1467,12 +1467,15 @@ struct Solve {
     std::vector<u32> htbl;        // host copy of the residual+syndrome table
     std::vector<u64> hflags;      // host copy of the flag masks
     std::vector<u64> binom;       // (K+1) x bk
-    // device copies, carved from one allocation: [gamma | check | binom | flags | tbl]
+    // device copies, carved from one allocation: [gamma | check | binom | flags | tbl]; the
+    // binomials only for K > kMaxTiledK (below that the kernels read the library's static
+    // table, whose rows are dbk = kMaxD + 1 wide)
     struct Ptr { u64* p = nullptr; };
     struct Ptr32 { u32* p = nullptr; };
     DevBuf<u64> arena;
     std::vector<u64> stage;       // host image of the arena (source of its upload)
     size_t n_gamma = 0, n_check = 0, n_binom = 0, n_flags = 0;
+    int dbk = 0;                  // row stride of the device binomial table
     Ptr gamma, check, dbinom, flags;
     Ptr32 tbl;
     u64 C(int nn, int kk) const {
@@ -1580,7 +1583,7 @@ struct Engine {
         u64* d = reinterpret_cast<u64*>(scratch_dev + kScratchHead);
         s->gamma.p = d;
         s->check.p = d + s->n_gamma;
-        s->dbinom.p = d + s->n_gamma + s->n_check;
+        s->dbinom.p = s->n_binom ? d + s->n_gamma + s->n_check : gbinom;
         s->flags.p = d + s->n_gamma + s->n_check + s->n_binom;
         s->tbl.p = reinterpret_cast<u32*>(d + s->n_gamma + s->n_check + s->n_binom + s->n_flags);
         // pageable source: the staging vector stays alive (in na_solve) until the stream drains
@@ -1783,13 +1786,15 @@ struct Engine {
         }
         // one staging vector, one allocation, one copy
         const size_t n_gamma = (size_t)s.ngt * K * st, n_check = (size_t)std::max(1, p.kcheck) * st;
-        const size_t n_binom = s.binom.size(), n_flags = fl.size(), n_tbl = (tbl.size() + 1) / 2;
+        // K <= kMaxTiledK: the static table (C(n <= 256, k <= 64)) serves the device
+        const size_t n_binom = K <= kMaxTiledK ? 0 : s.binom.size(), n_flags = fl.size(), n_tbl = (tbl.size() + 1) / 2;
+        s.dbk = n_binom ? s.bk : kMaxD + 1;
         std::vector<u64>& stage = s.stage;   // host image of the device arena
         stage.assign(n_gamma + n_check + n_binom + n_flags + n_tbl, 0);
         u64* at = stage.data();
         std::memcpy(at, p.gamma, n_gamma * 8);
         if (p.kcheck) std::memcpy(at + n_gamma, p.check, (size_t)p.kcheck * st * 8);
-        std::memcpy(at + n_gamma + n_check, s.binom.data(), n_binom * 8);
+        if (n_binom) std::memcpy(at + n_gamma + n_check, s.binom.data(), n_binom * 8);
         std::memcpy(at + n_gamma + n_check + n_binom, fl.data(), n_flags * 8);
         if (!tbl.empty()) std::memcpy(at + n_gamma + n_check + n_binom + n_flags, tbl.data(), tbl.size() * 4);
         s.n_gamma = n_gamma; s.n_check = n_check; s.n_binom = n_binom; s.n_flags = n_flags;
@@ -1853,7 +1858,7 @@ struct Engine {
             u64* d = s.arena.p;
             s.gamma.p = d;
             s.check.p = d + n_gamma;
-            s.dbinom.p = d + n_gamma + n_check;
+            s.dbinom.p = n_binom ? d + n_gamma + n_check : gbinom;
             s.flags.p = d + n_gamma + n_check + n_binom;
             s.tbl.p = reinterpret_cast<u32*>(d + n_gamma + n_check + n_binom + n_flags);
             std::memcpy(up_pin + at, stage.data(), stage.size() * 8);
@@ -2249,7 +2254,7 @@ struct Engine {
             J.binom = s.dbinom.p;
             J.best = reinterpret_cast<int*>(dev) + i;
             J.total = s.C(p.K, p.d);
-            J.K = p.K; J.d = p.d; J.stride = p.stride; J.kcheck = p.kcheck; J.ngamma = p.num_gamma; J.bk = s.bk;
+            J.K = p.K; J.d = p.d; J.stride = p.stride; J.kcheck = p.kcheck; J.ngamma = p.num_gamma; J.bk = s.dbk;
             // runs of <= 32 combinations, >= 1024 threads per generator when the level has them
             J.chunk = (int)std::max<u64>(1, std::min<u64>(32, (J.total + 1023) / 1024));
             const u64 threads = (J.total + (u64)J.chunk - 1) / (u64)J.chunk;
@@ -3212,7 +3217,7 @@ struct Engine {
         if (p.d > kMaxD) throw HipError{4, "weight level above 64"};
         SweepArgs a{};
         a.gamma = s.gamma.p; a.check = s.check.p; a.kcheck = p.kcheck; a.stride = p.stride;
-        a.K = p.K; a.d = p.d; a.ngamma = p.num_gamma; a.binom = s.dbinom.p; a.bk = s.bk;
+        a.K = p.K; a.d = p.d; a.ngamma = p.num_gamma; a.binom = s.dbinom.p; a.bk = s.dbk;
         a.total = s.C(p.K, p.d);
         const u64 target_threads = (u64)std::max(1, cus) * 2048;
         u64 chunk = std::max<u64>(64, (a.total * (u64)p.num_gamma + target_threads - 1) / target_threads);
