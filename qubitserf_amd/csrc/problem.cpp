@@ -231,7 +231,13 @@ namespace {
 // is dependent on every class, it displaces a member of a circuit, which is then placed
 // recursively (never back into the class it left). The span test and circuit bookkeeping
 // follow cpp/distfind/css.cpp:231-268 step for step, so the classes (and hence the
-// information sets and the BZ lower bounds) match the reference's.
+// information sets and the BZ lower bounds) match the reference's. Each class keeps its
+// reduced basis (with the origin of every basis vector) between span tests: a column that
+// joins a class extends it; a swap on an augmenting path (z in, y out, z the XOR of the
+// circuit C, y in C) keeps the span, so only the origins that hold y change, to
+// origin ^ C ^ {z}, and undoing the swap changes those that hold z by the same mask. The
+// span and the circuit of a column do not depend on how the basis was built, since a
+// class is independent.
 class Partitioner {
   public:
     explicit Partitioner(const BitMat& gen)
@@ -248,7 +254,10 @@ class Partitioner {
         seen_.assign(n_, 0);
         for (int z : nonzero_) {
             std::fill(seen_.begin(), seen_.end(), 0);
-            if (!place(z, -1)) classes_.push_back({z});
+            if (!place(z, -1)) {
+                classes_.push_back({z});
+                basis_.emplace_back();
+            }
         }
         if (classes_.empty()) classes_.push_back({});
         for (int j : zero_) classes_[0].push_back(j);
@@ -263,74 +272,102 @@ class Partitioner {
 
   private:
     const u64* colv(int j) const { return col_.data() + (size_t)j * kw_; }
-    // true iff column z lies in the span of class `cls`; then *circuit = the class columns
+    // A class's reduced basis: vector b (kw words) with pivot lead[b] (its lowest set bit
+    // when it was added; later vectors are reduced on it) and origin o (ow words: the class
+    // columns whose XOR it is).
+    struct Basis {
+        std::vector<u64> v, o;
+        std::vector<int> lead;
+        bool valid = false;
+    };
+    // tv_ = column col reduced by basis B, to_ the class columns it was reduced by
+    void reduce(const Basis& B, int col) {
+        const int kw = kw_, ow = ow_;
+        tv_.assign(colv(col), colv(col) + kw);
+        to_.assign(ow, 0ull);
+        for (size_t b = 0; b < B.lead.size(); ++b) {
+            const int l = B.lead[b];
+            if ((tv_[l >> 6] >> (l & 63)) & 1) {
+                const u64* v = B.v.data() + b * kw;
+                const u64* o = B.o.data() + b * ow;
+                for (int k = 0; k < kw; ++k) tv_[k] ^= v[k];
+                for (int k = 0; k < ow; ++k) to_[k] ^= o[k];
+            }
+        }
+    }
+    // add column col (reduced into tv_ / to_ by reduce) to B if it is independent of it
+    void extend(Basis& B, int col) {
+        int l = -1;
+        for (int k = 0; k < kw_ && l < 0; ++k)
+            if (tv_[k]) l = k * 64 + __builtin_ctzll(tv_[k]);
+        if (l < 0) return;
+        to_[col >> 6] ^= 1ull << (col & 63);
+        B.v.insert(B.v.end(), tv_.begin(), tv_.end());
+        B.o.insert(B.o.end(), to_.begin(), to_.end());
+        B.lead.push_back(l);
+    }
+    // class c's member `out` swapped for `in` on a circuit: the origins that hold `out`
+    // take delta = (circuit mask) ^ {z}
+    void swap_member(int c, const std::vector<u64>& delta, int out) {
+        Basis& B = basis_[c];
+        if (!B.valid) return;
+        for (size_t b = 0; b < B.lead.size(); ++b) {
+            u64* o = B.o.data() + b * ow_;
+            if ((o[out >> 6] >> (out & 63)) & 1)
+                for (int k = 0; k < ow_; ++k) o[k] ^= delta[k];
+        }
+    }
+    Basis& basis(int c) {
+        Basis& B = basis_[c];
+        if (!B.valid) {
+            B.v.clear(); B.o.clear(); B.lead.clear();
+            for (int idx : classes_[c]) {
+                reduce(B, idx);
+                extend(B, idx);
+            }
+            B.valid = true;
+        }
+        return B;
+    }
+    // true iff column z lies in the span of class c; then *circuit = the class columns
     // whose XOR is z, ascending. A class is always independent, so that set is unique and
     // equals the one the reference's origin-tracking reduction finds (css.cpp:231-268).
-    // Basis vectors and their origins (bitsets over columns) live in reused flat buffers.
-    bool spans(const std::vector<int>& cls, int z, std::vector<int>* circuit) {
-        const int kw = kw_, ow = ow_;
-        const bool track = circuit != nullptr;
-        bv_.resize(cls.size() * kw);
-        bl_.resize(cls.size());
-        if (track) bo_.resize(cls.size() * ow);
-        size_t nb = 0;
-        tv_.resize(kw);
-        to_.resize(ow);
-        auto reduce = [&](int col) {   // tv_ = column reduced by the basis so far, to_ its origin
-            std::copy(colv(col), colv(col) + kw, tv_.begin());
-            if (track) std::fill(to_.begin(), to_.end(), 0ull);
-            for (size_t b = 0; b < nb; ++b) {
-                const int l = bl_[b];
-                if ((tv_[l >> 6] >> (l & 63)) & 1) {
-                    const u64* v = bv_.data() + b * kw;
-                    for (int k = 0; k < kw; ++k) tv_[k] ^= v[k];
-                    if (track) {
-                        const u64* o = bo_.data() + b * ow;
-                        for (int k = 0; k < ow; ++k) to_[k] ^= o[k];
-                    }
-                }
-            }
-        };
-        for (int idx : cls) {
-            reduce(idx);
-            int l = -1;
-            for (int k = 0; k < kw && l < 0; ++k)
-                if (tv_[k]) l = k * 64 + __builtin_ctzll(tv_[k]);
-            if (l < 0) continue;
-            std::copy(tv_.begin(), tv_.end(), bv_.begin() + nb * kw);
-            bl_[nb] = l;
-            if (track) {
-                to_[idx >> 6] ^= 1ull << (idx & 63);
-                std::copy(to_.begin(), to_.end(), bo_.begin() + nb * ow);
-            }
-            ++nb;
-        }
-        reduce(z);
+    bool spans(int c, int z, std::vector<int>* circuit) {
+        reduce(basis(c), z);
         for (u64 x : tv_)
             if (x) return false;
-        if (track) {
+        if (circuit) {
             circuit->clear();
-            for (int k = 0; k < ow; ++k)
+            for (int k = 0; k < ow_; ++k)
                 for (u64 m = to_[k]; m; m &= m - 1) circuit->push_back(k * 64 + __builtin_ctzll(m));
         }
         return true;
     }
     bool place(int z, int forbidden) {
         for (int c = 0; c < (int)classes_.size(); ++c)
-            if (c != forbidden && !spans(classes_[c], z, nullptr)) { classes_[c].push_back(z); return true; }
+            if (c != forbidden && !spans(c, z, nullptr)) {   // tv_ / to_ hold z reduced by c
+                classes_[c].push_back(z);
+                extend(basis_[c], z);
+                return true;
+            }
         for (int c = 0; c < (int)classes_.size(); ++c) {
             if (c == forbidden) continue;
             std::vector<int> circ;
-            if (!spans(classes_[c], z, &circ)) continue;
+            if (!spans(c, z, &circ)) continue;
+            std::vector<u64> delta(ow_, 0ull);   // the circuit's mask, and z
+            for (int y : circ) delta[y >> 6] ^= 1ull << (y & 63);
+            delta[z >> 6] ^= 1ull << (z & 63);
             for (int y : circ) {
                 if (seen_[y]) continue;
                 seen_[y] = 1;
                 auto& cls = classes_[c];
                 cls.erase(std::find(cls.begin(), cls.end(), y));
                 cls.push_back(z);
+                swap_member(c, delta, y);
                 if (place(y, c)) return true;
                 cls.erase(std::find(cls.begin(), cls.end(), z));
                 cls.push_back(y);
+                swap_member(c, delta, z);
             }
         }
         return false;
@@ -338,10 +375,10 @@ class Partitioner {
 
     int n_, kw_, ow_;
     std::vector<u64> col_;
-    std::vector<u64> bv_, bo_, tv_, to_;   // span-test scratch
-    std::vector<int> bl_;
+    std::vector<u64> tv_, to_;   // span-test scratch
     std::vector<int> nonzero_, zero_;
     std::vector<std::vector<int>> classes_;
+    std::vector<Basis> basis_;   // of each class
     std::vector<char> seen_;
 };
 
