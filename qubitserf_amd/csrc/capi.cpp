@@ -16,6 +16,7 @@
 #include "../../include/distfind_capi.h"
 #include "../../include/qsf_hip.h"
 #include "driver.hpp"
+#include "pool.hpp"
 
 using namespace qsa;
 
@@ -307,21 +308,13 @@ int build_problems(const DistfindBatchItem* items, int count, std::vector<BatchP
     std::vector<int> rcs(count, RC_OK);
     std::vector<std::string> errs(count);
     std::vector<std::exception_ptr> exc(count);
-    std::atomic<int> next{0};
-    auto work = [&] {
-        for (int i; (i = next++) < count;) {
-            try {
-                rcs[i] = build_problem(items, i, probs[i], errs[i]);
-            } catch (...) {
-                exc[i] = std::current_exception();
-            }
+    HostPool::get().run((size_t)count, host_workers((size_t)count, 32), [&](size_t i) {
+        try {
+            rcs[i] = build_problem(items, (int)i, probs[i], errs[i]);
+        } catch (...) {
+            exc[i] = std::current_exception();
         }
-    };
-    const int workers = count >= 32 ? (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())) : 1;
-    std::vector<std::thread> pool;
-    for (int w = 1; w < workers; ++w) pool.emplace_back(work);
-    work();
-    for (auto& t : pool) t.join();
+    });
     for (int i = 0; i < count; ++i) {
         if (exc[i]) std::rethrow_exception(exc[i]);
         if (rcs[i] != RC_OK) {

@@ -12,6 +12,7 @@
 #include <unordered_map>
 
 #include "../../include/qsf_hip.h"
+#include "pool.hpp"
 
 namespace qsa {
 
@@ -359,25 +360,10 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
             const Proposal* pr = proposals.empty() ? nullptr : proposals[i];
             if (sides[i]->has && pr && pr->candidates > 0 && (int)pr->scores.size() == sides[i]->n) with.push_back(i);
         }
-        std::atomic<size_t> next{0};
-        std::vector<std::exception_ptr> fail(with.size());
-        auto work = [&] {
-            for (size_t k; (k = next++) < with.size();) {
-                try {
-                    const Proposal* pr = proposals[with[k]];
-                    cand[with[k]] = proposal_orders(pr->scores, pr->candidates, pr->seed);
-                } catch (...) {
-                    fail[k] = std::current_exception();
-                }
-            }
-        };
-        const size_t workers = with.size() >= 16 ? std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())) : 1;
-        std::vector<std::thread> pool;
-        for (size_t w = 1; w < workers; ++w) pool.emplace_back(work);
-        work();
-        for (auto& t : pool) t.join();
-        for (auto& e : fail)
-            if (e) std::rethrow_exception(e);
+        HostPool::get().run(with.size(), host_workers(with.size(), 16), [&](size_t k) {
+            const Proposal* pr = proposals[with[k]];
+            cand[with[k]] = proposal_orders(pr->scores, pr->candidates, pr->seed);
+        });
     }
     for (size_t i = 0; i < ns; ++i) {
         if (!sides[i]->has) continue;
@@ -437,20 +423,10 @@ void setup_sides(std::vector<Side*>& sides, const std::vector<const std::vector<
         want.push_back(i);
     }
     std::vector<char> ok(ns, 0);
-    {
-        const size_t workers = std::min<size_t>(want.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
-        std::atomic<size_t> next{0};
-        auto work = [&] {
-            for (size_t k; (k = next++) < want.size();) {
-                const size_t i = want[k];
-                ok[i] = matroid_parts(sides[i]->prob->gen, parts[i]) && !parts[i].empty();
-            }
-        };
-        std::vector<std::thread> pool;
-        for (size_t w = 1; w < workers; ++w) pool.emplace_back(work);
-        work();
-        for (auto& t : pool) t.join();
-    }
+    HostPool::get().run(want.size(), host_workers(want.size(), 2), [&](size_t k) {
+        const size_t i = want[k];
+        ok[i] = matroid_parts(sides[i]->prob->gen, parts[i]) && !parts[i].empty();
+    });
     for (size_t i : want) {
         if (!ok[i]) continue;
         part_req[i].gen = &sides[i]->prob->gen;

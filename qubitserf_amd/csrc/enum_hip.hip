@@ -24,6 +24,7 @@
 #include "elim.hpp"
 #include "kernels.hpp"
 #include "masks.hpp"
+#include "pool.hpp"
 #include "tuning.hpp"
 
 // Device memory the library owns for the life of the process (one copy per GPU it runs on).
@@ -1670,28 +1671,8 @@ struct Engine {
         std::vector<Solve*> up;
         for (auto& f : fresh) up.push_back(f.first);
         try {   // a solve whose build fails leaves the cache
-            const int workers = (int)std::min<size_t>(fresh.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
-            if (workers > 1) {
-                std::atomic<size_t> next{0};
-                std::vector<std::thread> pool;
-                std::exception_ptr fail;
-                std::mutex fail_mu;
-                for (int w = 0; w < workers; ++w)
-                    pool.emplace_back([&] {
-                        for (size_t i; (i = next++) < fresh.size();) {
-                            try {
-                                build_host(*fresh[i].first, *fresh[i].second);
-                            } catch (...) {
-                                std::lock_guard<std::mutex> lk(fail_mu);
-                                fail = std::current_exception();
-                            }
-                        }
-                    });
-                for (auto& t : pool) t.join();
-                if (fail) std::rethrow_exception(fail);
-            } else {
-                for (auto& f : fresh) build_host(*f.first, *f.second);
-            }
+            HostPool::get().run(fresh.size(), host_workers(fresh.size(), 2),
+                                [&](size_t i) { build_host(*fresh[i].first, *fresh[i].second); });
             evict(cache_cap);   // older solves leave first (the new ones are at the front), so
             upload_all(up);     // that the new ones can take their arenas
         } catch (...) {
@@ -2854,28 +2835,7 @@ struct Engine {
             ifixed[i] = std::move(B.fixed);
         };
         const auto prof_t1 = std::chrono::steady_clock::now();
-        const int workers = ni >= 16 ? (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())) : 1;
-        if (workers > 1) {
-            std::atomic<int> next{0};
-            std::vector<std::thread> pool;
-            std::exception_ptr fail;
-            std::mutex fail_mu;
-            for (int w = 0; w < workers; ++w)
-                pool.emplace_back([&] {
-                    for (int i; (i = next++) < ni;) {
-                        try {
-                            decompose(i);
-                        } catch (...) {
-                            std::lock_guard<std::mutex> lk(fail_mu);
-                            fail = std::current_exception();
-                        }
-                    }
-                });
-            for (auto& t : pool) t.join();
-            if (fail) std::rethrow_exception(fail);
-        } else {
-            for (int i = 0; i < ni; ++i) decompose(i);
-        }
+        HostPool::get().run((size_t)ni, host_workers((size_t)ni, 16), [&](size_t i) { decompose((int)i); });
         for (int i = 0; i < ni; ++i) {
             const TItem& it = items[i];
             const Solve& s = *it.s;
