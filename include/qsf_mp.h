@@ -38,6 +38,16 @@ typedef struct {
 } qsf_mp_gemm_desc;
 int qsf_mp_gemm(const qsf_mp_gemm_desc* d, void* stream);
 
+/* The scorer's inputs for `batch` codes from their 0/1 parity checks H (batch x m x n bytes,
+ * code b's hm[b] x hn[b] in the top-left corner, zeros elsewhere): Ac (batch x m x n) and Av
+ * (batch x n x m), H and H^T scaled by the inverse row and column degrees, and the column
+ * features X (batch x n x 16: mask, degree, mean check degree, m/n, noise[j][0..3], zeros),
+ * all BF16 patterns; inv[k - 1] is the pattern of 1 / k for k up to max(m, n), noise is
+ * n x 4 floats (every code's first hn[b] rows). Returns 1 for bad arguments or m + n >
+ * 12288. */
+int qsf_mp_pack(const uint8_t* H, const int* hm, const int* hn, int batch, int m, int n, const uint16_t* inv,
+                const float* noise, uint16_t* Ac, uint16_t* Av, uint16_t* X, void* stream);
+
 int qsf_mp_to_bf16(const float* x, uint16_t* y, long long n, void* stream);
 /* dy[i] = 0 where the BF16 activation y[i] <= 0 */
 int qsf_mp_relu_bwd(const uint16_t* y, float* dy, long long n, void* stream);

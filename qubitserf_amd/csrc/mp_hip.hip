@@ -121,6 +121,62 @@ __global__ void adam(float* w, const float* g, float* m, float* v, uint16_t* wb,
     }
 }
 
+// One workgroup per code: the scorer's operators and input features from the code's 0/1
+// parity check H (hm[b] x hn[b] of the padded m x n), as bfloat16 bit patterns:
+//   Ac[b][i][j] = H[i][j] ? inv[max(dc_i, 1) - 1] : 0       (dc_i: ones in row i)
+//   Av[b][j][i] = H[i][j] ? inv[max(dv_j, 1) - 1] : 0       (dv_j: ones in column j)
+//   X[b][j][f], f < 16: mask_j = (j < hn), dv_j / max(1, max dv),
+//     (sum_i dc_i H[i][j]) / max(dv_j, 1) / max(1, max dc), (hm / hn) mask_j,
+//     noise[j][0..3] mask_j, then zeros
+// with inv[k - 1] the pattern of 1 / k. The same values, bit for bit, as the host packing
+// (qubitserf_amd/proposal/model.py:pack_batch_bits): the sums are exact integers and every
+// division is one correctly rounded f32 division, in the same order.
+__global__ void pack_ops(const uint8_t* H, const int* hm, const int* hn, int m, int n, const uint16_t* inv,
+                         const float* noise, uint16_t* Ac, uint16_t* Av, uint16_t* X) {
+    extern __shared__ int deg[];   // dc[0..m), dv[m..m+n)
+    __shared__ int mx[2];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const uint8_t* h = H + (size_t)b * m * n;
+    if (t == 0) mx[0] = mx[1] = 0;
+    for (int i = t; i < m; i += blockDim.x) {
+        int c = 0;
+        for (int j = 0; j < n; ++j) c += h[(size_t)i * n + j];
+        deg[i] = c;
+    }
+    for (int j = t; j < n; j += blockDim.x) {
+        int c = 0;
+        for (int i = 0; i < m; ++i) c += h[(size_t)i * n + j];
+        deg[m + j] = c;
+    }
+    __syncthreads();
+    for (int i = t; i < m; i += blockDim.x) atomicMax(&mx[0], deg[i]);
+    for (int j = t; j < n; j += blockDim.x) atomicMax(&mx[1], deg[m + j]);
+    __syncthreads();
+    const float mdc = fmaxf(1.f, (float)mx[0]), mdv = fmaxf(1.f, (float)mx[1]);
+    const float ratio = (float)((double)hm[b] / (double)hn[b]);
+    for (int e = t; e < m * n; e += blockDim.x) {
+        const int i = e / n, j = e - i * n;
+        const bool one = h[e] != 0;
+        Ac[(size_t)b * m * n + e] = one ? inv[max(deg[i], 1) - 1] : (uint16_t)0;
+        Av[(size_t)b * n * m + (size_t)j * m + i] = one ? inv[max(deg[m + j], 1) - 1] : (uint16_t)0;
+    }
+    for (int j = t; j < n; j += blockDim.x) {
+        int s = 0;
+        for (int i = 0; i < m; ++i) s += h[(size_t)i * n + j] ? deg[i] : 0;
+        const float dv = (float)deg[m + j];
+        const float mask = j < hn[b] ? 1.f : 0.f;
+        float f[16] = {};
+        f[0] = mask;
+        f[1] = dv / mdv;
+        f[2] = ((float)s / fmaxf(dv, 1.f)) / mdc;
+        f[3] = ratio * mask;
+        if (j < hn[b])
+            for (int k = 0; k < 4; ++k) f[4 + k] = noise[j * 4 + k];
+        uint16_t* x = X + ((size_t)b * n + j) * 16;
+        for (int k = 0; k < 16; ++k) x[k] = f2bf(f[k]);
+    }
+}
+
 unsigned grid_for(long long n) {
     long long b = (n + 255) / 256;
     return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
@@ -162,6 +218,17 @@ int qsf_mp_gemm(const qsf_mp_gemm_desc* d, void* stream) {
                d->sA, d->sB, d->sC, d->transA, d->transB, d->out_bf16, d->accumulate, d->relu};
     dim3 grid((unsigned)((d->M + 15) / 16), (unsigned)((d->N + 15) / 16), (unsigned)d->batch);
     hipLaunchKernelGGL(gemm_bf16, grid, dim3(64), 0, (hipStream_t)stream, g);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+int qsf_mp_pack(const uint8_t* H, const int* hm, const int* hn, int batch, int m, int n, const uint16_t* inv,
+                const float* noise, uint16_t* Ac, uint16_t* Av, uint16_t* X, void* stream) {
+    if (batch < 0 || m < 0 || n < 0 || (batch > 0 && (!H || !hm || !hn || !inv || !noise || !Ac || !Av || !X))) return 1;
+    if (batch == 0 || n == 0) return 0;
+    const size_t lds = (size_t)(m + n) * sizeof(int);
+    if (lds > 48 * 1024) return 1;
+    hipLaunchKernelGGL(pack_ops, dim3((unsigned)batch), dim3(256), lds, (hipStream_t)stream, H, hm, hn, m, n, inv, noise,
+                       Ac, Av, X);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

@@ -10,6 +10,8 @@ import numpy as np
 from .. import _native
 
 F32, BF16 = "f32", "bf16"
+U8, I32 = "u8", "i32"          # raw inputs of the device packing (bytes, int32)
+_ITEMSIZE = {F32: 4, BF16: 2, U8: 1, I32: 4}
 
 
 def _lib():
@@ -61,7 +63,7 @@ class DevArray:
         self.shape = tuple(int(s) for s in shape)
         self.dtype = dtype
         self.numel = int(np.prod(self.shape)) if self.shape else 1
-        self.itemsize = 4 if dtype == F32 else 2
+        self.itemsize = _ITEMSIZE[dtype]
         self.nbytes = self.numel * self.itemsize
         self.L = _lib()
         self.ptr, self._block = _take(max(1, self.nbytes))
@@ -83,6 +85,17 @@ class DevArray:
         return d
 
     @classmethod
+    def from_raw(cls, a, dtype):
+        """An array uploaded byte for byte: uint8 (U8), int32 (I32), float32 (F32) or
+        bfloat16 patterns as uint16 (BF16)."""
+        np_type = {U8: np.uint8, I32: np.int32, F32: np.float32, BF16: np.uint16}[dtype]
+        a = np.ascontiguousarray(a, np_type)
+        d = cls(a.shape, dtype, zero=False)
+        if d.nbytes:
+            d._ok(d.L.qsf_dev_copy(d.ptr, a.ctypes.data, d.nbytes, 1), "qsf_dev_copy")
+        return d
+
+    @classmethod
     def from_bits(cls, bits):
         """A bfloat16 array from its bit patterns (uint16), uploaded as they are."""
         bits = np.ascontiguousarray(bits, np.uint16)
@@ -96,6 +109,14 @@ class DevArray:
         if self.nbytes:
             self._ok(self.L.qsf_dev_copy(host.ctypes.data, self.ptr, self.nbytes, 2), "qsf_dev_copy")
         return host if self.dtype == F32 else bf16_to_f32(host)
+
+    def bits(self) -> np.ndarray:
+        """A bfloat16 array's bit patterns (uint16), as stored."""
+        assert self.dtype == BF16
+        host = np.empty(self.shape, np.uint16)
+        if self.nbytes:
+            self._ok(self.L.qsf_dev_copy(host.ctypes.data, self.ptr, self.nbytes, 2), "qsf_dev_copy")
+        return host
 
     def zero_(self):
         if self.nbytes:

@@ -16,7 +16,7 @@ import math
 import numpy as np
 
 from .. import _native
-from .devarray import BF16, F32, DevArray
+from .devarray import BF16, F32, I32, U8, DevArray
 
 D = 64           # hidden width
 F = 16           # input features per column (padded)
@@ -37,12 +37,14 @@ def _lib():
         L._Desc = Desc
         L.qsf_mp_gemm.argtypes = [POINTER(Desc), c_void_p]
         L.qsf_mp_to_bf16.argtypes = [c_void_p, c_void_p, c_longlong, c_void_p]
+        L.qsf_mp_pack.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p]
         L.qsf_mp_relu_bwd.argtypes = [c_void_p, c_void_p, c_longlong, c_void_p]
         L.qsf_mp_colsum.argtypes = [c_void_p, c_void_p, c_longlong, c_int, c_int, c_void_p]
         L.qsf_mp_bce.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, c_float, c_void_p]
         L.qsf_mp_adam.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, c_float, c_float,
                                   c_float, c_float, c_int, c_void_p]
-        for f in ("qsf_mp_gemm", "qsf_mp_to_bf16", "qsf_mp_relu_bwd", "qsf_mp_colsum", "qsf_mp_bce", "qsf_mp_adam"):
+        for f in ("qsf_mp_gemm", "qsf_mp_pack", "qsf_mp_to_bf16", "qsf_mp_relu_bwd", "qsf_mp_colsum", "qsf_mp_bce", "qsf_mp_adam"):
             getattr(L, f).restype = c_int
         L._mp_bound = True
     return L
@@ -80,6 +82,32 @@ class Dev:
         d = self.L._Desc(A.ptr, B.ptr, C.ptr, bias.ptr if bias is not None else None,
                          batch, M, N, K, lda, ldb, ldc, sA, sB, sC, transA, transB, out_bf16, accumulate, relu)
         _ok(self.L.qsf_mp_gemm(ctypes.byref(d), self.stream), "qsf_mp_gemm")
+
+    def pack(self, Hs):
+        """pack_batch_bits(Hs) (unseeded) computed on the device from the padded 0/1 checks:
+        X, Ac, Av as bfloat16 DevArrays, and the column mask (numpy)."""
+        B = len(Hs)
+        m = max(h.shape[0] for h in Hs)
+        n = max(h.shape[1] for h in Hs)
+        Hb = np.zeros((B, m, n), np.uint8)
+        hms = np.zeros(B, np.int32)
+        hns = np.zeros(B, np.int32)
+        for b, H in enumerate(Hs):
+            H = np.asarray(H)
+            Hb[b, : H.shape[0], : H.shape[1]] = H.astype(np.uint8, copy=False) & 1
+            hms[b], hns[b] = H.shape
+        from .devarray import bf16_bits
+        inv = bf16_bits(np.float32(1.0) / np.arange(1, max(m, n) + 1, dtype=np.float32))
+        noise = np.random.default_rng(0).standard_normal((n, 4)).astype(np.float32) * 0.5
+        dH, dm, dn = DevArray.from_raw(Hb, U8), DevArray.from_raw(hms, I32), DevArray.from_raw(hns, I32)
+        dinv, dnoise = DevArray.from_raw(inv, BF16), DevArray.from_raw(noise, F32)
+        X = DevArray((B, n, F), BF16, zero=False)
+        Ac = DevArray((B, m, n), BF16, zero=False)
+        Av = DevArray((B, n, m), BF16, zero=False)
+        _ok(self.L.qsf_mp_pack(dH.ptr, dm.ptr, dn.ptr, B, m, n, dinv.ptr, dnoise.ptr, Ac.ptr, Av.ptr, X.ptr,
+                               self.stream), "qsf_mp_pack")
+        mask = (np.arange(n)[None, :] < hns[:, None]).astype(np.float32)
+        return X, Ac, Av, mask
 
     def bf16(self, x):
         y = DevArray(x.shape, BF16, zero=False)
@@ -218,10 +246,14 @@ class MPScorer:
         B, n, _ = X.shape
         m = Ac.shape[1]
         c = {"B": B, "n": n, "m": m}
-        c["X"] = DevArray.from_numpy(X, BF16)
-        # operators as float32 values, or already as bfloat16 patterns (pack_batch_bits)
-        c["Ac"] = DevArray.from_bits(Ac) if Ac.dtype == np.uint16 else DevArray.from_numpy(Ac, BF16)
-        c["Av"] = DevArray.from_bits(Av) if Av.dtype == np.uint16 else DevArray.from_numpy(Av, BF16)
+        # inputs on the host (float32, or the operators as bfloat16 patterns from
+        # pack_batch_bits), or already on the device (Dev.pack)
+        c["X"] = X if isinstance(X, DevArray) else DevArray.from_numpy(X, BF16)
+        if isinstance(Ac, DevArray):
+            c["Ac"], c["Av"] = Ac, Av
+        else:
+            c["Ac"] = DevArray.from_bits(Ac) if Ac.dtype == np.uint16 else DevArray.from_numpy(Ac, BF16)
+            c["Av"] = DevArray.from_bits(Av) if Av.dtype == np.uint16 else DevArray.from_numpy(Av, BF16)
         h = DevArray((B, n, D), BF16, zero=False)
         dv.gemm(c["X"], self.wb["Win"], h, B * n, D, F, bias=self.w["bin"], relu=1)
         c["h0"] = h
@@ -302,8 +334,9 @@ class MPScorer:
                                        self.step_no, self.dev.stream), "qsf_mp_adam")
 
     def scores(self, Hs, seeds=None):
-        """Column logits of each code (numpy, length n each)."""
-        X, Ac, Av, mask = pack_batch_bits(Hs, seeds)
+        """Column logits of each code (numpy, length n each). Unseeded, the inputs are made on
+        the device (Dev.pack, equal to pack_batch_bits); with seeds, on the host."""
+        X, Ac, Av, mask = self.dev.pack(Hs) if seeds is None else pack_batch_bits(Hs, seeds)
         s, _ = self.forward(X, Ac, Av)
         s = s.numpy()
         return [s[b, : H.shape[1]].copy() for b, H in enumerate(Hs)]

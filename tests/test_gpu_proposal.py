@@ -54,6 +54,33 @@ def test_gemm_matches_numpy(mp_dev, M, N, K, tA, tB, batch):
     assert np.allclose(gb, wb, rtol=2 ** -7, atol=1e-3)
 
 
+def test_device_packing_equals_host_packing(mp_dev):
+    """The scorer's inputs made on the device (qsf_mp_pack) equal the host packing
+    (pack_batch_bits, unseeded) bit for bit, on ragged codes with empty rows and columns,
+    a zero matrix and the 256-code random set; scores from either are the same."""
+    from qubitserf_amd.proposal.model import Dev, MPScorer, pack_batch_bits
+    rng = np.random.default_rng(8)
+    H_gap = codes.random_rate_code(rng, 30, 70).copy()
+    H_gap[:, 5] = 0
+    H_gap[3, :] = 0
+    sets = [codes.config3_batch(5, 12) + [np.eye(3, 7, dtype=np.uint8), np.zeros((2, 5), np.uint8), H_gap],
+            codes.config3_batch(0, 256), [codes.quasi_cyclic(rng, L) for L in (20, 33, 40)]]
+    dv = Dev()
+    for Hs in sets:
+        X, Ac, Av, mask = pack_batch_bits(Hs)
+        dX, dAc, dAv, dmask = dv.pack(Hs)
+        from qubitserf_amd.proposal.devarray import bf16_bits
+        assert np.array_equal(dmask, mask)
+        assert np.array_equal(dAc.bits(), Ac) and np.array_equal(dAv.bits(), Av)
+        assert np.array_equal(dX.bits(), bf16_bits(X))
+    Hs = sets[0]
+    sc = MPScorer.load(ROOT / "qubitserf_amd" / "proposal" / "weights.npz")
+    host = [np.asarray(x) for x in sc.forward(*pack_batch_bits(Hs)[:3])[0].numpy()]
+    dev = sc.scores(Hs)
+    for b, H in enumerate(Hs):
+        assert np.array_equal(dev[b], host[b][: H.shape[1]])
+
+
 def _scorer_inputs(seed=0, count=5):
     rng = np.random.default_rng(seed)
     Hs = [codes.random_rate_code(rng, 20, 60) for _ in range(count - 1)] + [codes.quasi_cyclic(rng, 7)]
