@@ -698,11 +698,11 @@ std::vector<long long> BatchSession::add(std::vector<BatchProblem>&& probs) {
 // stops where they would stop. A level with skipped tiles is not replayed: it runs again,
 // exactly, in the next round (rare: it needs a light codeword found at a higher level).
 // Results therefore equal the level-by-level loop's. Without lookahead (the default) a
-// round is one level per problem. Lookahead is off by default because each level's kernel
-// is chosen on the bound at the start of the round: on random codes, whose upper bound
-// keeps falling from level to level, the later levels are priced on a looser bound than
-// the one they meet when they run, and fall to the VALU test where one level per round
-// takes the matrix test (256 codes n ~ U{64..160}: 96.5 ms of kernels against 83.3).
+// round is one level per problem. Lookahead is off by default: its one decomposition of
+// every level runs before any launch, and the rounds' small ones cost less (256 codes
+// n ~ U{64..160}: 32.4 ms of kernels against 32.7, 50.8 ms wall against 46.6). Taking
+// only the cheap levels after each problem's next one (up to 2^20..2^26 codewords) cut
+// the rounds from 10 to 5..7 and made the batch 2-10% slower.
 int BatchSession::step() {
     Impl& I = *impl_;
     const char* la_env = std::getenv("QSA_LOOKAHEAD");
