@@ -3214,6 +3214,8 @@ struct Engine {
             out.reserve(cap * (u64)st);
             HIPCHK(hipMemsetAsync(cntp, 0, 4 * sizeof(unsigned long long), stream));
             double ms_total = 0;
+            // the tiled levels as the items of one run (a launch per variant, not per level)
+            std::vector<TItem> items;
             for (int d = 1; d <= p0.d; ++d) {
                 qsf_enum_plan p = p0;
                 p.d = d;
@@ -3222,10 +3224,7 @@ struct Engine {
                 if (tileable(s, p) && (policy == 2 || (policy == 0 && cw >= (1ull << 16)))) {
                     qsf_enum_plan pv = p;
                     pv.current_best = keep + 1;
-                    std::vector<TItem> items{TItem{&s, p, -1, choose_variant(s, pv, false)}};
-                    std::vector<int> res;
-                    run_tiled(items, true, keep, cap, res, cntp + 3);
-                    ms_total += stats.kernel_ms;
+                    items.push_back(TItem{&s, p, -1, choose_variant(s, pv, false)});
                 } else {
                     launch_sweep(s, p, true, keep, cap, bp, cntp);
                     HIPCHK(hipStreamSynchronize(stream));
@@ -3233,6 +3232,11 @@ struct Engine {
                     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
                     ms_total += ms;
                 }
+            }
+            if (!items.empty()) {
+                std::vector<int> res;
+                run_tiled(items, true, keep, cap, res, cntp + 3);
+                ms_total += stats.kernel_ms;
             }
             unsigned long long got = 0;
             HIPCHK(hipMemcpyAsync(&got, cntp + 3, sizeof(got), hipMemcpyDeviceToHost, stream));
