@@ -53,7 +53,7 @@ OPS_PER_CODEWORD = 5                                # 2 xor + 2 bcnt + 1 compare
 # Matrix-core hot tests by the trace's `mfma` field: kernel, ops per codeword (2 per
 # multiply-add: 32 int8 dimensions, or 64 / 128 fp4 ones), dense MI355X peak in TOP/s.
 MATRIX = {1: ("bz_mfma", 64, 5000.0), 2: ("bz_f4_32x32x64", 128, 10100.0), 3: ("bz_f4_16x16x128", 256, 10100.0),
-          4: ("bz_f4_2x32x32x64", 256, 10100.0)}
+          4: ("bz_f4_2x32x32x64", 256, 10100.0), 5: ("bz_f4_packed_32x32x64", 64, 10100.0)}
 TRAFFIC_FILES = {"bz_mfma": ROOT / "profiles" / "bz_mfma_gross_traffic.json",
                  "bz_tiled": ROOT / "profiles" / "bz_tiled_gross_traffic.json"}
 

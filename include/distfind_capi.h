@@ -93,7 +93,7 @@ typedef struct {
     int32_t kernel;        /* 0 none, 1 row sweep, 2 tiled, 3 small-level row sweep */
     int32_t fd, xd, rpl;   /* tiled kernel variant (hot-test dwords, held dwords, codewords/lane) */
     int32_t mfma;          /* matrix-core hot test: 0 none, 1 int8 32x32x32, 2 fp4 32x32x64, 3 fp4 16x16x128,
-                              4 two chained fp4 32x32x64 */
+                              4 two chained fp4 32x32x64, 5 packed fp4 32x32x64 */
     uint64_t codewords;    /* C(K, d) x active */
     uint64_t medium_hits;  /* stream steps past the hot-loop test */
     uint64_t slow_hits;    /* stream steps into the exact slow path */

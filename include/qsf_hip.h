@@ -231,7 +231,8 @@ typedef struct {
                                 a v_min3 per two codewords, 3 per stream step */
     int mfma;             /* tiled kernel: matrix-core hot test of the last launch: 0 none (VALU),
                              1 int8 32x32x32 products, 2 fp4 32x32x64, 3 fp4 16x16x128,
-                             4 two chained fp4 32x32x64 */
+                             4 two chained fp4 32x32x64, 5 packed fp4 32x32x64 (two lane
+                             elements per row, bounds <= 30) */
     double side_ms;       /* tiled kernel: of kernel_ms, the VALU launches that took a matrix-test
                              level's thin products (a list too short for the matrix tiles) */
     uint64_t side_codewords; /* tiled kernel: of codewords, those thin products' */

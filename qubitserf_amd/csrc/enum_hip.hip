@@ -774,6 +774,7 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
 //   MM = 2  v_mfma_scale_f32_32x32x64_f8f6f4   32 x 32 codewords per product, 64 dimensions
 //   MM = 3  v_mfma_scale_f32_16x16x128_f8f6f4  16 x 16 codewords per product, 128 dimensions
 //   MM = 4  two 32x32x64 products chained through the accumulator, 128 dimensions
+//   MM = 5  32x32x64 products over two lane elements per row (packed, 30 test bits)
 // gfx950 runs fp4 products at twice the int8 rate, so a product tests 60 (MM = 2) or 124
 // (MM = 3) residual bits for about the time the int8 test spends on 30. MM = 4 tests 124
 // bits on 32 x 32 outputs: twice the products of MM = 2 for the same reduction, with four
@@ -794,6 +795,16 @@ __global__ __launch_bounds__(256, 4) void bz_mfma(TiledArgs a) {
 // Padding: a lane element past its list holds only the constant 114 (digits 6, 6, 6, 6), a
 // stream element past its list only zeros with off_stream = 127; the host uses these
 // kernels only while the bound is at most 100, so neither can pass.
+// Packed (MM = 5): lanes 0..31 hold lane element (row, 0) and lanes 32..63 element (row, 1),
+// the second with block scale 2^8; both halves of B hold the stream element's 30 test bits.
+// Lane side -a_i per bit and -off_lane (clamped to the tile's threshold T, two digits
+// against the fixed (6, 1)), stream side 1 - 2 b_i; the accumulator starts at
+// 2^23 + 257 (127 + T - off_stream' - |b|), a per-column constant. Each f32 value is then
+// 2^23 + F0 + 256 F1 with F = 127 + T - (off_lane' + off_stream' + hd(a, b)) in [37, 157],
+// an exact integer whose bits are the value's low mantissa bits: bit 7 (15) is set exactly
+// when pair (row, 0) ((row, 1)) passes, so the reduction is an OR of the raw bits
+// (v_or3_b32). Padding elements hold no test bits and the constants that keep F <= 127.
+// The host runs it only at bounds <= kPackMaxT.
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
@@ -824,6 +835,15 @@ template <> struct F4Prod<3> {
 template <> struct F4Prod<4> : F4Prod<2> {
     static constexpr int NP = 2, NB = f4_blocks(4);
 };
+// Two lane elements per row: lanes 0..31 carry one, lanes 32..63 the other with block scale
+// 2^8 (E8M0 135), each lane's scale applying to the 32 dimensions it holds.
+template <> struct F4Prod<5> : F4Prod<2> {
+    __device__ static acc mms(i32x4 a, i32x4 b, acc c, int scale_a) {
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(i32x8{a[0], a[1], a[2], a[3], 0, 0, 0, 0},
+                                                               i32x8{b[0], b[1], b[2], b[3], 0, 0, 0, 0}, c,
+                                                               4, 4, 0, scale_a, 0, 127);
+    }
+};
 
 // fp4 e2m1 code of v in {-1, 0, 1, 2, 3, 4, 6}
 __device__ __forceinline__ u32 f4_code(int v) {
@@ -843,6 +863,19 @@ __device__ __forceinline__ u32 f4_lane_const(int L) {
     return f4_code(d0) | (f4_code(d1) << 4) | (f4_code(r - d1) << 8) | (f4_code(u) << 12);
 }
 
+// The packed test's lane constant, negated: -L (L clamped to [0, kPackMaxT], 29 taken as 28;
+// a smaller value only weakens the test) as -(6 d0 + d1), d0 in {0, .., 4} and d1 in
+// {-1, 0, .., 4, 6}: nibble 0 is the code of -d0, nibble 1 that of -d1.
+__device__ __forceinline__ u32 f4_neg_code(int v) { return v == 0 ? 0u : v > 0 ? 8u | f4_code(v) : f4_code(-v); }
+__device__ __forceinline__ u32 f4_pack_const(int L) {
+    L = max(0, min(L, kPackMaxT));
+    if (L == 29) L = 28;
+    int t = L / 6, u = L - 6 * t;
+    if (u == 5) { t += 1; u = -1; }
+    if (t == 5) { t = 4; u += 6; }
+    return f4_neg_code(t) | (f4_neg_code(u) << 4);
+}
+
 template <int XD, int MM>
 __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     typedef F4Prod<MM> P;
@@ -850,14 +883,16 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
     constexpr int NR = P::NR;                     // rows (lane elements) = columns (stream elements) per product
     constexpr int NKB = 64 / NR;                  // 32-dimension blocks across the lanes
     constexpr int NP = P::NP;                     // chained products per lane block
-    constexpr int NW = NKB * NP;                  // residual words tested
+    constexpr bool PK = MM == 5;                  // two lane elements per row, one per half
+    constexpr int NW = PK ? 1 : NKB * NP;         // residual words tested
     constexpr int NB = P::NB;                     // lane blocks per wave
     constexpr int NV = NR * NR / 64;              // product values per lane
-    constexpr int LW = NB * NR;                   // lane elements per wave
+    constexpr int LW = NB * NR * f4_per_row(MM);  // lane elements per wave
+    constexpr int NCB = (NB * NV * f4_per_row(MM) + 63) / 64;   // candidate words of a group
     constexpr int SL = f4_stream(MM);             // stream chunk length
     constexpr int EPT = SL / (64 * kMfmaWaves);   // stream elements each thread makes per chunk
     static_assert(XD >= NW, "the held residual dwords must cover the test bits");
-    static_assert(NB * NV <= 128, "candidate bits of a group");
+    static_assert(NB * NV * f4_per_row(MM) <= 256, "candidate bits of a group");
     __shared__ u32 rows_fast[kMaxMatrixK * XD];
     __shared__ i32x4 sbytes[NW * SL];             // word w of chunk position e at [w * SL + e]
     __shared__ float soff[SL];                    // off_stream of each chunk position (127: padding)
@@ -889,10 +924,16 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
             bad = bad || c != row || r < 0 || r >= NR;
             lay[lane * NV + v] = (unsigned short)((r & 31) | ((c & 31) << 8));
         }
+        if constexpr (PK) {   // each lane's block scale applies to the 32 dimensions it holds
+            const i32x4 one = {0x22222222, 0x22222222, 0x22222222, 0x22222222};
+            const acc ps = P::mms(one, one, acc{}, kb == 0 ? 127 : 135);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) bad = bad || ps[v] != 8224.0f;   // 32 + 256 * 32
+        }
         if (__ballot(bad) != 0ull && lane == 0) s_bad = 1;
     }
     __syncthreads();
-    if (s_bad) {   // another output layout: report it, enumerate nothing
+    if (s_bad) {   // another output layout or scale rule: report it, enumerate nothing
         if (tid == 0) atomicAdd(a.out_rows, 1ull);
         return;
     }
@@ -941,6 +982,13 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
             }
             if (J->shared != J->best) bound = min(bound, sb);   // the level loop's bound (as bz_tiled)
         }
+        // the packed test's tile threshold: it clamps the lane and stream constants to it,
+        // which keeps every 8-bit field in range; the host runs it only at bounds <= kPackMaxT
+        if (PK && bound > kPackMaxT) {
+            if (tid == 0) atomicAdd(a.out_rows, 1ull);
+            break;
+        }
+        const int ptile = PK ? bound : 0;
         const u64 tile_base = uni64(Rr.tile_base), stream_groups = uni64(Rr.stream_groups);
         const u64 stream_chunks = uni64(Rr.stream_chunks), n_lane = uni64(Rr.n_lane), n_stream = uni64(Rr.n_stream);
         const int fixed = uni(Rr.fixed);
@@ -970,7 +1018,8 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
         i32x4 av[NB][NP];
         u64* wlmk = lmk + LW * wave;
         {
-            const u64 first = bx * (u64)(kMfmaWaves * LW) + (u64)(LW * wave + NB * row);
+            const int lrow = PK ? 2 * row + kb : row;   // this lane's element row in the wave
+            const u64 first = bx * (u64)(kMfmaWaves * LW) + (u64)(LW * wave + NB * lrow);
             u64 mnext = (first < n_lane && lk) ? colex_unrank_any<u64>(first, lk, lm, sbin, a.binom, a.bk) : 0ull;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -978,7 +1027,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 const bool valid = idx < n_lane;
                 const u64 m = valid ? mnext : 0ull;
                 if (valid && lk && idx + 1 < n_lane) mnext = colex_next(mnext);
-                if (kb == 0) wlmk[NB * row + b] = m;
+                if (PK || kb == 0) wlmk[NB * lrow + b] = m;
                 u32 x[NW];
 #pragma unroll
                 for (int w = 0; w < NW; ++w) x[w] = 0u;
@@ -987,10 +1036,22 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
 #pragma unroll
                     for (int w = 0; w < NW; ++w) x[w] ^= rows_fast[r * XD + w];
                 }
-                x[NW - 1] &= 0x0fffffffu;
+                x[NW - 1] &= PK ? 0x3fffffffu : 0x0fffffffu;
                 int L = __popcll(m & lflags);
 #pragma unroll
                 for (int w = 0; w < NW; ++w) L += __popc(x[w]);
+                if constexpr (PK) {
+                    // -a_i per test bit (codes 0xA / 0x0), dimensions 30 and 31 the negated
+                    // off_lane (clamped to the tile threshold) against the stream's fixed (6, 1)
+                    u32 q[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) q[j] = valid ? ((x[0] >> j) & 0x11111111u) * 0xAu : 0u;
+                    const u32 c = f4_pack_const(valid ? min(__popcll(m & lflags), ptile) : kPackMaxT);
+                    q[2] = (q[2] & 0x0fffffffu) | ((c & 15u) << 28);
+                    q[3] = (q[3] & 0x0fffffffu) | ((c >> 4) << 28);
+                    av[b][0] = i32x4{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+                    continue;
+                }
 #pragma unroll
                 for (int p = 0; p < NP; ++p) {
                     u32 xk = x[p * NKB];
@@ -1044,16 +1105,21 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 if (sidx < n_stream) {
 #pragma unroll
                     for (int w = 0; w < NW; ++w) {
-                        const u32 sw = w == NW - 1 ? (sx[w] & 0x0fffffffu) : sx[w];
+                        const u32 sw = w == NW - 1 ? (sx[w] & (PK ? 0x3fffffffu : 0x0fffffffu)) : sx[w];
                         u32 q[4];
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) q[j] = ((sw >> j) & 0x11111111u) << 1;
-                        if (w == NW - 1) {   // the fixed values (6, 6, 6, 1) of the lane-constant dimensions
+                        for (int j = 0; j < 4; ++j)   // packed: 1 - 2 b_i (0x2 / 0xA), else b_i (0x2 / 0x0)
+                            q[j] = PK ? 0x22222222u ^ (((sw >> j) & 0x11111111u) << 3) : ((sw >> j) & 0x11111111u) << 1;
+                        if (w == NW - 1 && PK) {   // the fixed values (6, 1) of the lane-constant dimensions
+                            q[2] = (q[2] & 0x0fffffffu) | (7u << 28); q[3] = (q[3] & 0x0fffffffu) | (2u << 28);
+                        } else if (w == NW - 1) {   // the fixed values (6, 6, 6, 1) of the lane-constant dimensions
                             q[0] |= 7u << 28; q[1] |= 7u << 28; q[2] |= 7u << 28; q[3] |= 2u << 28;
                         }
                         sbytes[w * SL + pos] = i32x4{(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
                     }
-                    soff[pos] = (float)min(foff + __popcll(smask & sflags), 127);
+                    // packed: the column constant min(off_stream, tile threshold) + |b|
+                    soff[pos] = PK ? (float)(min(foff + __popcll(smask & sflags), ptile) + __popc(sx[0] & 0x3fffffffu))
+                                   : (float)min(foff + __popcll(smask & sflags), 127);
                     smk[pos] = smask;
                     if (sk && sidx + 1 < n_stream) {   // colex successor; XOR the rows that change
                         const u64 nm = colex_next(smask);
@@ -1068,16 +1134,18 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 } else {   // past the list: zeros, and a threshold no product reaches
 #pragma unroll
                     for (int w = 0; w < NW; ++w) sbytes[w * SL + pos] = i32x4{0, 0, 0, 0};
-                    soff[pos] = 127.0f;
+                    soff[pos] = PK ? (float)ptile : 127.0f;
                     smk[pos] = 0ull;
                 }
             }
             __syncthreads();
 
             float thrf = (float)min(bound, 127);
+            int tpk = min(bound, ptile);   // the packed test's threshold (as thrf)
+            const int sa = kb == 0 ? 127 : 135;
             // B operands and stream constant one group ahead (product p: word p NKB + kb; the
             // chained test loads its second product's operand in the group, for registers)
-            const i32x4* sb = sbytes + kb * SL + row;
+            const i32x4* sb = sbytes + (PK ? 0 : kb) * SL + row;
             i32x4 bnext = sb[0];
             float onext = soff[row];
             // lane block b's product with the group's B operands
@@ -1093,6 +1161,13 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
 #pragma unroll
                 for (int p = 1; p < NP; ++p) bv[p] = sb[p * NKB * SL + gi * NR];
                 const float thr = thrf - onext;   // this lane's column: L + sum < bound - off_stream
+                // packed: every value is 2^23 + F1 + 256 F2 with the fields
+                // F = 127 + tpk - off_stream' - |b| - off_lane' - sum_i a_i (1 - 2 b_i)
+                //   = 127 + tpk - (off_stream' + off_lane' + hd(a, b))   in [37, 157]
+                // (off' clamped to the tile threshold), so bit 7 of a field is set exactly
+                // when the element pair's weight on the test bits is below tpk
+                const int cx = (127 + tpk - (int)onext) * 257;
+                const float cf = __int_as_float(0x4B000000 + cx);
                 if (gi + 1 < gmax) {
                     bnext = sb[(gi + 1) * NR];
                     onext = soff[(gi + 1) * NR + row];
@@ -1102,7 +1177,26 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 // (only from a clamped lane constant) becomes a negative integer and sends the
                 // group to the exact tests, which compare in f32.
                 int mn = 0x7fffffff;
-                if constexpr (NV == 16) {
+                bool pass;
+                if constexpr (PK) {
+                    acc cv;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) cv[v] = cf;
+                    auto red = [](const acc& c) {
+                        int m = __float_as_int(c[0]) | __float_as_int(c[1]) | __float_as_int(c[2]);
+#pragma unroll
+                        for (int v = 3; v < 15; v += 2) m = m | __float_as_int(c[v]) | __float_as_int(c[v + 1]);
+                        return m | __float_as_int(c[15]);
+                    };
+                    int orr = 0;
+#pragma unroll
+                    for (int b0 = 0; b0 < NB; b0 += 2) {
+                        const acc p0 = P::mms(av[b0][0], bv[0], cv, sa);
+                        const acc p1 = P::mms(av[b0 + 1][0], bv[0], cv, sa);
+                        orr |= red(p0) | red(p1);
+                    }
+                    pass = __ballot((orr & 0x8080) != 0) != 0ull;
+                } else if constexpr (NV == 16) {
                     auto red = [](const acc& c) {
                         int m = min(min(__float_as_int(c[0]), __float_as_int(c[1])), __float_as_int(c[2]));
 #pragma unroll
@@ -1136,14 +1230,15 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                                  min(__float_as_int(p[b][2]), __float_as_int(p[b][3])));
                 }
                 // thr < 0 (a padded column) has a negative pattern below every non-negative one
-                if (__builtin_expect(__ballot(mn < __float_as_int(thr)) == 0ull, 1)) continue;
+                if constexpr (!PK) pass = __ballot(mn < __float_as_int(thr)) != 0ull;
+                if (__builtin_expect(!pass, 1)) continue;
                 // ---- exact tests of this group: the products are formed again, each lane
                 // marks its values below its threshold (bit NV b + v), then tests those
                 // (lane element, stream element) pairs; both elements are then real
                 ++medium_entries;
                 ++job_hits;
                 int mine = QSF_WEIGHT_NONE;
-                u64 cand[2] = {0ull, 0ull};
+                u64 cand[NCB] = {};
                 // an opaque copy of the B operands: the products are formed again here instead
                 // of all eight being kept live from the test above (which would spill)
                 i32x4 bx[NP];
@@ -1152,8 +1247,24 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                     bx[p] = bv[p];
                     asm volatile("" : "+v"(bx[p]));
                 }
+                if constexpr (PK)
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
+                    for (int b = 0; b < NB; ++b) {   // packed: bit 7 of field s marks element (row, s)
+                    // the products without the bias (exact integers in f32), the fields added back
+                    const acc p = P::mms(av[b][0], bx[0], acc{}, sa);
+                    int o = 0;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) o |= (int)p[v] + cx;
+                    if (__ballot((o & 0x8080) != 0) == 0ull) continue;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const int q = 2 * (NV * b + v), bits = (int)p[v] + cx;
+                        if (bits & 0x80) cand[q >> 6] |= 1ull << (q & 63);
+                        if (bits & 0x8000) cand[(q + 1) >> 6] |= 1ull << ((q + 1) & 63);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < NB && !PK; ++b) {
                     const acc p = prod(b, bx);
                     float pm = p[0];
 #pragma unroll
@@ -1165,12 +1276,13 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                         if (p[v] < thr) cand[q >> 6] |= 1ull << (q & 63);
                     }
                 }
-                for (int part = 0; part < 2; ++part) {
-                    for (u64 cm = part ? cand[1] : cand[0]; cm; cm &= cm - 1) {
-                        const int q = 64 * part + __builtin_ctzll(cm);
+                for (int part = 0; part < NCB; ++part) {
+                    for (u64 cm = cand[part]; cm; cm &= cm - 1) {
+                        const int q = (64 * part + __builtin_ctzll(cm)) >> (PK ? 1 : 0);
+                        const int half = PK ? (__builtin_ctzll(cm) & 1) : 0;
                         const int b = q / NV;
                         const u32 ij = lay[lane * NV + (q % NV)];
-                        const u64 lmask = wlmk[NB * (ij & 31) + b];
+                        const u64 lmask = wlmk[NB * (PK ? 2 * (int)(ij & 31) + half : (int)(ij & 31)) + b];
                         const int pos = gi * NR + (int)(ij >> 8);
                         const u64 sm_ = smk[pos];
                         const int base = __popcll(lmask & lflags) + foff + __popcll(sm_ & sflags);
@@ -1219,6 +1331,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                     if (lane == 0) atomicMin(J->best, wm);
                     if (lane == 0 && J->shared != J->best) atomicMin(J->shared, wm);
                     thrf = (float)min(bound, 127);
+                    tpk = min(bound, ptile);
                 }
                 if (bound <= floor) break;
             }
@@ -1494,7 +1607,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::list<std::unique_ptr<Solve>> cache;   // most recent first
-    static constexpr int kVariants = 23;  // kernel variants of a tiled run (vtab in run_tiled)
+    static constexpr int kVariants = 24;  // kernel variants of a tiled run (vtab in run_tiled)
     u64* gbinom = nullptr;               // device address of tiled_binom (filled at probe)
     unsigned char* scratch_dev = nullptr;   // device address of scratch (no-allocation calls)
     DevBuf<unsigned char> blk;           // one tiled run's uploaded block (see run_tiled)
@@ -2306,9 +2419,11 @@ struct Engine {
     // for the chained fp4 test; the VALU test stream steps of 64 x RPL codewords) and its
     // test bits (30 for int8, 60 for fp4 32x32x64, 124 for the 128-dimension fp4 tests,
     // 32 FD for VALU).
-    static double test_unit(const Variant& V) { return V.mm == 4 ? 4096.0 : V.mm > 0 ? 8192.0 : 64.0 * V.rpl; }
+    static double test_unit(const Variant& V) {
+        return V.mm == 5 ? 16384.0 : V.mm == 4 ? 4096.0 : V.mm > 0 ? 8192.0 : 64.0 * V.rpl;
+    }
     static int test_bits(const Variant& V, int res) {
-        return std::min(V.mm == 1 ? 30 : V.mm == 2 ? 60 : V.mm >= 3 ? 124 : 32 * V.fd, res);
+        return std::min(V.mm == 1 || V.mm == 5 ? 30 : V.mm == 2 ? 60 : V.mm >= 3 ? 124 : 32 * V.fd, res);
     }
     // Hot-test passes on a level of cw codewords with threshold thr if the residual bits
     // were fair coins (per-codeword pass probability p = P(Binomial(bits, 1/2) < thr)).
@@ -2334,6 +2449,7 @@ struct Engine {
         const double hits = std::max(fair_hits(V, cw, p) * f, (double)cw * std::min(1.0, p * f));
         if (V.mm == 1) return tuning::kMatrixLaunch + (double)cw * tuning::kInt8Cost + hits * tuning::kInt8Hit;
         if (V.mm == 4) return tuning::kMatrixLaunch + (double)cw * tuning::kF4x2Cost + hits * tuning::kF4x2Hit;
+        if (V.mm == 5) return tuning::kMatrixLaunch + (double)cw * tuning::kF4PkCost + hits * tuning::kF4PkHit;
         if (V.mm >= 2) return tuning::kMatrixLaunch + (double)cw * tuning::kF4Cost + hits * tuning::kF4Hit;
         const int k = valu_index(V);
         return tuning::kValuLaunch + (double)cw * tuning::kValuCost[k] + hits * tuning::kValuHit[k];
@@ -2360,7 +2476,8 @@ struct Engine {
     static Variant choose_variant_k128(const Solve& s, const qsf_enum_plan& p, bool allow_mfma, u64 min_work) {
         static const Variant table[] = {{1, 1, 32, 0}, {1, 2, 16, 0}, {2, 2, 16, 0}, {2, 4, 12, 0}, {3, 4, 12, 0},
                                         {4, 4, 12, 0}, {1, 2, 24, 0}, {1, 2, 32, 0}, {0, 1, 4, 1},  {0, 2, 4, 1},
-                                        {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3},  {0, 4, 4, 4}};
+                                        {0, 4, 4, 1},  {0, 2, 4, 2},  {0, 4, 4, 2},  {0, 4, 2, 3},  {0, 4, 4, 4},
+                                        {0, 4, 4, 5}};
         // the matrix tests compare their products with min(bound, 127); their padding
         // elements stay above any bound <= 100
         allow_mfma = allow_mfma && p.current_best <= 100;
@@ -2370,7 +2487,8 @@ struct Engine {
             if (got >= 3)
                 for (const Variant& v : table)
                     if (v.fd == f && v.xd == x && v.rpl == r && (got == 3 || v.mm == m) && (x <= s.RD || x == 4) &&
-                        (v.mm == 0 || allow_mfma) && (v.mm < 2 || s.RD >= (v.mm == 2 ? 2 : 3)))
+                        (v.mm == 0 || allow_mfma) && (v.mm < 2 || v.mm == 5 || s.RD >= (v.mm == 2 ? 2 : 3)) &&
+                        (v.mm != 5 || p.current_best <= kPackMaxT))
                         return v;
         }
         if (const char* env = std::getenv("QSA_TILED_FD")) {   // tuning/testing override
@@ -2409,6 +2527,9 @@ struct Engine {
             consider(s.RD >= 3 ? table[10] : s.RD == 2 ? table[9] : table[8]);
             if (!std::getenv("QSA_NO_F4") && s.RD >= 2) consider(s.RD >= 3 ? table[12] : table[11]);
             if (!std::getenv("QSA_NO_F4") && s.RD >= 3) consider(table[14]);
+            // the packed fp4 test: 30 test bits on two lane elements per product row (up to
+            // bound kPackMaxT, where its 8-bit fields hold every value)
+            if (!std::getenv("QSA_NO_F4") && p.current_best <= kPackMaxT) consider(table[15]);
         }
         return best;
     }
@@ -2656,7 +2777,7 @@ struct Engine {
             {1, 1, 32, 0},    {1, 2, 16, 0},    {1, 2, 24, 0},    {1, 2, 32, 0},    {2, 2, 16, 0},    {2, 4, 12, 0},
             {3, 4, 12, 0},    {4, 4, 12, 0},    {0, 1, 4, 1},     {0, 2, 4, 1},     {0, 4, 4, 1},     {0, 2, 4, 2},
             {0, 4, 4, 2},     {0, 4, 2, 3},     {1, 1, 32, 0, 2}, {1, 2, 16, 0, 2}, {1, 2, 24, 0, 2}, {1, 2, 32, 0, 2},
-            {2, 2, 16, 0, 2}, {2, 4, 12, 0, 2}, {3, 4, 12, 0, 2}, {4, 4, 12, 0, 2}, {0, 4, 4, 4}};
+            {2, 2, 16, 0, 2}, {2, 4, 12, 0, 2}, {3, 4, 12, 0, 2}, {4, 4, 12, 0, 2}, {0, 4, 4, 4}, {0, 4, 4, 5}};
         auto vidx = [&](const Variant& v) {
             for (int k = 0; k < NV; ++k)
                 if (vtab[k].fd == v.fd && vtab[k].xd == v.xd && vtab[k].rpl == v.rpl && vtab[k].mm == v.mm &&
@@ -3075,7 +3196,8 @@ struct Engine {
                 case 19: hipLaunchKernelGGL((bz_tiled<2, 4, 12, 2>), gd, bd, lds, stream, a); break;
                 case 20: hipLaunchKernelGGL((bz_tiled<3, 4, 12, 2>), gd, bd, lds, stream, a); break;
                 case 21: hipLaunchKernelGGL((bz_tiled<4, 4, 12, 2>), gd, bd, lds, stream, a); break;
-                default: hipLaunchKernelGGL((bz_f4<4, 4>), gd, bd, lds, stream, a); break;
+                case 22: hipLaunchKernelGGL((bz_f4<4, 4>), gd, bd, lds, stream, a); break;
+                default: hipLaunchKernelGGL((bz_f4<4, 5>), gd, bd, lds, stream, a); break;
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(gev_dyn[2 * gi + 1], stream));
@@ -3156,7 +3278,8 @@ struct Engine {
             stats.medium_hits += hc[kCntStride * gi + 2];
             if (vtab[grp_v[gi]].mm >= 2) {   // the fp4 kernels count launches that met another output layout here
                 if (hc[kCntStride * gi + 3] != 0)
-                    throw HipError{3, "fp4 matrix products: output layout differs from the one the hot test assumes"};
+                    throw HipError{3, "fp4 matrix products: output layout or block scales differ from the ones the hot "
+                                      "test assumes, or the packed test met a bound above its limit"};
             } else {
                 out_rows_last += hc[kCntStride * gi + 3];
             }

@@ -45,12 +45,15 @@ constexpr int kMfmaBlocks = 8;      // bz_mfma: 32-element lane blocks per wave
 constexpr int kMfmaWaves = 4;       // bz_mfma: waves per workgroup, sharing each stream chunk
 constexpr int kMfmaLane = 32 * kMfmaBlocks * kMfmaWaves;   // bz_mfma: lane elements per tile
 constexpr int kMfmaStream = 512;    // bz_mfma: stream chunk length (two elements per thread)
-// bz_f4<XD, MM>: 32 x 32 products (MM = 2), 16 x 16 products (MM = 3), or two chained
-// 32 x 32 products (MM = 4) per lane block
+// bz_f4<XD, MM>: 32 x 32 products (MM = 2), 16 x 16 products (MM = 3), two chained
+// 32 x 32 products (MM = 4), or 32 x 32 products over two lane elements per row (MM = 5)
+// per lane block
 constexpr int f4_rows(int mm) { return mm == 3 ? 16 : 32; }
 constexpr int f4_blocks(int mm) { return mm == 4 ? 4 : kMfmaBlocks; }                  // lane blocks per wave
-constexpr int f4_lane_tile(int mm) { return f4_rows(mm) * f4_blocks(mm) * kMfmaWaves; }   // lane elements per tile
-constexpr int f4_stream(int mm) { return mm == 2 ? 512 : 256; }                         // stream chunk length
+constexpr int f4_per_row(int mm) { return mm == 5 ? 2 : 1; }                           // lane elements per product row
+constexpr int f4_lane_tile(int mm) { return f4_rows(mm) * f4_blocks(mm) * f4_per_row(mm) * kMfmaWaves; }   // lane elements per tile
+constexpr int f4_stream(int mm) { return mm == 2 || mm == 5 ? 512 : 256; }             // stream chunk length
+constexpr int kPackMaxT = 30;       // bz_f4<xd, 5>: the largest bound the packed test runs at
 constexpr int kMaxRW = 64;          // residual + syndrome dwords per row in the tiled kernel
 constexpr int kLdsFullRows = 1024;  // dwords: the whole K x RW table is kept in LDS up to this size
 constexpr int kBigOffset = (1 << 30) + (1 << 29);  // masks lanes past the end of a list

@@ -54,6 +54,8 @@ def unit_and_bits(v):
         return 8192.0, 60
     if mm == 4:
         return 4096.0, 124
+    if mm == 5:
+        return 16384.0, 30
     return 64.0 * rpl, 32 * fd
 
 
@@ -146,7 +148,7 @@ def measure(log=print):
     run = Runner(systematic(rng, K, stride), n)
     cw = math.comb(K, d)
     tight = d + 2
-    variants = [(f, x, r, 0) for f, x, r in VALU] + [(0, 4, 4, 1), (0, 4, 4, 2), (0, 4, 4, 4)]
+    variants = [(f, x, r, 0) for f, x, r in VALU] + [(0, 4, 4, 1), (0, 4, 4, 2), (0, 4, 4, 4), (0, 4, 4, 5)]
     out = {"level": f"K={K} n={n} d={d} random systematic, {cw:.3e} codewords", "variants": {}}
     small_d = 4
     small_cw = math.comb(K, small_d)
@@ -171,7 +173,7 @@ def measure(log=print):
 def header(m, device):
     V = m["variants"]
     valu = [V[f"{f},{x},{r},0"] for f, x, r in VALU]
-    i8, f4, f4x2 = V["0,4,4,1"], V["0,4,4,2"], V["0,4,4,4"]
+    i8, f4, f4x2, f4pk = V["0,4,4,1"], V["0,4,4,2"], V["0,4,4,4"], V["0,4,4,5"]
     hits = [r["hit"] for r in valu if r["hit"]]
     valu_hit_default = sorted(hits)[len(hits) // 2] if hits else 1.5e-9
 
@@ -179,7 +181,7 @@ def header(m, device):
         return f"{x:.3e}"
 
     launch_valu = sorted(r["launch"] for r in valu)[len(valu) // 2]
-    launch_mm = max(i8["launch"], f4["launch"], f4x2["launch"])
+    launch_mm = max(i8["launch"], f4["launch"], f4x2["launch"], f4pk["launch"])
     lines = [
         "// Kernel-choice constants of the enumeration engine, measured on an MI355X.",
         "// Regenerate with:  python -m qubitserf_amd.tune --write   (on a gfx950 device)",
@@ -204,6 +206,10 @@ def header(m, device):
         "// Two chained fp4 32x32x64 products (124 test bits): per codeword, and per group of 4096.",
         f"constexpr double kF4x2Cost = {fmt(f4x2['cost'])};",
         f"constexpr double kF4x2Hit = {fmt(f4x2['hit'] or f4['hit'] or 1.8e-10)};",
+        "// Packed fp4 32x32x64 products (30 test bits, two lane elements per row): per codeword, and",
+        "// per group of 16384.",
+        f"constexpr double kF4PkCost = {fmt(f4pk['cost'])};",
+        f"constexpr double kF4PkHit = {fmt(f4pk['hit'] or i8['hit'] or 1.8e-10)};",
         "// Fixed seconds per level launch (set-up, tail) of the VALU and matrix-core kernels.",
         f"constexpr double kValuLaunch = {fmt(launch_valu)};",
         f"constexpr double kMatrixLaunch = {fmt(launch_mm)};",

@@ -17,13 +17,13 @@ def _gross_z():
 
 
 def test_gross_levels_choice():
-    """Tight bound 12 on K = 78: the largest levels (2^36+ codewords) take the int8 test on
-    a fair-coin prior; a structure factor as measured on this code (7 at the d = 8 level's
-    30-bit pass probability) moves d = 8 to the 60-bit fp4 test; the small levels stay on the
-    VALU kernel."""
+    """Tight bound 12 on K = 78: the largest levels (2^36+ codewords) take the packed fp4
+    test (30 bits, cheaper per codeword than int8) on a fair-coin prior; a structure factor
+    as measured on this code (7 at the d = 8 level's 30-bit pass probability) moves d = 8 to
+    the 60-bit fp4 test; the small levels stay on the VALU kernel."""
     gamma, check = _gross_z()
     pick = {d: engine.plan_variant(gamma, 144, d, check=check, num_gamma=1, current_best=12) for d in range(3, 11)}
-    assert pick[10][3] == 1 and pick[9][3] == 1
+    assert pick[10][3] == 5 and pick[9][3] == 5, pick
     assert all(pick[d][3] == 0 for d in (3, 4, 5, 6)), pick
     structured = engine.plan_variant(gamma, 144, 8, check=check, num_gamma=1, current_best=12, factor=7.0,
                                      at_p=cdf_below(30, 4))
@@ -31,15 +31,16 @@ def test_gross_levels_choice():
 
 
 def test_more_structure_never_returns_to_the_narrower_test():
-    """At a fixed level, raising the structure factor moves the choice from int8 (30 bits)
-    to fp4 (60 bits), never back."""
+    """At a fixed level, raising the structure factor moves the choice from a 30-bit test
+    (int8 or packed fp4) to fp4 (60 bits), never back."""
     gamma, check = _gross_z()
     order = []
     for f in (1, 2, 4, 8, 16, 64, 256):
         v = engine.plan_variant(gamma, 144, 8, check=check, num_gamma=1, current_best=12, factor=f,
                                 at_p=cdf_below(30, 4))
         order.append(v[3])
-    assert order == sorted(order) and order[0] == 1 and order[-1] == 2, order
+    bits = [{1: 30, 5: 30, 2: 60, 4: 124}[m] for m in order]
+    assert bits == sorted(bits) and order[0] in (1, 5) and order[-1] == 2, order
 
 
 def test_loose_bounds_stay_on_valu():
@@ -53,7 +54,7 @@ def test_loose_bounds_stay_on_valu():
     v = engine.plan_variant(gamma, 192, 4, current_best=40)
     assert v[3] == 0 and v[0] >= 3, v
     tight = engine.plan_variant(gamma, 192, 7, current_best=10)
-    assert tight[3] == 1, tight
+    assert tight[3] in (1, 5), tight   # a 30-bit matrix test
 
 
 def test_choice_never_depends_on_device():
