@@ -55,6 +55,7 @@ OPS_PER_CODEWORD = 5                                # 2 xor + 2 bcnt + 1 compare
 MATRIX = {1: ("bz_mfma", 64, 5000.0), 2: ("bz_f4_32x32x64", 128, 10100.0), 3: ("bz_f4_16x16x128", 256, 10100.0),
           4: ("bz_f4_2x32x32x64", 256, 10100.0), 5: ("bz_f4_packed_32x32x64", 64, 10100.0)}
 TRAFFIC_FILES = {"bz_mfma": ROOT / "profiles" / "bz_mfma_gross_traffic.json",
+                 "bz_f4_packed_32x32x64": ROOT / "profiles" / "bz_f4_packed_gross_traffic.json",
                  "bz_tiled": ROOT / "profiles" / "bz_tiled_gross_traffic.json"}
 
 

@@ -2529,7 +2529,9 @@ struct Engine {
             if (!std::getenv("QSA_NO_F4") && s.RD >= 3) consider(table[14]);
             // the packed fp4 test: 30 test bits on two lane elements per product row (up to
             // bound kPackMaxT, where its 8-bit fields hold every value)
-            if (!std::getenv("QSA_NO_F4") && p.current_best <= kPackMaxT) consider(table[15]);
+            // (QSA_NO_PACK=1 leaves it out: the int8 test's figures on the same levels)
+            if (!std::getenv("QSA_NO_F4") && !std::getenv("QSA_NO_PACK") && p.current_best <= kPackMaxT)
+                consider(table[15]);
         }
         return best;
     }
