@@ -1178,6 +1178,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 // group to the exact tests, which compare in f32.
                 int mn = 0x7fffffff;
                 bool pass;
+                int pk_pair = 0;
                 if constexpr (PK) {
                     acc cv;
 #pragma unroll
@@ -1193,7 +1194,9 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                     for (int b0 = 0; b0 < NB; b0 += 2) {
                         const acc p0 = P::mms(av[b0][0], bv[0], cv, sa);
                         const acc p1 = P::mms(av[b0 + 1][0], bv[0], cv, sa);
-                        orr |= red(p0) | red(p1);
+                        const int o = red(p0) | red(p1);
+                        pk_pair |= ((o & 0x8080) != 0 ? 1 : 0) << (b0 / 2);   // pairs of blocks that pass
+                        orr |= o;
                     }
                     pass = __ballot((orr & 0x8080) != 0) != 0ull;
                 } else if constexpr (NV == 16) {
@@ -1250,6 +1253,7 @@ __global__ __launch_bounds__(256, 4) void bz_f4(TiledArgs a) {
                 if constexpr (PK)
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {   // packed: bit 7 of field s marks element (row, s)
+                    if (__ballot((pk_pair >> (b / 2)) & 1) == 0ull) continue;   // only the passing pairs
                     // the products without the bias (exact integers in f32), the fields added back
                     const acc p = P::mms(av[b][0], bx[0], acc{}, sa);
                     int o = 0;
